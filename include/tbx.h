@@ -1,0 +1,181 @@
+/*
+ * tbx.h -- C-ABI of the MI355X (gfx950) GFN2-xTB integral / zeroth-order
+ * Hamiltonian path.  Plain pointers and sizes only; every array keeps the
+ * Fortran column-major layout tblite uses, with 0-based indices.
+ *
+ * The entry points replace tblite's Fortran module procedures
+ *   get_hamiltonian           src/tblite/xtb/h0.f90:220-429
+ *   get_hamiltonian_gradient  src/tblite/xtb/h0.f90:432-666
+ *   get_selfenergy            src/tblite/xtb/h0.f90:113-217
+ * which have no C binding in the reference (they are called directly by
+ * xtb_singlepoint, src/tblite/xtb/singlepoint.f90:228,241,370).  The
+ * conventions follow tblite's own C API (the headers in include/tblite/): an opaque
+ * context, int status returns (0 = success) and a retrievable message.
+ *
+ * Units: bohr and hartree.  Angular momenta l <= 2 (GFN2, xtb/gfn2.f90:928);
+ * at most TBX_MAXSH shells per species and TBX_MAXG primitives per shell.
+ */
+#ifndef TBX_H
+#define TBX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBX_MAXG 12  /* basis/type.f90:32 */
+#define TBX_MAXSH 4  /* shells per species */
+#define TBX_MAXL 2
+
+/* status codes */
+#define TBX_OK 0
+#define TBX_ERR_ARG 1      /* invalid argument (null pointer, size, NaN) */
+#define TBX_ERR_BASIS 2    /* unsupported basis (l > 2, too many shells/primitives) */
+#define TBX_ERR_DEVICE 3   /* HIP runtime error */
+#define TBX_ERR_ALLOC 4    /* device allocation failed */
+
+typedef struct tbx_context tbx_context;
+
+/* Species-resolved contracted Gaussian basis (basis_type, basis/type.f90:58-100).
+ * Shell ish (0-based) of species isp lives in slot ish + msh*isp. */
+typedef struct {
+    int nid;              /* number of species */
+    int msh;              /* leading dimension of the per-shell tables (<= TBX_MAXSH) */
+    const int *nsh_id;    /* [nid]            shells per species */
+    const int *ang;       /* [msh*nid]        angular momentum */
+    const int *nprim;     /* [msh*nid]        primitives */
+    const double *alpha;  /* [TBX_MAXG*msh*nid] exponents */
+    const double *coeff;  /* [TBX_MAXG*msh*nid] contraction coefficients incl. normalisation */
+    double intcut;        /* primitive screening threshold (25 for every accuracy) */
+} tbx_basis;
+
+/* Zeroth-order Hamiltonian parameters (tb_hamiltonian, xtb/h0.f90:39-67). */
+typedef struct {
+    const double *hscale; /* [msh*msh*nid*nid] hscale(jsh, ish, jsp, isp) */
+    const double *shpoly; /* [msh*nid]         shpoly(ish, isp) */
+    const double *rad;    /* [nid]             atomic radii (bohr) */
+} tbx_h0par;
+
+/* Structure and lattice images (structure_type + trans, cutoff.f90:97-121). */
+typedef struct {
+    int nat;
+    const int *id;        /* [nat]      0-based species */
+    const double *xyz;    /* [3*nat]    xyz(3, nat) */
+    int ntrans;
+    const double *trans;  /* [3*ntrans] translation vectors; one zero vector for molecules */
+    double cutoff;        /* real-space cutoff, get_cutoff(bas) (basis/type.f90:227-246) */
+} tbx_geom;
+
+/* Half neighbour list in CSR form (adjacency_list, adjlist.f90:44-56), 0-based. */
+typedef struct {
+    const int *inl;   /* [nat] offset */
+    const int *nnl;   /* [nat] count */
+    int nnbr;         /* total entries */
+    const int *nlat;  /* [nnbr] neighbouring atom (jat <= iat) */
+    const int *nltr;  /* [nnbr] translation index */
+} tbx_list;
+
+/* --- context ----------------------------------------------------------- */
+
+/* Create a context on HIP device `device`.  Returns NULL on failure. */
+tbx_context *tbx_context_new(int device);
+void tbx_context_delete(tbx_context *ctx);
+/* Message of the last failing call on this thread ("" if none). */
+const char *tbx_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int tbx_version(void);
+
+/* --- host-pointer entry points (drop-in for the Fortran procedures) ------ */
+
+/* get_selfenergy (xtb/h0.f90:113-217), CN branch: se = selfenergy_sp - kcn*cn,
+ * dsedcn = -kcn.  selfenergy_sp, kcn: [msh*nid]; cn may be NULL (no shift);
+ * dsedcn may be NULL.  Host-side, no device work. */
+int tbx_get_selfenergy(int nat, const int *id, const int *ish_at, const int *nsh_id, int msh,
+                       const double *selfenergy_sp, const double *kcn, const double *cn,
+                       double *selfenergy, double *dsedcn);
+
+/* get_hamiltonian (xtb/h0.f90:220-429): dense S(nao,nao), D(3,nao,nao),
+ * Q(6,nao,nao) (traceless, operator on the column function) and H0(nao,nao),
+ * all intent(out).  `list` may be NULL: neighbours are the pairs within
+ * geom->cutoff, the rule new_adjacency_list applies (adjlist.f90:125-129).
+ * iao_sh/ish_at: [nsh]/[nat] AO and shell offsets; selfenergy: [nsh]. */
+int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const tbx_list *list,
+                        const tbx_basis *bas, const tbx_h0par *h0, const int *ish_at,
+                        const int *iao_sh, int nao, const double *selfenergy, double *overlap,
+                        double *dpint, double *qpint, double *hamiltonian);
+
+/* get_hamiltonian_gradient (xtb/h0.f90:432-666).  pmat, xmat: (nao,nao,nspin)
+ * (magnet representation, wavefunction/spin.f90:104-153); vao: (nao,nspin);
+ * vdp: (3,nat,nspin); vqp: (6,nat,nspin).  dEdcn(nat), gradient(3,nat) and
+ * sigma(3,3) are ACCUMULATED into, as the reference's intent(inout). */
+int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *geom, const tbx_list *list,
+                                 const tbx_basis *bas, const tbx_h0par *h0, const int *ish_at,
+                                 const int *iao_sh, int nao, const double *selfenergy,
+                                 const double *dsedcn, int nspin, const double *vao,
+                                 const double *vdp, const double *vqp, const double *pmat,
+                                 const double *xmat, double *dEdcn, double *gradient,
+                                 double *sigma);
+
+/* --- device-resident batched path ---------------------------------------- */
+
+/* A batch: nmol independent molecules sharing one species table.  Built once
+ * (host tables uploaded), then run many times on device-resident inputs. */
+typedef struct tbx_batch tbx_batch;
+
+typedef struct {
+    int nmol;
+    const int *mol_nat;   /* [nmol] atoms per molecule */
+    const int *id;        /* [sum nat] species, 0-based, into the basis tables */
+    const tbx_basis *bas;
+    const tbx_h0par *h0;
+    double cutoff;        /* neighbour cutoff (bohr) */
+} tbx_batch_desc;
+
+/* Sizes of a batch: total atoms, shells, AOs and dense elements sum(nao^2). */
+typedef struct {
+    int nmol, nat, nsh, nao;
+    long long ndense;
+    long long npair; /* reserved, 0 */
+} tbx_batch_sizes;
+
+int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tbx_batch **out);
+void tbx_batch_delete(tbx_batch *b);
+int tbx_batch_sizes_get(const tbx_batch *b, tbx_batch_sizes *sizes);
+/* Offsets into the packed per-molecule arrays: atom_off[nmol+1], sh_off[nmol+1],
+ * ao_off[nmol+1], dense_off[nmol+1] (elements of S/H; x3 for D, x6 for Q). */
+int tbx_batch_offsets(const tbx_batch *b, int *atom_off, int *sh_off, int *ao_off,
+                      long long *dense_off);
+
+/* Forward: dense S, D, Q, H per molecule into device buffers of
+ * ndense, 3*ndense, 6*ndense, ndense doubles.  d_xyz: [3*nat], d_se: [nsh],
+ * device pointers.  Neighbours are the pairs within desc->cutoff, decided on
+ * the device from d_xyz (no host neighbour list).  `stream` may be NULL (the
+ * context's stream). */
+int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double *d_se, double *d_S,
+                      double *d_D, double *d_Q, double *d_H, void *stream);
+
+/* Plain gradient contraction for every molecule (nspin = 1): d_P, d_W dense
+ * per molecule (ndense each), d_vao [nao], d_vdp [3*nat], d_vqp [6*nat];
+ * d_dsedcn [nsh].  d_dEdcn [nat], d_grad [3*nat] and d_sigma [9*nmol] are
+ * OVERWRITTEN. */
+int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
+                       const double *d_dsedcn, const double *d_P, const double *d_W,
+                       const double *d_vao, const double *d_vdp, const double *d_vqp,
+                       double *d_dEdcn, double *d_grad, double *d_sigma, void *stream);
+
+/* Adjoint (backward) of tbx_batch_forward: given dense adjoints d_Sbar,
+ * d_Hbar (ndense), d_Dbar (3*ndense), d_Qbar (6*ndense) of the forward
+ * outputs, write dE/dxyz [3*nat] and dE/dse [nsh] (OVERWRITTEN).  The
+ * integrals are recomputed, no derivative tensors are stored. */
+int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
+                      const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
+                      const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream);
+
+/* Kernel timing of the last batched call on the context's internal events:
+ * milliseconds of the forward / pair-list / gradient / adjoint kernels. */
+int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pairs,
+                          double *ms_gradient, double *ms_adjoint);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TBX_H */

@@ -1,0 +1,704 @@
+// tbx_capi.hip -- host implementation of the C-ABI declared in include/tbx.h.
+//
+// The host-pointer entry points (tbx_get_hamiltonian, tbx_get_hamiltonian_
+// gradient) wrap a one-molecule batch: they upload the caller's arrays into
+// device buffers cached on the context, launch the same kernels as the
+// batched path on the context's stream, and copy the results back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/tbx.h"
+#include "tbx_tables.h"
+
+extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
+                                         const double *xyz, const double *se, const int *order,
+                                         double *S, double *D, double *Q, double *H,
+                                         const int *nnl, int *flag, hipStream_t stream);
+extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const TbxMols *mol,
+                                          const double *xyz, const double *se, const int *order,
+                                          const double *P, const double *W, const double *vao,
+                                          const double *vdp, const double *vqp, int nspin,
+                                          long long pstride, long long vstride,
+                                          const double *Sb, const double *Db, const double *Qb,
+                                          const double *Hb, double *grad, double *dse,
+                                          double *sigma, const int *nnl, int *flag,
+                                          hipStream_t stream);
+extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
+                                       const double *dse, const double *dsedcn, const double *P,
+                                       double *dEdcn, hipStream_t stream);
+extern "C" hipError_t tbx_launch_fill(double *p, long long n, double v, hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? TBX_ERR_ALLOC : TBX_ERR_DEVICE,     \
+                        "%s: %s", #expr, hipGetErrorString(e_));                        \
+    } while (0)
+
+// Device buffer that grows on demand and is freed with its owner.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+template <class T>
+hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s)
+{
+    hipError_t e = b.reserve(sizeof(T) * std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// host-side tables
+
+struct HostSpecies {
+    int nsp = 0, msh = 0, maxnc = 0;
+    std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao;
+    std::vector<double> alpha, coeff, shpoly, rad, hscale;
+    double intcut = 25.0;
+};
+
+static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies &hs)
+{
+    if (!bas || !h0) return fail(TBX_ERR_ARG, "basis or h0 parameters missing");
+    if (bas->nid <= 0 || bas->msh <= 0 || bas->msh > TBX_MAXSH)
+        return fail(TBX_ERR_BASIS, "nid=%d msh=%d outside 1..%d", bas->nid, bas->msh, TBX_MAXSH);
+    if (!bas->nsh_id || !bas->ang || !bas->nprim || !bas->alpha || !bas->coeff || !h0->hscale ||
+        !h0->shpoly || !h0->rad)
+        return fail(TBX_ERR_ARG, "null basis/h0 table");
+    const int nsp = bas->nid, msh = bas->msh;
+    hs.nsp = nsp;
+    hs.msh = msh;
+    hs.intcut = bas->intcut;
+    hs.nsh.assign(bas->nsh_id, bas->nsh_id + nsp);
+    hs.ang.assign(bas->ang, bas->ang + nsp * msh);
+    hs.nprim.assign(bas->nprim, bas->nprim + nsp * msh);
+    hs.alpha.assign(bas->alpha, bas->alpha + TBX_MAXG * nsp * msh);
+    hs.coeff.assign(bas->coeff, bas->coeff + TBX_MAXG * nsp * msh);
+    hs.shpoly.assign(h0->shpoly, h0->shpoly + nsp * msh);
+    hs.rad.assign(h0->rad, h0->rad + nsp);
+    hs.hscale.assign(h0->hscale, h0->hscale + (size_t)msh * msh * nsp * nsp);
+    hs.ncart.assign(nsp, 0);
+    hs.nao.assign(nsp, 0);
+    hs.cart_sh.assign((size_t)nsp * TBX_MAXCART, 0);
+    hs.cart_exp.assign((size_t)nsp * TBX_MAXCART, 0);
+    hs.ao_sh.assign((size_t)nsp * TBX_MAXAO, 0);
+    hs.ao_m.assign((size_t)nsp * TBX_MAXAO, 0);
+    hs.sh_cart.assign((size_t)nsp * msh, 0);
+    hs.sh_ao.assign((size_t)nsp * msh, 0);
+    for (int isp = 0; isp < nsp; ++isp) {
+        if (hs.nsh[isp] < 1 || hs.nsh[isp] > msh)
+            return fail(TBX_ERR_BASIS, "species %d has %d shells (msh=%d)", isp, hs.nsh[isp], msh);
+        int nc = 0, na = 0;
+        for (int ish = 0; ish < hs.nsh[isp]; ++ish) {
+            const int slot = ish + msh * isp;
+            const int l = hs.ang[slot];
+            if (l < 0 || l > TBX_MAXL)
+                return fail(TBX_ERR_BASIS, "angular momentum l=%d > %d (species %d shell %d)", l,
+                            TBX_MAXL, isp, ish);
+            if (hs.nprim[slot] < 1 || hs.nprim[slot] > TBX_MAXG)
+                return fail(TBX_ERR_BASIS, "nprim=%d outside 1..%d", hs.nprim[slot], TBX_MAXG);
+            for (int p = 0; p < hs.nprim[slot]; ++p)
+                if (!(hs.alpha[TBX_MAXG * slot + p] > 0.0) ||
+                    !std::isfinite(hs.coeff[TBX_MAXG * slot + p]))
+                    return fail(TBX_ERR_BASIS, "invalid primitive in species %d shell %d", isp, ish);
+            hs.sh_cart[slot] = nc;
+            hs.sh_ao[slot] = na;
+            int k = 0;  // CCA order: decreasing lx, then decreasing ly
+            for (int lx = l; lx >= 0; --lx)
+                for (int ly = l - lx; ly >= 0; --ly, ++k) {
+                    const int lz = l - lx - ly;
+                    if (nc >= TBX_MAXCART) return fail(TBX_ERR_BASIS, "too many Cartesian functions");
+                    hs.cart_sh[isp * TBX_MAXCART + nc] = ish;
+                    hs.cart_exp[isp * TBX_MAXCART + nc] = lx | (ly << 4) | (lz << 8) | (k << 12);
+                    ++nc;
+                }
+            for (int mm = 0; mm < 2 * l + 1; ++mm) {
+                if (na >= TBX_MAXAO) return fail(TBX_ERR_BASIS, "too many spherical functions");
+                hs.ao_sh[isp * TBX_MAXAO + na] = ish;
+                hs.ao_m[isp * TBX_MAXAO + na] = mm;
+                ++na;
+            }
+        }
+        hs.ncart[isp] = nc;
+        hs.nao[isp] = na;
+        hs.maxnc = std::max(hs.maxnc, nc);
+    }
+    // the kernels' windows hold at least one atom: (tasks per window)/nc >= nc
+    if (hs.maxnc > 16) return fail(TBX_ERR_BASIS, "%d Cartesian functions per atom (max 16)", hs.maxnc);
+    return TBX_OK;
+}
+
+struct HostMols {
+    int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
+    long long ndense = 0;
+    std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
+        cart_atom, ao_atom, order, sh_off;
+    std::vector<long long> mol_dense;
+};
+
+static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const int *id,
+                      HostMols &hm)
+{
+    if (nmol <= 0 || !mol_nat || !id) return fail(TBX_ERR_ARG, "empty batch");
+    hm.nmol = nmol;
+    hm.mol_atom.assign(nmol + 1, 0);
+    for (int m = 0; m < nmol; ++m) {
+        if (mol_nat[m] <= 0) return fail(TBX_ERR_ARG, "molecule %d has %d atoms", m, mol_nat[m]);
+        hm.mol_atom[m + 1] = hm.mol_atom[m] + mol_nat[m];
+    }
+    const int nat = hm.mol_atom[nmol];
+    hm.nat = nat;
+    hm.at_sp.assign(id, id + nat);
+    hm.at_mol.resize(nat);
+    hm.at_sh.resize(nat);
+    hm.at_ao.resize(nat);
+    hm.at_cart.resize(nat);
+    hm.mol_nao.assign(nmol, 0);
+    hm.mol_cart.assign(nmol + 1, 0);
+    hm.mol_ao.assign(nmol + 1, 0);
+    hm.mol_dense.assign(nmol + 1, 0);
+    hm.sh_off.assign(nmol + 1, 0);
+    int sh = 0, aog = 0, cg = 0;
+    for (int m = 0; m < nmol; ++m) {
+        int ao = 0, c = 0;
+        hm.sh_off[m] = sh;
+        for (int a = hm.mol_atom[m]; a < hm.mol_atom[m + 1]; ++a) {
+            const int s = hm.at_sp[a];
+            if (s < 0 || s >= hs.nsp) return fail(TBX_ERR_ARG, "atom %d: species %d out of range", a, s);
+            hm.at_mol[a] = m;
+            hm.at_sh[a] = sh;
+            hm.at_ao[a] = ao;
+            hm.at_cart[a] = c;
+            sh += hs.nsh[s];
+            ao += hs.nao[s];
+            c += hs.ncart[s];
+            for (int k = 0; k < hs.ncart[s]; ++k) hm.cart_atom.push_back(a);
+            for (int k = 0; k < hs.nao[s]; ++k) hm.ao_atom.push_back(a);
+        }
+        hm.mol_nao[m] = ao;
+        hm.mol_ao[m + 1] = hm.mol_ao[m] + ao;
+        hm.mol_cart[m + 1] = hm.mol_cart[m] + c;
+        hm.mol_dense[m + 1] = hm.mol_dense[m] + (long long)ao * ao;
+        aog += ao;
+        cg += c;
+    }
+    hm.sh_off[nmol] = sh;
+    hm.nsh = sh;
+    hm.nao = aog;
+    hm.ncart = cg;
+    hm.ndense = hm.mol_dense[nmol];
+    // panels ordered by work, heaviest first (rows x Cartesian columns)
+    std::vector<long long> cost(nat);
+    for (int a = 0; a < nat; ++a)
+        cost[a] = (long long)hm.mol_nao[hm.at_mol[a]] * hs.ncart[hm.at_sp[a]];
+    hm.order.resize(nat);
+    std::iota(hm.order.begin(), hm.order.end(), 0);
+    std::stable_sort(hm.order.begin(), hm.order.end(),
+                     [&](int x, int y) { return cost[x] > cost[y]; });
+    return TBX_OK;
+}
+
+// Device copies of the tables
+struct DevTables {
+    DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
+        ao_m, sh_cart, sh_ao;
+    DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
+        cart_atom, ao_atom, mol_ao, order;
+    TbxSpecies sp{};
+    TbxMols mt{};
+
+    void release()
+    {
+        DevBuf *all[] = {&nsh, &ang, &nprim, &alpha, &coeff, &shpoly, &rad, &hscale, &ncart, &nao,
+                         &cart_sh, &cart_exp, &ao_sh, &ao_m, &sh_cart, &sh_ao, &trans, &mol_atom,
+                         &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
+                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &order};
+        for (DevBuf *b : all) b->release();
+    }
+};
+
+static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cutoff, hipStream_t s)
+{
+    hipError_t e;
+#define UP(name) if ((e = upload(d.name, hs.name, s)) != hipSuccess) return e
+    UP(nsh); UP(ang); UP(nprim); UP(alpha); UP(coeff); UP(shpoly); UP(rad); UP(hscale); UP(ncart);
+    UP(nao); UP(cart_sh); UP(cart_exp); UP(ao_sh); UP(ao_m); UP(sh_cart); UP(sh_ao);
+#undef UP
+    TbxSpecies &sp = d.sp;
+    sp.nsp = hs.nsp;
+    sp.msh = hs.msh;
+    sp.nsh = d.nsh.as<int>();
+    sp.ang = d.ang.as<int>();
+    sp.nprim = d.nprim.as<int>();
+    sp.alpha = d.alpha.as<double>();
+    sp.coeff = d.coeff.as<double>();
+    sp.shpoly = d.shpoly.as<double>();
+    sp.rad = d.rad.as<double>();
+    sp.hscale = d.hscale.as<double>();
+    sp.ncart = d.ncart.as<int>();
+    sp.nao = d.nao.as<int>();
+    sp.cart_sh = d.cart_sh.as<int>();
+    sp.cart_exp = d.cart_exp.as<int>();
+    sp.ao_sh = d.ao_sh.as<int>();
+    sp.ao_m = d.ao_m.as<int>();
+    sp.sh_cart = d.sh_cart.as<int>();
+    sp.sh_ao = d.sh_ao.as<int>();
+    sp.intcut = hs.intcut;
+    sp.cutoff2 = cutoff * cutoff;
+    return hipSuccess;
+}
+
+static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vector<double> &trans,
+                              hipStream_t s)
+{
+    hipError_t e;
+#define UP(name) if ((e = upload(d.name, hm.name, s)) != hipSuccess) return e
+    UP(mol_atom); UP(mol_nao); UP(mol_cart); UP(mol_dense); UP(at_sp); UP(at_mol); UP(at_sh);
+    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(order);
+#undef UP
+    if ((e = upload(d.trans, trans, s)) != hipSuccess) return e;
+    TbxMols &mt = d.mt;
+    mt.nmol = hm.nmol;
+    mt.nat = hm.nat;
+    mt.ntrans = (int)(trans.size() / 3);
+    mt.trans = d.trans.as<double>();
+    mt.mol_atom = d.mol_atom.as<int>();
+    mt.mol_nao = d.mol_nao.as<int>();
+    mt.mol_cart = d.mol_cart.as<int>();
+    mt.mol_dense = d.mol_dense.as<long long>();
+    mt.at_sp = d.at_sp.as<int>();
+    mt.at_mol = d.at_mol.as<int>();
+    mt.at_sh = d.at_sh.as<int>();
+    mt.at_ao = d.at_ao.as<int>();
+    mt.at_cart = d.at_cart.as<int>();
+    mt.cart_atom = d.cart_atom.as<int>();
+    mt.ao_atom = d.ao_atom.as<int>();
+    mt.mol_ao = d.mol_ao.as<int>();
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+
+struct tbx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevTables tab;
+    DevBuf xyz, se, dsedcn, S, D, Q, H, P, W, vao, vdp, vqp, grad, dse, sigma, dEdcn, nnl, flag;
+    std::vector<double> host_tmp;
+};
+
+struct tbx_batch {
+    tbx_context *ctx = nullptr;
+    HostSpecies hs;
+    HostMols hm;
+    DevTables tab;
+    DevBuf flag, dse_tmp;
+    hipEvent_t ev[8] = {};
+    bool timed = false;
+};
+
+extern "C" {
+
+int tbx_version(void) { return 100; }
+
+const char *tbx_last_error(void) { return g_error.c_str(); }
+
+tbx_context *tbx_context_new(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        fail(TBX_ERR_DEVICE, "no HIP device %d (count %d)", device, n);
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        fail(TBX_ERR_DEVICE, "hipSetDevice(%d) failed", device);
+        return nullptr;
+    }
+    tbx_context *ctx = new tbx_context;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        fail(TBX_ERR_DEVICE, "stream creation failed");
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+void tbx_context_delete(tbx_context *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->tab.release();
+    DevBuf *all[] = {&ctx->xyz, &ctx->se, &ctx->dsedcn, &ctx->S, &ctx->D, &ctx->Q, &ctx->H,
+                     &ctx->P, &ctx->W, &ctx->vao, &ctx->vdp, &ctx->vqp, &ctx->grad, &ctx->dse,
+                     &ctx->sigma, &ctx->dEdcn, &ctx->nnl, &ctx->flag};
+    for (DevBuf *b : all) b->release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int tbx_get_selfenergy(int nat, const int *id, const int *ish_at, const int *nsh_id, int msh,
+                       const double *selfenergy_sp, const double *kcn, const double *cn,
+                       double *selfenergy, double *dsedcn)
+{
+    if (nat < 0 || !id || !ish_at || !nsh_id || !selfenergy_sp || !selfenergy || (cn && !kcn))
+        return fail(TBX_ERR_ARG, "tbx_get_selfenergy: invalid argument");
+    for (int iat = 0; iat < nat; ++iat) {
+        const int izp = id[iat], ii = ish_at[iat];
+        for (int ish = 0; ish < nsh_id[izp]; ++ish) {
+            double se = selfenergy_sp[ish + msh * izp];
+            if (cn) se -= kcn[ish + msh * izp] * cn[iat];
+            selfenergy[ii + ish] = se;
+            if (dsedcn) dsedcn[ii + ish] = cn ? -kcn[ish + msh * izp] : 0.0;
+        }
+    }
+    return TBX_OK;
+}
+
+} // extern "C"
+
+// Shared set-up of the single-molecule host path.
+static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list *list,
+                          const tbx_basis *bas, const tbx_h0par *h0, const int *ish_at,
+                          const int *iao_sh, int nao, HostSpecies &hs, HostMols &hm)
+{
+    if (!ctx) return fail(TBX_ERR_ARG, "null context");
+    if (!geom || geom->nat <= 0 || !geom->id || !geom->xyz)
+        return fail(TBX_ERR_ARG, "invalid geometry");
+    if (geom->ntrans < 1 || !geom->trans) return fail(TBX_ERR_ARG, "need at least one translation");
+    if (!(geom->cutoff > 0.0) || !std::isfinite(geom->cutoff))
+        return fail(TBX_ERR_ARG, "invalid cutoff %g", geom->cutoff);
+    for (int k = 0; k < 3 * geom->nat; ++k)
+        if (!std::isfinite(geom->xyz[k])) return fail(TBX_ERR_ARG, "non-finite coordinate %d", k);
+    int st = build_species(bas, h0, hs);
+    if (st) return st;
+    st = build_mols(hs, 1, &geom->nat, geom->id, hm);
+    if (st) return st;
+    if (hm.nao != nao) return fail(TBX_ERR_ARG, "nao=%d but the basis gives %d", nao, hm.nao);
+    if (ish_at && iao_sh) {
+        for (int a = 0; a < geom->nat; ++a) {
+            if (ish_at[a] != hm.at_sh[a])
+                return fail(TBX_ERR_ARG, "ish_at[%d]=%d, expected %d", a, ish_at[a], hm.at_sh[a]);
+            const int s = hm.at_sp[a];
+            for (int k = 0; k < hs.nsh[s]; ++k)
+                if (iao_sh[ish_at[a] + k] != hm.at_ao[a] + hs.sh_ao[k + hs.msh * s])
+                    return fail(TBX_ERR_ARG, "iao_sh does not follow the basis order");
+        }
+    }
+    if (list && (!list->nnl || !list->inl)) return fail(TBX_ERR_ARG, "incomplete neighbour list");
+    std::vector<double> trans(geom->trans, geom->trans + 3 * geom->ntrans);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
+    HIP_TRY(upload_mols(ctx->tab, hm, trans, s));
+    HIP_TRY(ctx->xyz.reserve(sizeof(double) * 3 * geom->nat));
+    HIP_TRY(hipMemcpyAsync(ctx->xyz.p, geom->xyz, sizeof(double) * 3 * geom->nat,
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->flag.reserve(sizeof(int)));
+    HIP_TRY(hipMemsetAsync(ctx->flag.p, 0, sizeof(int), s));
+    if (list) {
+        HIP_TRY(ctx->nnl.reserve(sizeof(int) * geom->nat));
+        HIP_TRY(hipMemcpyAsync(ctx->nnl.p, list->nnl, sizeof(int) * geom->nat,
+                               hipMemcpyHostToDevice, s));
+    }
+    return TBX_OK;
+}
+
+static int check_flag(tbx_context *ctx)
+{
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, ctx->flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (flag)
+        return fail(TBX_ERR_ARG,
+                    "neighbour list does not match the cutoff rule of new_adjacency_list");
+    return TBX_OK;
+}
+
+extern "C" int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const tbx_list *list,
+                                   const tbx_basis *bas, const tbx_h0par *h0, const int *ish_at,
+                                   const int *iao_sh, int nao, const double *selfenergy,
+                                   double *overlap, double *dpint, double *qpint,
+                                   double *hamiltonian)
+{
+    HostSpecies hs;
+    HostMols hm;
+    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, hs, hm);
+    if (st) return st;
+    if (!selfenergy || !overlap || !dpint || !qpint || !hamiltonian)
+        return fail(TBX_ERR_ARG, "null output array");
+    hipStream_t s = ctx->stream;
+    const size_t n2 = (size_t)nao * nao;
+    HIP_TRY(ctx->se.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(hipMemcpyAsync(ctx->se.p, selfenergy, sizeof(double) * hm.nsh, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->S.reserve(sizeof(double) * n2));
+    HIP_TRY(ctx->D.reserve(sizeof(double) * 3 * n2));
+    HIP_TRY(ctx->Q.reserve(sizeof(double) * 6 * n2));
+    HIP_TRY(ctx->H.reserve(sizeof(double) * n2));
+    HIP_TRY(tbx_launch_forward(&ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
+                               ctx->se.as<double>(), ctx->tab.order.as<int>(),
+                               ctx->S.as<double>(), ctx->D.as<double>(), ctx->Q.as<double>(),
+                               ctx->H.as<double>(), list ? ctx->nnl.as<int>() : nullptr,
+                               ctx->flag.as<int>(), s));
+    HIP_TRY(hipMemcpyAsync(overlap, ctx->S.p, sizeof(double) * n2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(dpint, ctx->D.p, sizeof(double) * 3 * n2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(qpint, ctx->Q.p, sizeof(double) * 6 * n2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hamiltonian, ctx->H.p, sizeof(double) * n2, hipMemcpyDeviceToHost, s));
+    return check_flag(ctx);
+}
+
+extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *geom,
+                                            const tbx_list *list, const tbx_basis *bas,
+                                            const tbx_h0par *h0, const int *ish_at,
+                                            const int *iao_sh, int nao, const double *selfenergy,
+                                            const double *dsedcn, int nspin, const double *vao,
+                                            const double *vdp, const double *vqp,
+                                            const double *pmat, const double *xmat,
+                                            double *dEdcn, double *gradient, double *sigma)
+{
+    HostSpecies hs;
+    HostMols hm;
+    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, hs, hm);
+    if (st) return st;
+    if (nspin < 1 || nspin > 2) return fail(TBX_ERR_ARG, "nspin=%d", nspin);
+    if (!selfenergy || !dsedcn || !vao || !vdp || !vqp || !pmat || !xmat || !dEdcn || !gradient ||
+        !sigma)
+        return fail(TBX_ERR_ARG, "null argument");
+    hipStream_t s = ctx->stream;
+    const int nat = geom->nat;
+    const size_t n2 = (size_t)nao * nao;
+    HIP_TRY(ctx->se.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(hipMemcpyAsync(ctx->se.p, selfenergy, sizeof(double) * hm.nsh, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->dsedcn.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(hipMemcpyAsync(ctx->dsedcn.p, dsedcn, sizeof(double) * hm.nsh, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->P.reserve(sizeof(double) * n2 * nspin));
+    HIP_TRY(hipMemcpyAsync(ctx->P.p, pmat, sizeof(double) * n2 * nspin, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->W.reserve(sizeof(double) * n2));
+    HIP_TRY(hipMemcpyAsync(ctx->W.p, xmat, sizeof(double) * n2, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->vao.reserve(sizeof(double) * nao * nspin));
+    HIP_TRY(hipMemcpyAsync(ctx->vao.p, vao, sizeof(double) * nao * nspin, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->vdp.reserve(sizeof(double) * 3 * nat));
+    HIP_TRY(hipMemcpyAsync(ctx->vdp.p, vdp, sizeof(double) * 3 * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->vqp.reserve(sizeof(double) * 6 * nat));
+    HIP_TRY(hipMemcpyAsync(ctx->vqp.p, vqp, sizeof(double) * 6 * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->grad.reserve(sizeof(double) * 3 * nat));
+    HIP_TRY(ctx->dse.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(ctx->sigma.reserve(sizeof(double) * 9));
+    HIP_TRY(ctx->dEdcn.reserve(sizeof(double) * nat));
+    HIP_TRY(tbx_launch_fill(ctx->grad.as<double>(), 3LL * nat, 0.0, s));
+    HIP_TRY(tbx_launch_fill(ctx->dse.as<double>(), hm.nsh, 0.0, s));
+    HIP_TRY(tbx_launch_fill(ctx->sigma.as<double>(), 9, 0.0, s));
+    HIP_TRY(tbx_launch_gradient(0, &ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
+                                ctx->se.as<double>(), ctx->tab.order.as<int>(),
+                                ctx->P.as<double>(), ctx->W.as<double>(), ctx->vao.as<double>(),
+                                ctx->vdp.as<double>(), ctx->vqp.as<double>(), nspin, (long long)n2,
+                                nao, nullptr, nullptr, nullptr, nullptr, ctx->grad.as<double>(),
+                                ctx->dse.as<double>(), ctx->sigma.as<double>(),
+                                list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), s));
+    HIP_TRY(tbx_launch_dedcn(&ctx->tab.sp, &ctx->tab.mt, ctx->dse.as<double>(),
+                             ctx->dsedcn.as<double>(), ctx->P.as<double>(),
+                             ctx->dEdcn.as<double>(), s));
+    std::vector<double> g(3 * nat), c(nat), sg(9);
+    HIP_TRY(hipMemcpyAsync(g.data(), ctx->grad.p, sizeof(double) * 3 * nat, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c.data(), ctx->dEdcn.p, sizeof(double) * nat, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(sg.data(), ctx->sigma.p, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
+    st = check_flag(ctx);  // synchronises the stream
+    if (st) return st;
+    for (int k = 0; k < 3 * nat; ++k) gradient[k] += g[k];
+    for (int k = 0; k < nat; ++k) dEdcn[k] += c[k];
+    for (int k = 0; k < 9; ++k) sigma[k] += sg[k];
+    return TBX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched path
+
+extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tbx_batch **out)
+{
+    if (!ctx || !desc || !out) return fail(TBX_ERR_ARG, "null argument");
+    *out = nullptr;
+    tbx_batch *b = new tbx_batch;
+    b->ctx = ctx;
+    int st = build_species(desc->bas, desc->h0, b->hs);
+    if (!st) st = build_mols(b->hs, desc->nmol, desc->mol_nat, desc->id, b->hm);
+    if (st) {
+        delete b;
+        return st;
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    std::vector<double> trans(3, 0.0);
+    if (e == hipSuccess) e = upload_species(b->tab, b->hs, desc->cutoff, ctx->stream);
+    if (e == hipSuccess) e = upload_mols(b->tab, b->hm, trans, ctx->stream);
+    if (e == hipSuccess) e = b->flag.reserve(sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(b->flag.p, 0, sizeof(int), ctx->stream);
+    for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        tbx_batch_delete(b);
+        return fail(TBX_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return TBX_OK;
+}
+
+extern "C" void tbx_batch_delete(tbx_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->tab.release();
+    b->flag.release();
+    b->dse_tmp.release();
+    for (auto &ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    delete b;
+}
+
+extern "C" int tbx_batch_sizes_get(const tbx_batch *b, tbx_batch_sizes *sz)
+{
+    if (!b || !sz) return fail(TBX_ERR_ARG, "null argument");
+    sz->nmol = b->hm.nmol;
+    sz->nat = b->hm.nat;
+    sz->nsh = b->hm.nsh;
+    sz->nao = b->hm.nao;
+    sz->ndense = b->hm.ndense;
+    sz->npair = 0;
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_offsets(const tbx_batch *b, int *atom_off, int *sh_off, int *ao_off,
+                                 long long *dense_off)
+{
+    if (!b) return fail(TBX_ERR_ARG, "null batch");
+    const int n = b->hm.nmol + 1;
+    if (atom_off) std::copy(b->hm.mol_atom.begin(), b->hm.mol_atom.end(), atom_off);
+    if (sh_off) std::copy(b->hm.sh_off.begin(), b->hm.sh_off.end(), sh_off);
+    if (ao_off) std::copy(b->hm.mol_ao.begin(), b->hm.mol_ao.end(), ao_off);
+    if (dense_off) std::copy(b->hm.mol_dense.begin(), b->hm.mol_dense.end(), dense_off);
+    (void)n;
+    return TBX_OK;
+}
+
+static hipStream_t pick_stream(tbx_batch *b, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : b->ctx->stream;
+}
+
+extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                 double *d_S, double *d_D, double *d_Q, double *d_H, void *stream)
+{
+    if (!b || !d_xyz || !d_se || !d_S || !d_D || !d_Q || !d_H) return fail(TBX_ERR_ARG, "null argument");
+    hipStream_t s = pick_stream(b, stream);
+    HIP_TRY(hipEventRecord(b->ev[0], s));
+    HIP_TRY(tbx_launch_forward(&b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(), d_S,
+                               d_D, d_Q, d_H, nullptr, b->flag.as<int>(), s));
+    HIP_TRY(hipEventRecord(b->ev[1], s));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                  const double *d_dsedcn, const double *d_P, const double *d_W,
+                                  const double *d_vao, const double *d_vdp, const double *d_vqp,
+                                  double *d_dEdcn, double *d_grad, double *d_sigma, void *stream)
+{
+    if (!b || !d_xyz || !d_se || !d_dsedcn || !d_P || !d_W || !d_vao || !d_vdp || !d_vqp ||
+        !d_dEdcn || !d_grad || !d_sigma)
+        return fail(TBX_ERR_ARG, "null argument");
+    hipStream_t s = pick_stream(b, stream);
+    HIP_TRY(b->dse_tmp.reserve(sizeof(double) * std::max(1, b->hm.nsh)));
+    HIP_TRY(hipEventRecord(b->ev[4], s));
+    HIP_TRY(tbx_launch_fill(d_grad, 3LL * b->hm.nat, 0.0, s));
+    HIP_TRY(tbx_launch_fill(b->dse_tmp.as<double>(), b->hm.nsh, 0.0, s));
+    HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
+    HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(),
+                                d_P, d_W, d_vao, d_vdp, d_vqp, 1, b->hm.ndense, b->hm.nao,
+                                nullptr, nullptr, nullptr, nullptr, d_grad,
+                                b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(), s));
+    HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
+                             d_dEdcn, s));
+    HIP_TRY(hipEventRecord(b->ev[5], s));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                 const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
+                                 const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream)
+{
+    if (!b || !d_xyz || !d_se || !d_Sbar || !d_Dbar || !d_Qbar || !d_Hbar || !d_dxyz || !d_dse)
+        return fail(TBX_ERR_ARG, "null argument");
+    hipStream_t s = pick_stream(b, stream);
+    HIP_TRY(b->dse_tmp.reserve(sizeof(double) * 9 * b->hm.nmol));
+    HIP_TRY(hipEventRecord(b->ev[6], s));
+    HIP_TRY(tbx_launch_fill(d_dxyz, 3LL * b->hm.nat, 0.0, s));
+    HIP_TRY(tbx_launch_fill(d_dse, b->hm.nsh, 0.0, s));
+    HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(),
+                                nullptr, nullptr, nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar,
+                                d_Dbar, d_Qbar, d_Hbar, d_dxyz, d_dse, b->dse_tmp.as<double>(),
+                                nullptr, b->flag.as<int>(), s));
+    HIP_TRY(hipEventRecord(b->ev[7], s));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pairs,
+                                     double *ms_gradient, double *ms_adjoint)
+{
+    if (!b) return fail(TBX_ERR_ARG, "null batch");
+    auto el = [&](int a, int c) {
+        float ms = 0.0f;
+        if (hipEventQuery(b->ev[c]) != hipSuccess) return -1.0;
+        if (hipEventElapsedTime(&ms, b->ev[a], b->ev[c]) != hipSuccess) return -1.0;
+        return (double)ms;
+    };
+    if (ms_forward) *ms_forward = el(0, 1);
+    if (ms_pairs) *ms_pairs = 0.0;
+    if (ms_gradient) *ms_gradient = el(4, 5);
+    if (ms_adjoint) *ms_adjoint = el(6, 7);
+    return TBX_OK;
+}

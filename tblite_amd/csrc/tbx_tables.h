@@ -1,0 +1,50 @@
+// tbx_tables.h -- layout shared by the host set-up (tbx_capi.hip) and the
+// gfx950 kernels (tbx_kernels.hip).  Internal, not part of the C-ABI.
+#pragma once
+#include <stdint.h>
+
+#define TBX_MAXCART 24 /* Cartesian functions per species: <= 4 shells, l <= 2 */
+#define TBX_MAXAO 20   /* spherical functions per species */
+
+// Species tables (device pointers).  Shell slot = ish + msh*isp.
+struct TbxSpecies {
+    int nsp, msh;
+    const int *nsh;       // [nsp]
+    const int *ang;       // [nsp*msh]
+    const int *nprim;     // [nsp*msh]
+    const double *alpha;  // [nsp*msh*MAXG]
+    const double *coeff;  // [nsp*msh*MAXG]
+    const double *shpoly; // [nsp*msh]
+    const double *rad;    // [nsp]
+    const double *hscale; // [msh*msh*nsp*nsp], (jsh, ish, jsp, isp)
+    const int *ncart;     // [nsp] Cartesian functions of the species
+    const int *nao;       // [nsp] spherical functions of the species
+    const int *cart_sh;   // [nsp*TBX_MAXCART] shell of each Cartesian function
+    const int *cart_exp;  // [nsp*TBX_MAXCART] lx | ly<<4 | lz<<8 | (index in shell)<<12
+    const int *ao_sh;     // [nsp*TBX_MAXAO]  shell of each spherical function
+    const int *ao_m;      // [nsp*TBX_MAXAO]  m index (0..2l) within the shell
+    const int *sh_cart;   // [nsp*msh] first Cartesian function of the shell within the atom
+    const int *sh_ao;     // [nsp*msh] first spherical function of the shell within the atom
+    double intcut;
+    double cutoff2;
+};
+
+// Molecule / atom tables of a batch (device pointers).  "local" indices
+// count from the start of the atom's molecule.
+struct TbxMols {
+    int nmol, nat;
+    int ntrans;
+    const double *trans;           // [3*ntrans]
+    const int *mol_atom;           // [nmol+1] first atom
+    const int *mol_nao;            // [nmol]
+    const int *mol_cart;           // [nmol+1] first Cartesian function (global)
+    const long long *mol_dense;    // [nmol+1] first dense element of S/H
+    const int *at_sp;              // [nat]
+    const int *at_mol;             // [nat]
+    const int *at_sh;              // [nat] first shell (global) -> self-energies
+    const int *at_ao;              // [nat] first AO, molecule-local
+    const int *at_cart;            // [nat] first Cartesian function, molecule-local
+    const int *cart_atom;          // [total Cartesian] atom (global)
+    const int *ao_atom;            // [total AO] atom (global)
+    const int *mol_ao;             // [nmol+1] first AO (global)
+};

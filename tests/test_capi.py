@@ -1,0 +1,83 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+``include/tbx.h`` declares, host-only entry points match the oracle, and
+argument errors come back as status codes (no aborts)."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tblite_amd import h0, lib as L, structure, synthetic
+
+
+def test_library_exports_header_symbols():
+    names = L.header_functions()
+    assert "tbx_get_hamiltonian" in names and "tbx_batch_adjoint" in names
+    so = L.lib()
+    for n in names:
+        assert hasattr(so, n), n
+
+
+def test_selfenergy_matches_oracle():
+    z, x = synthetic.octane()
+    st = structure.setup(structure.Structure(z, x))
+    cn = np.linspace(0.5, 3.5, st.mol.nat)
+    se, dse = h0.get_selfenergy(st, cn)
+    se_ref, dse_ref = st.selfenergy(cn)
+    assert np.array_equal(se, se_ref) and np.array_equal(dse, dse_ref)
+    # oracle restatement of get_selfenergy
+    nsh = st.bas.nsh
+    out, dout = np.zeros(nsh), np.zeros(nsh)
+    p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+    ids = np.ascontiguousarray(st.mol.id, dtype=np.int32)
+    ish = np.ascontiguousarray(st.bas.ish_at, dtype=np.int32)
+    nid = np.ascontiguousarray(st.spec.nsh_id, dtype=np.int32)
+    sesp = np.ascontiguousarray(st.spec.selfenergy.reshape(-1, order="F"))
+    kcn = np.ascontiguousarray(st.spec.kcn.reshape(-1, order="F"))
+    O.lib().orc_get_selfenergy(st.mol.nat, p(ids), p(ish), p(nid), st.msh, p(sesp), p(kcn),
+                               p(np.ascontiguousarray(cn)), p(out), p(dout))
+    assert np.array_equal(se, out) and np.array_equal(dse, dout)
+
+
+def test_null_context_is_an_error_not_a_crash():
+    z, x = synthetic.octane()
+    st = structure.setup(structure.Structure(z, x))
+    a = h0._Args(st)
+    nao = st.bas.nao
+    S = np.zeros((nao, nao), order="F")
+    se = np.zeros(st.bas.nsh)
+    st_code = L.lib().tbx_get_hamiltonian(None, ct.byref(a.geom), ct.byref(a.list),
+                                          ct.byref(a.basis), ct.byref(a.h0), h0._p(a.ish_at),
+                                          h0._p(a.iao_sh), nao, h0._p(se), h0._p(S), h0._p(S),
+                                          h0._p(S), h0._p(S))
+    assert st_code == 1
+    assert b"context" in L.lib().tbx_last_error()
+
+
+def test_selfenergy_rejects_null():
+    assert L.lib().tbx_get_selfenergy(1, None, None, None, 1, None, None, None, None, None) == 1
+
+
+def test_batch_create_rejects_l3_shell_without_gpu():
+    """A basis with an f shell is refused with TBX_ERR_BASIS before any device
+    work (the host validation runs first)."""
+    ang = np.array([3], dtype=np.int32)
+    nprim = np.array([1], dtype=np.int32)
+    nsh = np.array([1], dtype=np.int32)
+    alpha = np.zeros(12)
+    alpha[0] = 1.0
+    coeff = np.zeros(12)
+    coeff[0] = 1.0
+    hs = np.ones(1)
+    p = h0._p
+    bas = L.Basis(1, 1, p(nsh), p(ang), p(nprim), p(alpha), p(coeff), 25.0)
+    par = L.H0Par(p(hs), p(hs), p(hs))
+    mol_nat = np.array([1], dtype=np.int32)
+    ids = np.array([0], dtype=np.int32)
+    desc = L.BatchDesc(1, p(mol_nat), p(ids), ct.pointer(bas), ct.pointer(par), 10.0)
+    out = ct.c_void_p()
+    # a dummy non-null context pointer is never dereferenced before validation
+    code = L.lib().tbx_batch_create(ct.c_void_p(1), ct.byref(desc), ct.byref(out))
+    assert code == 2
+    assert b"l=3" in L.lib().tbx_last_error()
+    assert not out.value
