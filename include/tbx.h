@@ -148,8 +148,9 @@ int tbx_batch_offsets(const tbx_batch *b, int *atom_off, int *sh_off, int *ao_of
 /* Forward: dense S, D, Q, H per molecule into device buffers of
  * ndense, 3*ndense, 6*ndense, ndense doubles.  d_xyz: [3*nat], d_se: [nsh],
  * device pointers.  Neighbours are the pairs within desc->cutoff, decided on
- * the device from d_xyz (no host neighbour list).  `stream` may be NULL (the
- * context's stream). */
+ * the device from d_xyz (no host neighbour list).  `stream` is the HIP
+ * stream to launch on; NULL is the default (null) stream.  All tbx_batch_*
+ * launches are asynchronous and allocate nothing on the device. */
 int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double *d_se, double *d_S,
                       double *d_D, double *d_Q, double *d_H, void *stream);
 

@@ -1,0 +1,199 @@
+"""Batched, device-resident GFN2 integral path over many molecules, and its
+``torch.autograd.Function``.
+
+One ``Batch`` holds the species tables and molecule offsets of ``nmol``
+independent molecules (uploaded once); each call launches one kernel over all
+molecules on torch's current HIP stream, so the launches order with torch work
+and can be captured in a HIP graph (``torch.cuda.CUDAGraph``).
+
+Packed layout (all float64, device):
+  xyz   [3*nat]                 atoms of molecule m at atom_off[m]:atom_off[m+1]
+  se    [nsh]                   self-energies (get_selfenergy) per shell
+  S, H  [ndense]                molecule m: nao_m x nao_m column-major at dense_off[m]
+  D     [3*ndense], Q [6*ndense]  (3|6, nao_m, nao_m) Fortran order at 3|6*dense_off[m]
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+import torch
+
+from . import lib as L
+from . import params, structure
+
+
+def _ptr(t: torch.Tensor):
+    assert t.is_contiguous() and t.dtype == torch.float64
+    return ct.c_void_p(t.data_ptr())
+
+
+class Batch:
+    """Batched GFN2 S/D/Q/H0 + gradient for a list of ``(numbers, xyz_bohr)``."""
+
+    def __init__(self, molecules, device: int = 0):
+        self.device = device
+        numbers = [np.asarray(z, dtype=np.int32) for z, _ in molecules]
+        allz = np.concatenate(numbers)
+        self.nums, ids = params.species_of(allz)
+        self.spec = params.gfn2_species(self.nums)
+        self.msh = int(self.spec.nsh_id.max())
+        bas = params.Basis.new(ids, self.spec.nsh_id, self.spec.basis_cgto)
+        self.cutoff = bas.get_cutoff()
+        self.intcut = bas.intcut
+        self.mol_nat = np.array([len(z) for z in numbers], dtype=np.int32)
+        self.ids = ids
+        self.xyz0 = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)
+                                    for _, x in molecules])
+        ang, nprim, alpha, coeff = structure.flat_cgto(self.spec, self.msh)
+        self._keep = [np.ascontiguousarray(a) for a in (
+            self.spec.nsh_id.astype(np.int32), ang, nprim, alpha, coeff,
+            self.spec.hscale.reshape(-1, order="F"), self.spec.shpoly.reshape(-1, order="F"),
+            self.spec.rad, self.mol_nat, ids.astype(np.int32))]
+        k = self._keep
+        p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+        self._basis = L.Basis(len(self.nums), self.msh, p(k[0]), p(k[1]), p(k[2]), p(k[3]),
+                              p(k[4]), self.intcut)
+        self._h0 = L.H0Par(p(k[5]), p(k[6]), p(k[7]))
+        desc = L.BatchDesc(len(molecules), p(k[8]), p(k[9]), ct.pointer(self._basis),
+                           ct.pointer(self._h0), self.cutoff)
+        handle = ct.c_void_p()
+        L.check(L.lib().tbx_batch_create(L.context(device), ct.byref(desc), ct.byref(handle)))
+        self.handle = handle
+        sz = L.BatchSizes()
+        L.check(L.lib().tbx_batch_sizes_get(handle, ct.byref(sz)))
+        self.nmol, self.nat, self.nsh, self.nao, self.ndense = (sz.nmol, sz.nat, sz.nsh, sz.nao,
+                                                                 sz.ndense)
+        n = self.nmol + 1
+        self.atom_off = np.zeros(n, dtype=np.int32)
+        self.sh_off = np.zeros(n, dtype=np.int32)
+        self.ao_off = np.zeros(n, dtype=np.int32)
+        self.dense_off = np.zeros(n, dtype=np.int64)
+        L.check(L.lib().tbx_batch_offsets(handle, p(self.atom_off), p(self.sh_off),
+                                          p(self.ao_off), p(self.dense_off)))
+        # per-molecule views of the setup, for host-side references
+        self._shell_tables()
+
+    def _shell_tables(self):
+        spec = self.spec
+        self.selfenergy_sp = np.concatenate([spec.selfenergy[: spec.nsh_id[s], s]
+                                             for s in self.ids])
+        self.kcn_sh = np.concatenate([spec.kcn[: spec.nsh_id[s], s] for s in self.ids])
+        self.sh_atom = np.repeat(np.arange(self.nat), spec.nsh_id[self.ids])
+        nao_sp = np.array([sum(2 * c.ang + 1 for c in shells) for shells in spec.basis_cgto])
+        self.ao_atom = np.repeat(np.arange(self.nat), nao_sp[self.ids])
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                L.lib().tbx_batch_delete(h)
+            except Exception:  # interpreter shutdown
+                pass
+
+    # -- helpers -----------------------------------------------------------
+    @property
+    def torch_device(self):
+        return torch.device("cuda", self.device)
+
+    def tensor_xyz(self):
+        return torch.tensor(self.xyz0.reshape(-1), dtype=torch.float64, device=self.torch_device)
+
+    def selfenergy(self, cn):
+        """``get_selfenergy`` over the batch (torch, differentiable in ``cn``)."""
+        cn_sh = cn[torch.as_tensor(self.sh_atom, device=cn.device)]
+        se0 = torch.as_tensor(self.selfenergy_sp, device=cn.device)
+        kcn = torch.as_tensor(self.kcn_sh, device=cn.device)
+        return se0 - kcn * cn_sh, -kcn
+
+    def molecule_views(self, flat, comps=1):
+        """Split a packed dense tensor into per-molecule (comps, nao, nao)
+        Fortran-order views (returned as torch tensors of shape
+        (nao, nao, comps) in C order = Fortran (comps, nao, nao))."""
+        out = []
+        for m in range(self.nmol):
+            n = int(self.ao_off[m + 1] - self.ao_off[m])
+            a, b = int(self.dense_off[m]), int(self.dense_off[m + 1])
+            out.append(flat[comps * a: comps * b].view(n, n, comps))
+        return out
+
+    def _stream(self):
+        return ct.c_void_p(torch.cuda.current_stream(self.torch_device).cuda_stream)
+
+    # -- kernels -----------------------------------------------------------
+    def forward(self, xyz, se, out=None):
+        """Dense S, D, Q, H0 of every molecule (packed)."""
+        dev = self.torch_device
+        if out is None:
+            out = (torch.empty(self.ndense, dtype=torch.float64, device=dev),
+                   torch.empty(3 * self.ndense, dtype=torch.float64, device=dev),
+                   torch.empty(6 * self.ndense, dtype=torch.float64, device=dev),
+                   torch.empty(self.ndense, dtype=torch.float64, device=dev))
+        S, D, Q, H = out
+        L.check(L.lib().tbx_batch_forward(self.handle, _ptr(xyz), _ptr(se), _ptr(S), _ptr(D),
+                                          _ptr(Q), _ptr(H), self._stream()))
+        return S, D, Q, H
+
+    def gradient(self, xyz, se, dsedcn, P, W, vao, vdp, vqp, out=None):
+        """Plain ``get_hamiltonian_gradient`` contraction for every molecule:
+        returns dEdcn [nat], gradient [3*nat] (xyz order), sigma [9*nmol]."""
+        dev = self.torch_device
+        if out is None:
+            out = (torch.empty(self.nat, dtype=torch.float64, device=dev),
+                   torch.empty(3 * self.nat, dtype=torch.float64, device=dev),
+                   torch.empty(9 * self.nmol, dtype=torch.float64, device=dev))
+        dEdcn, grad, sigma = out
+        L.check(L.lib().tbx_batch_gradient(self.handle, _ptr(xyz), _ptr(se), _ptr(dsedcn),
+                                           _ptr(P), _ptr(W), _ptr(vao), _ptr(vdp), _ptr(vqp),
+                                           _ptr(dEdcn), _ptr(grad), _ptr(sigma), self._stream()))
+        return dEdcn, grad, sigma
+
+    def adjoint(self, xyz, se, Sb, Db, Qb, Hb, out=None):
+        """Backward of ``forward``: dE/dxyz [3*nat] and dE/dse [nsh]."""
+        dev = self.torch_device
+        if out is None:
+            out = (torch.empty(3 * self.nat, dtype=torch.float64, device=dev),
+                   torch.empty(self.nsh, dtype=torch.float64, device=dev))
+        dxyz, dse = out
+        L.check(L.lib().tbx_batch_adjoint(self.handle, _ptr(xyz), _ptr(se), _ptr(Sb), _ptr(Db),
+                                          _ptr(Qb), _ptr(Hb), _ptr(dxyz), _ptr(dse),
+                                          self._stream()))
+        return dxyz, dse
+
+    def last_timing(self):
+        v = [ct.c_double() for _ in range(4)]
+        L.check(L.lib().tbx_batch_last_timing(self.handle, *[ct.byref(x) for x in v]))
+        return {"forward_ms": v[0].value, "gradient_ms": v[2].value, "adjoint_ms": v[3].value}
+
+
+class H0Integrals(torch.autograd.Function):
+    """(xyz, se) -> (S, D, Q, H0) packed over a ``Batch``; the backward runs
+    the adjoint kernel, recomputing the integrals and their derivatives
+    instead of storing derivative tensors."""
+
+    @staticmethod
+    def forward(ctx, xyz, se, batch):
+        ctx.batch = batch
+        ctx.save_for_backward(xyz, se)
+        return batch.forward(xyz.detach(), se.detach())
+
+    @staticmethod
+    def backward(ctx, Sb, Db, Qb, Hb):
+        xyz, se = ctx.saved_tensors
+        b = ctx.batch
+        dev = xyz.device
+
+        def dense(t, n):
+            if t is None:
+                return torch.zeros(n, dtype=torch.float64, device=dev)
+            return t.contiguous()
+
+        dxyz, dse = b.adjoint(xyz.detach(), se.detach(), dense(Sb, b.ndense),
+                              dense(Db, 3 * b.ndense), dense(Qb, 6 * b.ndense),
+                              dense(Hb, b.ndense))
+        return dxyz, dse, None
+
+
+def h0_integrals(batch: Batch, xyz, se):
+    """Differentiable batched S, D, Q, H0 (see ``H0Integrals``)."""
+    return H0Integrals.apply(xyz, se, batch)

@@ -577,6 +577,9 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     if (e == hipSuccess) e = upload_species(b->tab, b->hs, desc->cutoff, ctx->stream);
     if (e == hipSuccess) e = upload_mols(b->tab, b->hm, trans, ctx->stream);
     if (e == hipSuccess) e = b->flag.reserve(sizeof(int));
+    // scratch of the gradient / adjoint launches: dE/dse (plain) or sigma (adjoint)
+    if (e == hipSuccess)
+        e = b->dse_tmp.reserve(sizeof(double) * std::max({1, b->hm.nsh, 9 * b->hm.nmol}));
     if (e == hipSuccess) e = hipMemsetAsync(b->flag.p, 0, sizeof(int), ctx->stream);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -626,9 +629,21 @@ extern "C" int tbx_batch_offsets(const tbx_batch *b, int *atom_off, int *sh_off,
     return TBX_OK;
 }
 
-static hipStream_t pick_stream(tbx_batch *b, void *stream)
+// The caller's stream, as HIP defines it: NULL is the default (null) stream,
+// which is also what torch reports for its default stream.
+static hipStream_t pick_stream(tbx_batch *, void *stream)
 {
-    return stream ? static_cast<hipStream_t>(stream) : b->ctx->stream;
+    return static_cast<hipStream_t>(stream);
+}
+
+// Timing events are recorded only outside HIP graph capture.
+static hipError_t mark(tbx_batch *b, int k, hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(s, &st);
+    if (e != hipSuccess) return e;
+    if (st != hipStreamCaptureStatusNone) return hipSuccess;
+    return hipEventRecord(b->ev[k], s);
 }
 
 extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double *d_se,
@@ -636,10 +651,10 @@ extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double
 {
     if (!b || !d_xyz || !d_se || !d_S || !d_D || !d_Q || !d_H) return fail(TBX_ERR_ARG, "null argument");
     hipStream_t s = pick_stream(b, stream);
-    HIP_TRY(hipEventRecord(b->ev[0], s));
+    HIP_TRY(mark(b, 0, s));
     HIP_TRY(tbx_launch_forward(&b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(), d_S,
                                d_D, d_Q, d_H, nullptr, b->flag.as<int>(), s));
-    HIP_TRY(hipEventRecord(b->ev[1], s));
+    HIP_TRY(mark(b, 1, s));
     return TBX_OK;
 }
 
@@ -652,8 +667,7 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
         !d_dEdcn || !d_grad || !d_sigma)
         return fail(TBX_ERR_ARG, "null argument");
     hipStream_t s = pick_stream(b, stream);
-    HIP_TRY(b->dse_tmp.reserve(sizeof(double) * std::max(1, b->hm.nsh)));
-    HIP_TRY(hipEventRecord(b->ev[4], s));
+    HIP_TRY(mark(b, 4, s));
     HIP_TRY(tbx_launch_fill(d_grad, 3LL * b->hm.nat, 0.0, s));
     HIP_TRY(tbx_launch_fill(b->dse_tmp.as<double>(), b->hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
@@ -663,7 +677,7 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
                                 b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(), s));
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
                              d_dEdcn, s));
-    HIP_TRY(hipEventRecord(b->ev[5], s));
+    HIP_TRY(mark(b, 5, s));
     return TBX_OK;
 }
 
@@ -674,15 +688,14 @@ extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double
     if (!b || !d_xyz || !d_se || !d_Sbar || !d_Dbar || !d_Qbar || !d_Hbar || !d_dxyz || !d_dse)
         return fail(TBX_ERR_ARG, "null argument");
     hipStream_t s = pick_stream(b, stream);
-    HIP_TRY(b->dse_tmp.reserve(sizeof(double) * 9 * b->hm.nmol));
-    HIP_TRY(hipEventRecord(b->ev[6], s));
+    HIP_TRY(mark(b, 6, s));
     HIP_TRY(tbx_launch_fill(d_dxyz, 3LL * b->hm.nat, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_dse, b->hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(),
                                 nullptr, nullptr, nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar,
                                 d_Dbar, d_Qbar, d_Hbar, d_dxyz, d_dse, b->dse_tmp.as<double>(),
                                 nullptr, b->flag.as<int>(), s));
-    HIP_TRY(hipEventRecord(b->ev[7], s));
+    HIP_TRY(mark(b, 7, s));
     return TBX_OK;
 }
 
