@@ -1,0 +1,296 @@
+"""Throughput of the GFN2 S/D/Q + H0 + gradient path on one MI355X.
+
+Workload (one "step"): the batched GFN2 integral path over 2048 synthetic
+molecules (atom counts log-uniform in [8, 256], H/C/N/O/F/S/Cl), FP64:
+
+  forward   dense S, D, Q (traceless), H0 for every molecule  (one launch)
+  backward  torch.autograd through the adjoint kernel with the adjoints of
+            get_hamiltonian_gradient's energy functional, i.e. dE/dR and dE/dse
+            equal to the plain gradient (one launch)
+
+Inputs are resident in HBM when the timed region starts.  ``value`` is
+molecules per second of that batched-autograd step.  Beside it the line
+reports the batched plain-gradient step, the plain per-molecule GPU path (the
+drop-in host-pointer entry points, PCIe copies included, on a sample), the
+roofline of the dominant kernel (HIP events on the launch stream) and the CPU
+baseline (the oracle, a C restatement of the reference loop nest with OpenMP,
+timed in a child process on a bounded sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+NMOL = 2048
+SEED = 1004
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+# --------------------------------------------------------------------------
+# CPU baseline (child process: no torch, its own OpenMP runtime)
+
+def cpu_baseline_child(budget_s: float):
+    import numpy as np
+
+    from oracle import oracle as O
+    from tblite_amd import structure, synthetic
+
+    mols = synthetic.molecule_batch(NMOL, seed=SEED, workers=min(8, _cpu_threads()))
+    rng = np.random.default_rng(7)
+    done, t_work, atoms = 0, 0.0, 0
+    for z, x in mols:
+        st = structure.setup(structure.Structure(z, x))
+        nao, nat = st.bas.nao, st.mol.nat
+        cn = rng.uniform(0.5, 3.5, nat)
+        g = rng.normal(size=(nao, nao))
+        P = np.asfortranarray((0.5 * (g + g.T))[:, :, None])
+        W = np.asfortranarray(0.1 * P)
+        vao = np.zeros((nao, 1), order="F")
+        vdp = np.zeros((3, nat, 1), order="F")
+        vqp = np.zeros((6, nat, 1), order="F")
+        t0 = time.perf_counter()
+        # timed region of the reference path: selfenergy, neighbour list,
+        # get_hamiltonian, get_hamiltonian_gradient
+        se, dse = st.selfenergy(cn)
+        structure.new_adjacency_list(st.mol.xyz, st.trans, st.cutoff)
+        O.get_hamiltonian(st, se)
+        O.get_hamiltonian_gradient(st, se, dse, vao, vdp, vqp, P, W)
+        t_work += time.perf_counter() - t0
+        done += 1
+        atoms += nat
+        if t_work > budget_s:
+            break
+    print(json.dumps({"molecules": done, "atoms": atoms, "seconds": t_work}))
+
+
+def run_cpu_baseline(budget_s=15.0):
+    threads = _cpu_threads()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_SCHEDULE="dynamic",
+               OMP_PROC_BIND="spread", OMP_STACKSIZE="4G")
+    out = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-baseline-child",
+                          str(budget_s)], env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        log("cpu baseline failed: " + out.stderr[-2000:])
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["molecules"] / r["seconds"], "unit": "molecules/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {r['molecules']} of the 2048 molecules ({r['atoms']} atoms), "
+                      f"oracle get_selfenergy+adjacency list+get_hamiltonian+"
+                      f"get_hamiltonian_gradient, OpenMP {threads} threads, "
+                      f"{r['seconds']:.1f} s"}
+
+
+# --------------------------------------------------------------------------
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nmol", type=int, default=NMOL)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-baseline-child", type=float, default=None)
+    args = ap.parse_args()
+
+    if args.cpu_baseline_child is not None:
+        cpu_baseline_child(args.cpu_baseline_child)
+        return
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              "(molecules shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_budget)
+
+    import numpy as np
+    import torch
+
+    from tblite_amd import h0 as h0mod
+    from tblite_amd import params, roofline, structure, synthetic
+    from tblite_amd.batch import Batch, h0_integrals
+
+    t0 = time.time()
+    mols = synthetic.molecule_batch(args.nmol, seed=SEED, workers=min(8, _cpu_threads()))
+    log(f"generated {len(mols)} molecules in {time.time() - t0:.1f} s")
+    b = Batch(mols)
+    dev = torch.device("cuda", 0)
+    log(f"batch: nat={b.nat} nsh={b.nsh} nao={b.nao} ndense={b.ndense}")
+    ids_m = [b.ids[b.atom_off[m]: b.atom_off[m + 1]] for m in range(b.nmol)]
+    kc_off, kc_on = roofline.count_kc(b.spec, ids_m, [x for _, x in mols], b.cutoff, b.intcut)
+    kc = kc_off + kc_on
+
+    # inputs (seeded); the values do not change the work of any kernel
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    xyz = b.tensor_xyz()
+    cn = torch.rand(b.nat, generator=g, dtype=torch.float64, device=dev) * 3.0 + 0.5
+    se, dsedcn = b.selfenergy(cn)
+    se, dsedcn = se.contiguous(), dsedcn.contiguous()
+    P = torch.empty(b.ndense, dtype=torch.float64, device=dev)
+    W = torch.empty(b.ndense, dtype=torch.float64, device=dev)
+    for m, (pv, wv) in enumerate(zip(b.molecule_views(P), b.molecule_views(W))):
+        n = pv.shape[0]
+        a = torch.randn(n, n, generator=g, dtype=torch.float64, device=dev)
+        pv[:, :, 0] = 0.5 * (a + a.T)
+        wv[:, :, 0] = 0.05 * pv[:, :, 0]
+    vao = torch.randn(b.nao, generator=g, dtype=torch.float64, device=dev) * 1e-2
+    vdp = torch.randn(3 * b.nat, generator=g, dtype=torch.float64, device=dev) * 1e-3
+    vqp = torch.randn(6 * b.nat, generator=g, dtype=torch.float64, device=dev) * 1e-3
+    # adjoints of the functional get_hamiltonian_gradient differentiates
+    ao_atom = torch.as_tensor(b.ao_atom, device=dev)
+    Hb = P
+    Sb = torch.empty_like(P)
+    Db = torch.empty(3 * b.ndense, dtype=torch.float64, device=dev)
+    Qb = torch.empty(6 * b.ndense, dtype=torch.float64, device=dev)
+    views = zip(b.molecule_views(P), b.molecule_views(W), b.molecule_views(Sb),
+                b.molecule_views(Db, 3), b.molecule_views(Qb, 6))
+    for m, (pv, wv, sv, dv, qv) in enumerate(views):
+        ao = slice(int(b.ao_off[m]), int(b.ao_off[m + 1]))
+        v = vao[ao]
+        p = pv[:, :, 0]  # [nu, mu] = P(mu, nu)
+        sv[:, :, 0] = -wv[:, :, 0] - 0.5 * p * (v[None, :] + v[:, None])
+        at = ao_atom[ao]
+        dv.copy_(-p[:, :, None] * vdp.view(-1, 3)[at][:, None, :])  # atom of column nu
+        qv.copy_(-p[:, :, None] * vqp.view(-1, 6)[at][:, None, :])
+    torch.cuda.synchronize()
+    outs = (torch.empty(b.ndense, dtype=torch.float64, device=dev),
+            torch.empty(3 * b.ndense, dtype=torch.float64, device=dev),
+            torch.empty(6 * b.ndense, dtype=torch.float64, device=dev),
+            torch.empty(b.ndense, dtype=torch.float64, device=dev))
+    gouts = (torch.empty(b.nat, dtype=torch.float64, device=dev),
+             torch.empty(3 * b.nat, dtype=torch.float64, device=dev),
+             torch.empty(9 * b.nmol, dtype=torch.float64, device=dev))
+    log(f"inputs ready; Kc={kc:.3e} (off-site {kc_off:.3e})")
+
+    def step_autograd():
+        x = xyz.detach().requires_grad_(True)
+        s = se.detach().requires_grad_(True)
+        o = h0_integrals(b, x, s)
+        torch.autograd.backward(o, (Sb, Db, Qb, Hb))
+        return x.grad, s.grad
+
+    def step_plain():
+        b.forward(xyz, se, out=outs)
+        return b.gradient(xyz, se, dsedcn, P, W, vao, vdp, vqp, out=gouts)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        fwd, bwd = [], []
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+            torch.cuda.synchronize()
+            tm = b.last_timing()
+            fwd.append(tm["forward_ms"])
+            bwd.append(tm["adjoint_ms"] if fn is step_autograd else tm["gradient_ms"])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        return el, float(np.mean(fwd)), float(np.mean(bwd))
+
+    el_a, fwd_a, adj_a = timed(step_autograd, args.steps, args.warmup)
+    log(f"autograd: {el_a / args.steps * 1e3:.2f} ms/step (forward {fwd_a:.2f} ms, "
+        f"adjoint {adj_a:.2f} ms)")
+    el_p, fwd_p, grad_p = timed(step_plain, args.steps, args.warmup)
+    log(f"plain batched: {el_p / args.steps * 1e3:.2f} ms/step (forward {fwd_p:.2f} ms, "
+        f"gradient {grad_p:.2f} ms)")
+
+    # agreement of the two batched gradients (autograd vs plain kernel)
+    gx, _ = step_autograd()
+    _, gp, _ = step_plain()
+    torch.cuda.synchronize()
+    agree = float((gx - gp).abs().max())
+
+    # plain per-molecule GPU path (drop-in host-pointer entry points) on a sample
+    nsample = min(64, len(mols))
+    rng = np.random.default_rng(3)
+    t = 0.0
+    for z, x in mols[:nsample]:
+        st = structure.setup(structure.Structure(z, x))
+        nao_m, nat_m = st.bas.nao, st.mol.nat
+        sem, dsem = st.selfenergy(rng.uniform(0.5, 3.5, nat_m))
+        a = rng.normal(size=(nao_m, nao_m))
+        Pm = np.asfortranarray((0.5 * (a + a.T))[:, :, None])
+        t1 = time.perf_counter()
+        h0mod.get_hamiltonian(st, sem)
+        h0mod.get_hamiltonian_gradient(st, sem, dsem, np.zeros((nao_m, 1), order="F"),
+                                       np.zeros((3, nat_m, 1), order="F"),
+                                       np.zeros((6, nat_m, 1), order="F"), Pm, 0.05 * Pm,
+                                       np.zeros(nat_m), np.zeros((3, nat_m), order="F"),
+                                       np.zeros((3, 3), order="F"))
+        t += time.perf_counter() - t1
+    plain_rate = nsample / t
+
+    ms_step = el_a / args.steps * 1e3
+    value = b.nmol / (el_a / args.steps)
+    # dominant kernel of the autograd step
+    fwd_bytes = roofline.BYTES_FWD_PER_ELEM * b.ndense
+    adj_flop = roofline.FLOP_BWD_PER_KC * kc
+    if adj_a >= fwd_a:
+        ach = adj_flop / (adj_a * 1e-3) / 1e12
+        roof = roofline.roofline_entry("mfma", ach, roofline.PEAK_FP64_TFLOPS, "TFLOP/s")
+        roof["kernel"] = "k_gradient<adjoint> (FP64 vector)"
+        roof["kernel_ms"] = adj_a
+    else:
+        ach = fwd_bytes / (fwd_a * 1e-3) / 1e9
+        roof = roofline.roofline_entry("hbm", ach, roofline.PEAK_HBM_GBS, "GB/s")
+        roof["kernel"] = "k_forward"
+        roof["kernel_ms"] = fwd_a
+    line = {
+        "metric": "molecules/s for GFN2 S/D/Q+H0+gradient, batched-autograd vs plain GPU; "
+                  "% FP64 peak",
+        "value": value,
+        "unit": "molecules/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random molecules; random symmetric P, W and potentials)",
+        "config": {"workload": f"batched GFN2 S/D/Q+H0 forward + autograd gradient, "
+                               f"{b.nmol} molecules, 8-256 atoms log-uniform",
+                   "molecules": b.nmol, "atoms": b.nat, "nao": b.nao,
+                   "dense_elements": b.ndense, "kept_prim_x_cart_pairs": kc,
+                   "parallelism": "single GPU"},
+        "batched_plain_molecules_per_s": b.nmol / (el_p / args.steps),
+        "plain_gpu_molecules_per_s": plain_rate,
+        "plain_gpu_sample": f"first {nsample} molecules, host-pointer get_hamiltonian + "
+                            f"get_hamiltonian_gradient (PCIe copies included)",
+        "kernel_ms": {"forward": fwd_a, "adjoint": adj_a, "gradient": grad_p},
+        "forward_hbm_GBs": fwd_bytes / (fwd_a * 1e-3) / 1e9,
+        "adjoint_fp64_TFLOPs": adj_flop / (adj_a * 1e-3) / 1e12,
+        "gradient_fp64_TFLOPs": adj_flop / (grad_p * 1e-3) / 1e12,
+        "autograd_vs_plain_max_abs_diff": agree,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

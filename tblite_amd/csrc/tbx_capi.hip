@@ -713,5 +713,6 @@ extern "C" int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, dou
     if (ms_pairs) *ms_pairs = 0.0;
     if (ms_gradient) *ms_gradient = el(4, 5);
     if (ms_adjoint) *ms_adjoint = el(6, 7);
+    (void)hipGetLastError();  // a query of a never-recorded event is not an error here
     return TBX_OK;
 }

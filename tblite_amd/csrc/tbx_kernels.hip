@@ -731,6 +731,7 @@ extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mo
                                          const int *nnl, int *flag, hipStream_t stream)
 {
     FwdArgs a{*sp, *mol, xyz, se, order, S, D, Q, H, nnl, flag};
+    (void)hipGetLastError();  // launch errors are read below; drop stale ones
     hipLaunchKernelGGL(k_forward, dim3(mol->nat), dim3(NTHREADS), 0, stream, a);
     return hipGetLastError();
 }
@@ -747,6 +748,7 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
 {
     GradArgs a{*sp, *mol, xyz, se, order, P, W, vao, vdp, vqp, nspin, pstride, vstride,
                Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag};
+    (void)hipGetLastError();
     if (mode == GRAD_PLAIN)
         hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nat), dim3(NTHREADS), 0, stream, a);
     else
@@ -759,6 +761,7 @@ extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
                                        double *dEdcn, hipStream_t stream)
 {
     CnArgs a{*sp, *mol, dse, dsedcn, P, dEdcn};
+    (void)hipGetLastError();
     const int nb = (mol->nat + 255) / 256;
     hipLaunchKernelGGL(k_dedcn, dim3(nb), dim3(256), 0, stream, a);
     return hipGetLastError();
@@ -776,6 +779,7 @@ extern "C" hipError_t tbx_launch_fill(double *p, long long n, double v, hipStrea
 {
     if (n <= 0) return hipSuccess;
     const long long nb = (n + 255) / 256;
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(256), 0, stream, p, n, v);
     return hipGetLastError();
 }

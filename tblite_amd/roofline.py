@@ -1,0 +1,86 @@
+"""Algorithmic work of the integral path, for roofline reporting.
+
+K is the set of kept primitive pairs -- every shell pair of every half-list
+atom pair (``adjlist.f90:125-129``) plus every on-site shell pair, screened
+with ``ai*aj*r2/(ai+aj) <= intcut`` (``integral/native/integrals.f90:1211-1215``)
+-- and c = ncart(l_i)*ncart(l_j).  Per kept primitive pair and Cartesian pair:
+
+- forward  (S, D, Q):                40 FLOP   (10 moments x (2 mul for the
+  x*y*z product + 2 for the weighted accumulate))
+- gradient / adjoint (S, D, Q and their R derivatives, contracted): 160 FLOP
+  (40 quantities x 4)
+
+Bytes: the forward writes S, H0, D, Q densely (88 bytes per nao^2 element);
+the plain gradient reads P and W (16 B), the adjoint reads the four dense
+adjoints (88 B).  Setup, spherical transform and scatter are not counted, so
+a fraction of peak derived from these figures is a lower bound.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FLOP_FWD_PER_KC = 40
+FLOP_BWD_PER_KC = 160
+BYTES_FWD_PER_ELEM = 88
+BYTES_GRAD_PER_ELEM = 16
+BYTES_ADJ_PER_ELEM = 88
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
+
+
+def _ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def count_kc(spec, ids_per_mol, xyz_per_mol, cutoff, intcut=25.0):
+    """Return (Kc_offsite, Kc_onsite): sums over kept primitive pairs of
+    ncart_i*ncart_j for the half-list pairs and for the on-site blocks."""
+    nsp = len(spec.nums)
+    # thresholds: kept iff r2 <= intcut*(ai+aj)/(ai*aj)
+    shells = [[(c.ang, np.asarray(c.alpha)) for c in spec.basis_cgto[s]] for s in range(nsp)]
+    thr = {}
+    onsite = np.zeros(nsp)
+    for si in range(nsp):
+        for sj in range(nsp):
+            lst = []
+            for li, ai in shells[si]:
+                for lj, aj in shells[sj]:
+                    t = (intcut * (ai[:, None] + aj[None, :]) / (ai[:, None] * aj[None, :])).ravel()
+                    lst.append((np.sort(t), _ncart(li) * _ncart(lj)))
+            thr[(si, sj)] = lst
+        onsite[si] = sum(len(ai) * len(aj) * _ncart(li) * _ncart(lj)
+                         for li, ai in shells[si] for lj, aj in shells[si])
+    cut2 = cutoff * cutoff
+    r2_by = {k: [] for k in thr}
+    kc_on = 0.0
+    for ids, xyz in zip(ids_per_mol, xyz_per_mol):
+        ids = np.asarray(ids)
+        x = np.asarray(xyz).reshape(-1, 3)
+        n = len(ids)
+        kc_on += onsite[ids].sum()
+        if n < 2:
+            continue
+        iu, ju = np.triu_indices(n, 1)
+        d = x[iu] - x[ju]
+        r2 = np.einsum("ij,ij->i", d, d)
+        keep = r2 <= cut2
+        iu, ju, r2 = iu[keep], ju[keep], r2[keep]
+        code = ids[iu] * nsp + ids[ju]
+        for c in np.unique(code):
+            sel = code == c
+            r2_by[(int(c) // nsp, int(c) % nsp)].append(r2[sel])
+    kc_off = 0.0
+    for key, chunks in r2_by.items():
+        if not chunks:
+            continue
+        r2 = np.concatenate(chunks)
+        for t, nc in thr[key]:
+            kept = len(t) * len(r2) - np.searchsorted(t, r2, side="left").sum()
+            kc_off += nc * float(kept)
+    return kc_off, kc_on
+
+
+def roofline_entry(bound, achieved, peak, unit, traffic=None):
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": traffic}

@@ -214,11 +214,23 @@ def random_molecule(nat: int, rng) -> tuple:
     return np.array(zs, dtype=np.int32), pos * AATOAU
 
 
-def molecule_batch(nmol: int = 2048, nat_min: int = 8, nat_max: int = 256, seed: int = 1004):
-    """``nmol`` random molecules with log-uniform atom counts in [nat_min, nat_max]."""
+def _one_molecule(args):
+    seed, nat_min, nat_max = args
     rng = np.random.default_rng(seed)
-    mols = []
-    for _ in range(nmol):
-        nat = int(round(math.exp(rng.uniform(math.log(nat_min), math.log(nat_max)))))
-        mols.append(random_molecule(nat, rng))
-    return mols
+    nat = int(round(math.exp(rng.uniform(math.log(nat_min), math.log(nat_max)))))
+    return random_molecule(nat, rng)
+
+
+def molecule_batch(nmol: int = 2048, nat_min: int = 8, nat_max: int = 256, seed: int = 1004,
+                   workers: int = 0):
+    """``nmol`` random molecules with log-uniform atom counts in [nat_min, nat_max].
+
+    Molecule k uses the k-th child of ``SeedSequence(seed)``, so the batch is
+    the same whether it is built serially or with ``workers`` processes."""
+    seeds = np.random.SeedSequence(seed).spawn(nmol)
+    args = [(sq, nat_min, nat_max) for sq in seeds]
+    if workers and workers > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            return list(ex.map(_one_molecule, args, chunksize=16))
+    return [_one_molecule(a) for a in args]
