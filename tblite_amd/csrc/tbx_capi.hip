@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -98,8 +99,9 @@ hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s)
 
 struct HostSpecies {
     int nsp = 0, msh = 0, maxnc = 0;
-    std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao;
-    std::vector<double> alpha, coeff, shpoly, rad, hscale;
+    std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao,
+        prim_off, pp_off;
+    std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp;
     double intcut = 25.0;
 };
 
@@ -171,6 +173,55 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
     }
     // the kernels' windows hold at least one atom: (tasks per window)/nc >= nc
     if (hs.maxnc > 16) return fail(TBX_ERR_BASIS, "%d Cartesian functions per atom (max 16)", hs.maxnc);
+    // packed primitive list, staged whole in LDS by the gradient kernels
+    hs.prim_off.assign((size_t)nsp * msh, 0);
+    hs.prim.clear();
+    for (int isp = 0; isp < nsp; ++isp)
+        for (int ish = 0; ish < hs.nsh[isp]; ++ish) {
+            const int slot = ish + msh * isp;
+            hs.prim_off[slot] = (int)(hs.prim.size() / 2);
+            for (int p = 0; p < hs.nprim[slot]; ++p) {
+                hs.prim.push_back(hs.alpha[TBX_MAXG * slot + p]);
+                hs.prim.push_back(hs.coeff[TBX_MAXG * slot + p]);
+            }
+        }
+    if (hs.prim.size() / 2 > TBX_MAXPRIM)
+        return fail(TBX_ERR_BASIS, "%zu primitives over all species (max %d)", hs.prim.size() / 2,
+                    TBX_MAXPRIM);
+    // primitive pairs of every slot pair, sorted by the reduced exponent so
+    // that the pairs kept by the screening ai aj r2/(ai+aj) <= intcut form a prefix
+    const int nslot = nsp * msh;
+    hs.pp_off.assign((size_t)nslot * nslot + 1, 0);
+    hs.pp.clear();
+    std::vector<std::array<double, TBX_PP_N>> ent;
+    for (int si = 0; si < nslot; ++si)
+        for (int sj = 0; sj < nslot; ++sj) {
+            hs.pp_off[(size_t)si * nslot + sj] = (int)(hs.pp.size() / TBX_PP_N);
+            const bool vi = (si % msh) < hs.nsh[si / msh], vj = (sj % msh) < hs.nsh[sj / msh];
+            if (!vi || !vj) continue;
+            ent.clear();
+            for (int ip = 0; ip < hs.nprim[si]; ++ip)
+                for (int jp = 0; jp < hs.nprim[sj]; ++jp) {
+                    const double ai = hs.alpha[TBX_MAXG * si + ip], aj = hs.alpha[TBX_MAXG * sj + jp];
+                    const double oab = 1.0 / (ai + aj);
+                    std::array<double, TBX_PP_N> e;
+                    e[TBX_PP_MU] = ai * aj * oab;
+                    e[TBX_PP_AJO] = aj * oab;
+                    e[TBX_PP_AIO] = ai * oab;
+                    e[TBX_PP_H] = 0.5 * oab;
+                    e[TBX_PP_AI] = ai;
+                    e[TBX_PP_PRE] = 5.5683279968317078453 * oab * std::sqrt(oab) *
+                                    hs.coeff[TBX_MAXG * si + ip] * hs.coeff[TBX_MAXG * sj + jp];
+                    ent.push_back(e);
+                }
+            std::stable_sort(ent.begin(), ent.end(),
+                             [](const std::array<double, TBX_PP_N> &a,
+                                const std::array<double, TBX_PP_N> &b) {
+                                 return a[TBX_PP_MU] < b[TBX_PP_MU];
+                             });
+            for (const auto &e : ent) hs.pp.insert(hs.pp.end(), e.begin(), e.end());
+        }
+    hs.pp_off[(size_t)nslot * nslot] = (int)(hs.pp.size() / TBX_PP_N);
     return TBX_OK;
 }
 
@@ -247,7 +298,7 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
 // Device copies of the tables
 struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
-        ao_m, sh_cart, sh_ao;
+        ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
         cart_atom, ao_atom, mol_ao, order;
     TbxSpecies sp{};
@@ -256,7 +307,9 @@ struct DevTables {
     void release()
     {
         DevBuf *all[] = {&nsh, &ang, &nprim, &alpha, &coeff, &shpoly, &rad, &hscale, &ncart, &nao,
-                         &cart_sh, &cart_exp, &ao_sh, &ao_m, &sh_cart, &sh_ao, &trans, &mol_atom,
+                         &cart_sh, &cart_exp, &ao_sh, &ao_m, &sh_cart, &sh_ao, &prim, &prim_off,
+                         &pp, &pp_off,
+                         &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
                          &at_cart, &cart_atom, &ao_atom, &mol_ao, &order};
         for (DevBuf *b : all) b->release();
@@ -268,7 +321,8 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     hipError_t e;
 #define UP(name) if ((e = upload(d.name, hs.name, s)) != hipSuccess) return e
     UP(nsh); UP(ang); UP(nprim); UP(alpha); UP(coeff); UP(shpoly); UP(rad); UP(hscale); UP(ncart);
-    UP(nao); UP(cart_sh); UP(cart_exp); UP(ao_sh); UP(ao_m); UP(sh_cart); UP(sh_ao);
+    UP(nao); UP(cart_sh); UP(cart_exp); UP(ao_sh); UP(ao_m); UP(sh_cart); UP(sh_ao); UP(prim);
+    UP(prim_off); UP(pp); UP(pp_off);
 #undef UP
     TbxSpecies &sp = d.sp;
     sp.nsp = hs.nsp;
@@ -289,6 +343,12 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     sp.ao_m = d.ao_m.as<int>();
     sp.sh_cart = d.sh_cart.as<int>();
     sp.sh_ao = d.sh_ao.as<int>();
+    sp.prim = d.prim.as<double>();
+    sp.prim_off = d.prim_off.as<int>();
+    sp.nprim_tot = (int)(hs.prim.size() / 2);
+    sp.nslot = hs.nsp * hs.msh;
+    sp.pp_off = d.pp_off.as<int>();
+    sp.pp = d.pp.as<double>();
     sp.intcut = hs.intcut;
     sp.cutoff2 = cutoff * cutoff;
     return hipSuccess;

@@ -406,290 +406,584 @@ struct GradArgs {
     int *flag;
 };
 
-template <int MODE>
-__global__ void __launch_bounds__(NTHREADS) k_gradient(GradArgs A)
+// Cartesian exponents (CCA order) of the functions of a shell, l <= 2.
+static constexpr int kCartExp[3][6][3] = {
+    {{0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
+
+__device__ __forceinline__ constexpr int ncart_of(int l) { return (l + 1) * (l + 2) / 2; }
+
+// c[0..N-1] = coefficients of (y + p)^n, runtime n in [0, N-1].
+template <int N>
+__device__ __forceinline__ void pow_poly(int n, double p, double *c)
 {
-    __shared__ double accJ[TW_GRAD][3 + MAXSH];  // per window atom: dG (3), dse of its shells
-    __shared__ double accI[3 + MAXSH + 9];       // atom i: dG, dse, sigma
+    c[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < N; ++k) c[k] = 0.0;
+#pragma unroll
+    for (int r = 0; r < N - 1; ++r) {
+        const bool on = r < n;
+#pragma unroll
+        for (int k = N - 1; k >= 1; --k) c[k] = on ? c[k - 1] + p * c[k] : c[k];
+        c[0] = on ? p * c[0] : c[0];
+    }
+}
+
+// Gradient 1D factors with compile-time degree bounds (a <= LI, b <= LJ),
+// moments about the centre A of function i:
+//   V[m] = int (y+pa)^(a+m) (y+pb)^b,   G[m] = d/dA_x V[m]
+// (function i and the moment operator both move with A).
+template <int LI, int LJ>
+__device__ __forceinline__ void grad1d_t(int a, int b, double pa, double pb, double ai,
+                                         const Mom &M, double V[3], double G[3])
+{
+    double A0[LI + 1], Am[LI + 1], B[LJ + 1], A1[LI + 2], Ga[LI + 2];
+    double P[LI + LJ + 1], PG[LI + LJ + 2];
+    pow_poly<LI + 1>(a, pa, A0);
+    pow_poly<LI + 1>(a > 0 ? a - 1 : 0, pa, Am);
+    pow_poly<LJ + 1>(b, pb, B);
+    A1[0] = pa * A0[0];
+#pragma unroll
+    for (int k = 1; k <= LI; ++k) A1[k] = A0[k - 1] + pa * A0[k];
+    A1[LI + 1] = A0[LI];
+    const double ta = 2.0 * ai, fa = (double)a;
+#pragma unroll
+    for (int k = 0; k <= LI; ++k) Ga[k] = ta * A1[k] - fa * Am[k];
+    Ga[LI + 1] = ta * A1[LI + 1];
+#pragma unroll
+    for (int k = 0; k < LI + LJ + 1; ++k) P[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < LI + LJ + 2; ++k) PG[k] = 0.0;
+#pragma unroll
+    for (int x = 0; x <= LI; ++x)
+#pragma unroll
+        for (int y = 0; y <= LJ; ++y) P[x + y] += A0[x] * B[y];
+#pragma unroll
+    for (int x = 0; x <= LI + 1; ++x)
+#pragma unroll
+        for (int y = 0; y <= LJ; ++y) PG[x + y] += Ga[x] * B[y];
+    mom3<LI + LJ + 1>(P, pa, M, V[0], V[1], V[2]);
+    double g1, g2;
+    mom3<LI + LJ + 2>(PG, pa, M, G[0], g1, g2);
+    G[1] = g1 - V[0];
+    G[2] = g2 - 2.0 * V[1];
+}
+
+// sum over the moments (S, D, Q about A) of w * F_x F_y F_z
+__device__ __forceinline__ double contract_moments(const double *fx, const double *fy,
+                                                   const double *fz, double wS,
+                                                   const double wD[3], const double wQ[6])
+{
+    // pair the y and z factors first
+    const double yz00 = fy[0] * fz[0], yz10 = fy[1] * fz[0], yz01 = fy[0] * fz[1];
+    const double yz20 = fy[2] * fz[0], yz11 = fy[1] * fz[1], yz02 = fy[0] * fz[2];
+    const double t0 = wS * yz00 + wD[1] * yz10 + wD[2] * yz01 + wQ[2] * yz20 + wQ[4] * yz11 +
+                      wQ[5] * yz02;
+    const double t1 = wD[0] * yz00 + wQ[1] * yz10 + wQ[3] * yz01;
+    const double t2 = wQ[0] * yz00;
+    return fx[0] * t0 + fx[1] * t1 + fx[2] * t2;
+}
+
+// packed symmetric index (xx, xy, yy, xz, yz, zz) of the 3x3 element a + 3b
+static constexpr int kSigPack[9] = {0, 1, 3, 1, 2, 4, 3, 4, 5};
+
+// LDS accumulators of the row atom i of a gradient workgroup (one copy per
+// wavefront): dE/dR_i (3), dE/dse of its shells (MAXSH), sigma packed (6)
+#define ACCI_N (3 + MAXSH + 6)
+#define CHUNK 8    // primitive pairs per work item
+#define NREC 23    // doubles of a task record
+
+// Task record fields (LDS, structure of arrays over the tasks of a window)
+enum {
+    R_W = 0,     // 10 contraction weights about centre i: S, D (3), Q (6)
+    R_V = 10,    // v = R_i - R_j - T (3)
+    R_EX = 13,   // weights of S in the explicit v-derivative (3)
+    R_WQ = 16,   // traceless-folded Q weights of the j-centred moments (6)
+    R_HSW = 22,  // dE/dse_i = dE/dse_j per unit S
+};
+
+// inclusive prefix sum over the 64 lanes of a wavefront
+__device__ __forceinline__ int wave_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
+    }
+    return v;
+}
+
+// Number of leading entries of a sorted primitive-pair list kept at r2.
+__device__ __forceinline__ int kept_pairs(const double *pp, int n, double r2, double intcut)
+{
+    int lo = 0, hi = n;  // invariant: entries < lo kept, entries >= hi dropped
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pp[TBX_PP_N * mid + TBX_PP_MU] * r2 <= intcut) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Task set-up: one Cartesian function pair (ki of shell ish on atom i, kj of
+// shell jsh on atom j) at lattice vector itr.  Folds the spherical weights of
+// the pair's block into Cartesian ones (transpose of the spherical map),
+// shifts the j-centred multipole weights to centre i and stores the record.
+// Returns the number of primitive pairs kept by the screening.
+template <int MODE>
+__device__ int task_setup(const GradArgs &A, int i, int j, int itr, int ish, int jsh, int ki,
+                          int kj, double *rec, int ld, int &ppoff)
+{
+    const TbxSpecies &sp = A.sp;
+    const TbxMols &mt = A.mol;
+    const int spi = mt.at_sp[i], spj = mt.at_sp[j];
+    const int sloti = ish + sp.msh * spi, slotj = jsh + sp.msh * spj;
+    const int li = sp.ang[sloti], lj = sp.ang[slotj];
+    const double vx = A.xyz[3 * i] - A.xyz[3 * j] - mt.trans[3 * itr];
+    const double vy = A.xyz[3 * i + 1] - A.xyz[3 * j + 1] - mt.trans[3 * itr + 1];
+    const double vz = A.xyz[3 * i + 2] - A.xyz[3 * j + 2] - mt.trans[3 * itr + 2];
+    const double r2 = vx * vx + vy * vy + vz * vz;
+    if (r2 < kEps || r2 > sp.cutoff2) return 0;
+    const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
+    const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
+    const int kept = kept_pairs(sp.pp + (long long)TBX_PP_N * pi0, npp, r2, sp.intcut);
+    ppoff = pi0;
+    if (kept == 0) return 0;
+
+    const int m = mt.at_mol[i];
+    const int nao = mt.mol_nao[m];
+    const long long dense0 = mt.mol_dense[m];
+    const int ao_m = mt.mol_ao[m];
+    const int muj0 = mt.at_ao[j] + sp.sh_ao[slotj];
+    const int nui0 = mt.at_ao[i] + sp.sh_ao[sloti];
+    const int na = kCartN[lj][kj], nb = kCartN[li][ki];
+    double wS = 0.0, wH = 0.0;
+    double wDi[3] = {0, 0, 0}, wDj[3] = {0, 0, 0};
+    double wQi[6] = {0, 0, 0, 0, 0, 0}, wQj[6] = {0, 0, 0, 0, 0, 0};
+    if (MODE == GRAD_PLAIN) {
+        double pt = 0.0;
+        for (int a = 0; a < na; ++a) {
+            const int mu = muj0 + kCartM[lj][kj][a];
+            for (int b = 0; b < nb; ++b) {
+                const int nu = nui0 + kCartM[li][ki][b];
+                const double w = kCartW[lj][kj][a] * kCartW[li][ki][b];
+                const long long e1 = dense0 + (long long)nu * nao + mu;  // (mu, nu)
+                double vs = 0.0;
+                for (int sgn = 0; sgn < A.nspin; ++sgn)
+                    vs += A.P[e1 + sgn * A.pstride] *
+                          (A.vao[ao_m + mu + sgn * A.vstride] + A.vao[ao_m + nu + sgn * A.vstride]);
+                wS += w * (-vs - 2.0 * A.W[e1]);
+                pt += w * A.P[e1];
+            }
+        }
+        // the multipole-potential weights factorise: -P x (per-atom vector)
+        wH = 2.0 * pt;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            wDi[k] = -pt * A.vdp[3 * i + k];
+            wDj[k] = -pt * A.vdp[3 * j + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            wQi[k] = -pt * A.vqp[6 * i + k];
+            wQj[k] = -pt * A.vqp[6 * j + k];
+        }
+    } else {
+        for (int a = 0; a < na; ++a) {
+            const int mu = muj0 + kCartM[lj][kj][a];
+            for (int b = 0; b < nb; ++b) {
+                const int nu = nui0 + kCartM[li][ki][b];
+                const double w = kCartW[lj][kj][a] * kCartW[li][ki][b];
+                const long long e1 = dense0 + (long long)nu * nao + mu;  // (mu, nu)
+                const long long e2 = dense0 + (long long)mu * nao + nu;  // (nu, mu)
+                wS += w * (A.Sb[e1] + A.Sb[e2]);
+                wH += w * (A.Hb[e1] + A.Hb[e2]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    wDi[k] += w * A.Db[3 * e1 + k];
+                    wDj[k] += w * A.Db[3 * e2 + k];
+                }
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    wQi[k] += w * A.Qb[6 * e1 + k];
+                    wQj[k] += w * A.Qb[6 * e2 + k];
+                }
+            }
+        }
+    }
+    // quadrupoles are contracted after trace removal: fold the map into the weights
+    {
+        const double ti = 0.5 * (wQi[0] + wQi[2] + wQi[5]);
+        const double tj = 0.5 * (wQj[0] + wQj[2] + wQj[5]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            wQi[k] *= 1.5;
+            wQj[k] *= 1.5;
+        }
+        wQi[0] -= ti; wQi[2] -= ti; wQi[5] -= ti;
+        wQj[0] -= tj; wQj[2] -= tj; wQj[5] -= tj;
+    }
+    // Moments about j are shifted moments about i (x - B = (x - A) + v, v = A - B):
+    //   D_j = D_i + v S,   Q_j = Q_i + v D_i^T + D_i v^T + v v^T S,
+    // so the j-centred weights fold into i-centred ones plus an explicit v
+    // dependence, and only i-centred 1D factors are needed.
+    const double Wv[3] = {wQj[0] * vx + 0.5 * (wQj[1] * vy + wQj[3] * vz),
+                          wQj[2] * vy + 0.5 * (wQj[1] * vx + wQj[4] * vz),
+                          wQj[5] * vz + 0.5 * (wQj[3] * vx + wQj[4] * vy)};
+    // H0 = S * (se_i + se_j)/2 * hscale * shpoly(R); shpoly's R dependence is explicit
+    const int spi_ = spi, spj_ = spj;
+    const double hsc = sp.hscale[jsh + sp.msh * (ish + sp.msh * (spj_ + sp.nsp * spi_))];
+    const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
+    const double rr = sqrt(sqrt(r2) / (sp.rad[spi] + sp.rad[spj]));
+    const double shi = 1.0 + kpi * rr, shj = 1.0 + kpj * rr;
+    const double shp = shi * shj;
+    const double dshp = (shi * kpj + shj * kpi) * 0.5 * rr / r2;  // (d shp / d r) / r
+    const double hs = hsc * shp;
+    const double sesum = 0.5 * (A.se[mt.at_sh[i] + ish] + A.se[mt.at_sh[j] + jsh]);
+    const double hw = sesum * hs * wH;
+    rec[(R_W + 0) * ld] = wS + hw + wDj[0] * vx + wDj[1] * vy + wDj[2] * vz + Wv[0] * vx +
+                          Wv[1] * vy + Wv[2] * vz;
+    rec[(R_W + 1) * ld] = wDi[0] + wDj[0] + 2.0 * Wv[0];
+    rec[(R_W + 2) * ld] = wDi[1] + wDj[1] + 2.0 * Wv[1];
+    rec[(R_W + 3) * ld] = wDi[2] + wDj[2] + 2.0 * Wv[2];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec[(R_W + 4 + k) * ld] = wQi[k] + wQj[k];
+    rec[(R_V + 0) * ld] = vx;
+    rec[(R_V + 1) * ld] = vy;
+    rec[(R_V + 2) * ld] = vz;
+    const double hd = hw * dshp / shp;
+    rec[(R_EX + 0) * ld] = wDj[0] + 2.0 * Wv[0] + hd * vx;
+    rec[(R_EX + 1) * ld] = wDj[1] + 2.0 * Wv[1] + hd * vy;
+    rec[(R_EX + 2) * ld] = wDj[2] + 2.0 * Wv[2] + hd * vz;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rec[(R_WQ + k) * ld] = wQj[k];
+    rec[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
+    return kept;
+}
+
+// Primitive pairs [p0, p1) of one task of type (LI, LJ): accumulate the
+// contracted derivative a3 = sum cc w.dM/dR_i, the overlap s and the i-centred
+// dipole moments dacc.
+template <int LI, int LJ>
+__device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const double *rec, int ld,
+                                          int ki, int kj, int p0, int p1, double a3[3],
+                                          double &s, double dacc[3])
+{
+    const int ei[3] = {kCartExp[LI][ki][0], kCartExp[LI][ki][1], kCartExp[LI][ki][2]};
+    const int ej[3] = {kCartExp[LJ][kj][0], kCartExp[LJ][kj][1], kCartExp[LJ][kj][2]};
+    double w[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) w[k] = rec[(R_W + k) * ld];
+    const double vec[3] = {rec[R_V * ld], rec[(R_V + 1) * ld], rec[(R_V + 2) * ld]};
+    const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    for (int p = p0; p < p1; ++p) {
+        const double *e = pp + TBX_PP_N * p;
+        const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
+        const double h = e[TBX_PP_H], ai = e[TBX_PP_AI], pre = e[TBX_PP_PRE];
+        const double cc = exp(-(mu * r2)) * pre;
+        const Mom M = make_mom(h);
+        double V[3][3], G[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            grad1d_t<LI, LJ>(ei[k], ej[k], -vec[k] * ajo, vec[k] * aio, ai, M, V[k], G[k]);
+        s += cc * (V[0][0] * V[1][0] * V[2][0]);
+        dacc[0] += cc * (V[0][1] * V[1][0] * V[2][0]);
+        dacc[1] += cc * (V[0][0] * V[1][1] * V[2][0]);
+        dacc[2] += cc * (V[0][0] * V[1][0] * V[2][1]);
+        a3[0] += cc * contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
+        a3[1] += cc * contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
+        a3[2] += cc * contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
+    }
+}
+
+// Workgroup of atom i.  For every lattice vector, the atoms j <= i of the
+// molecule are walked in windows.  In a window: (1) the shell pairs are laid
+// out by type (l_i, l_j), (2) one lane per Cartesian-pair task builds the
+// task record and counts its kept primitive pairs, (3) the kept pairs are cut
+// into chunks of CHUNK and a block scan turns them into a dense list of work
+// items, (4) every lane runs work items (type-major, so a wavefront mostly
+// runs one specialised code path) and adds its results to the LDS sums of
+// atoms i and j, (5) the window's atom-j sums go to HBM with FP64 atomics.
+template <int MODE>
+__global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
+{
+    __shared__ double rec[NREC * TW_GRAD];
+    __shared__ double accJ[TW_GRAD][3 + MAXSH];
+    __shared__ double accI[NTHREADS / 64][ACCI_N];
+    __shared__ int4 spl[TW_GRAD];          // (j, jsh, ish, type), type-major
+    __shared__ int tinfo[TW_GRAD];         // type | ki << 4 | kj << 8 | jw << 12
+    __shared__ int tpp[TW_GRAD];           // first primitive pair of the task
+    __shared__ int tkept[TW_GRAD];
+    __shared__ int coff[TW_GRAD + 1];      // first chunk of each task
+    __shared__ int wsum[NTHREADS / 64][9];
+    __shared__ int toff[10], tsp[10];      // task / shell-pair offsets per type
     __shared__ int s_count;
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int i = A.order[blockIdx.x];
     const int m = mt.at_mol[i];
     const int a0 = mt.mol_atom[m];
-    const int nao = mt.mol_nao[m];
     const int spi = mt.at_sp[i];
     const int nci = sp.ncart[spi];
-    const double xi = A.xyz[3 * i], yi = A.xyz[3 * i + 1], zi = A.xyz[3 * i + 2];
-    const long long dense0 = mt.mol_dense[m];
+    const int nshi = sp.nsh[spi];
     const int cart_m = mt.mol_cart[m];
-    const int ao_m = mt.mol_ao[m];
-    const int iao0 = mt.at_ao[i];
     const int rows_cap = TW_GRAD / nci;
-    // half list: j < i, plus j == i for lattice self-images; the adjoint also
-    // visits the on-site block (j == i, T = 0) for dE/dse.
-    const bool with_self = (mt.ntrans > 1) || (MODE == GRAD_ADJOINT);
-    const int jend = with_self ? i + 1 : i;
+    // half list: j < i, plus j == i for the lattice self-images
+    const int jend = (mt.ntrans > 1) ? i + 1 : i;
     const int cend = (jend > a0) ? (mt.at_cart[jend - 1] + sp.ncart[mt.at_sp[jend - 1]]) : 0;
-    for (int k = threadIdx.x; k < 3 + MAXSH + 9; k += NTHREADS) accI[k] = 0.0;
-    if (threadIdx.x == 0) s_count = 0;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) s_count = 0;
+    for (int k = tid; k < (NTHREADS / 64) * ACCI_N; k += NTHREADS) (&accI[0][0])[k] = 0.0;
+    __syncthreads();
 
-    int c0 = 0;
-    while (c0 < cend) {
-        const int jfirst = mt.cart_atom[cart_m + c0];
-        int lo = jfirst, hi = jend;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            int end = mt.at_cart[mid] + sp.ncart[mt.at_sp[mid]];
-            if (end <= c0 + rows_cap) lo = mid; else hi = mid;
+    // on-site block (adjoint only): H = S (se_i + se_j)/2 gives dE/dse only
+    if (MODE == GRAD_ADJOINT && tid < nci * nci) {
+        const int ci = tid / nci, cj = tid - ci * nci;
+        const int ei_ = sp.cart_exp[spi * TBX_MAXCART + ci];
+        const int ej_ = sp.cart_exp[spi * TBX_MAXCART + cj];
+        const int ish = sp.cart_sh[spi * TBX_MAXCART + ci];
+        const int jsh = sp.cart_sh[spi * TBX_MAXCART + cj];
+        const int ki = ei_ >> 12, kj = ej_ >> 12;
+        const int sloti = ish + sp.msh * spi, slotj = jsh + sp.msh * spi;
+        const int li = sp.ang[sloti], lj = sp.ang[slotj];
+        const int nao = mt.mol_nao[m];
+        const long long dense0 = mt.mol_dense[m];
+        const int muj0 = mt.at_ao[i] + sp.sh_ao[slotj];
+        const int nui0 = mt.at_ao[i] + sp.sh_ao[sloti];
+        double wHon = 0.0;
+        for (int a = 0; a < kCartN[lj][kj]; ++a)
+            for (int b = 0; b < kCartN[li][ki]; ++b)
+                wHon += kCartW[lj][kj][a] * kCartW[li][ki][b] *
+                        A.Hb[dense0 + (long long)(nui0 + kCartM[li][ki][b]) * nao + muj0 +
+                             kCartM[lj][kj][a]];
+        const int e0 = (ei_ & 15) + (ej_ & 15), e1 = ((ei_ >> 4) & 15) + ((ej_ >> 4) & 15),
+                  e2 = ((ei_ >> 8) & 15) + ((ej_ >> 8) & 15);
+        const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
+        const int pi1 = sp.pp_off[sloti * sp.nslot + slotj + 1];
+        double s = 0.0;
+        for (int p = pi0; p < pi1; ++p) {
+            const Mom M = make_mom(sp.pp[TBX_PP_N * p + TBX_PP_H]);
+            s += sp.pp[TBX_PP_N * p + TBX_PP_PRE] * mom(M, e0) * mom(M, e1) * mom(M, e2);
         }
-        const int jlast = lo;
-        const int c1 = mt.at_cart[jlast] + sp.ncart[mt.at_sp[jlast]];
-        const int ntask = (c1 - c0) * nci;
-        const int nwin = jlast - jfirst + 1;
-        __syncthreads();
-        for (int k = threadIdx.x; k < nwin * (3 + MAXSH); k += NTHREADS)
-            accJ[k / (3 + MAXSH)][k % (3 + MAXSH)] = 0.0;
-        __syncthreads();
+        if (wHon * s != 0.0) {
+            atomicAdd(&accI[wv][3 + ish], 0.5 * wHon * s);
+            atomicAdd(&accI[wv][3 + jsh], 0.5 * wHon * s);
+        }
+    }
 
-        for (int t = threadIdx.x; t < ntask; t += NTHREADS) {
-            const int cjl = t / nci, ci = t - cjl * nci;
-            const int j = mt.cart_atom[cart_m + c0 + cjl];
-            const int spj = mt.at_sp[j];
-            const int cj = c0 + cjl - mt.at_cart[j];
-            const int slj = sp.cart_sh[spj * TBX_MAXCART + cj];
-            const int sli = sp.cart_sh[spi * TBX_MAXCART + ci];
-            const int expj = sp.cart_exp[spj * TBX_MAXCART + cj];
-            const int expi = sp.cart_exp[spi * TBX_MAXCART + ci];
-            int ej[3], ei[3];
-            unpack_exp(expj, ej);
-            unpack_exp(expi, ei);
-            const int kj = expj >> 12, ki = expi >> 12;  // index within the shell
-            const int slotj = slj + sp.msh * spj, sloti = sli + sp.msh * spi;
-            const int lj = sp.ang[slotj], li = sp.ang[sloti];
+    // shells of atom i per angular momentum
+    int nli[3] = {0, 0, 0};
+    for (int q = 0; q < nshi; ++q) ++nli[sp.ang[q + sp.msh * spi]];
 
-            // Cartesian weights from the spherical ones (transpose of the transform)
-            double wS = 0.0, wH = 0.0, wHon = 0.0, wDi[3] = {0, 0, 0}, wDj[3] = {0, 0, 0};
-            double wQi[6] = {0, 0, 0, 0, 0, 0}, wQj[6] = {0, 0, 0, 0, 0, 0};
-            const int muj0 = mt.at_ao[j] + sp.sh_ao[slotj];
-            const int nui0 = iao0 + sp.sh_ao[sloti];
-            for (int a = 0; a < kCartN[lj][kj]; ++a) {
-                const int mu = muj0 + kCartM[lj][kj][a];
-                for (int b = 0; b < kCartN[li][ki]; ++b) {
-                    const int nu = nui0 + kCartM[li][ki][b];
-                    const double w = kCartW[lj][kj][a] * kCartW[li][ki][b];
-                    const long long e1 = dense0 + (long long)nu * nao + mu;  // (mu, nu)
-                    const long long e2 = dense0 + (long long)mu * nao + nu;  // (nu, mu)
-                    if (MODE == GRAD_PLAIN) {
-                        const double p = A.P[e1];
-                        double vs = 0.0;
-                        for (int s = 0; s < A.nspin; ++s)
-                            vs += A.P[e1 + s * A.pstride] *
-                                  (A.vao[ao_m + mu + s * A.vstride] + A.vao[ao_m + nu + s * A.vstride]);
-                        wS += w * (-vs - 2.0 * A.W[e1]);
-                        wH += w * 2.0 * p;
+    for (int itr = 0; itr < mt.ntrans; ++itr) {
+        int c0 = 0;
+        while (c0 < cend) {
+            const int jfirst = mt.cart_atom[cart_m + c0];
+            int lo = jfirst, hi = jend;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                int end = mt.at_cart[mid] + sp.ncart[mt.at_sp[mid]];
+                if (end <= c0 + rows_cap) lo = mid; else hi = mid;
+            }
+            const int jlast = lo;
+            const int c1 = mt.at_cart[jlast] + sp.ncart[mt.at_sp[jlast]];
+            const int nwin = jlast - jfirst + 1;
+            __syncthreads();
+            for (int k = tid; k < nwin * (3 + MAXSH); k += NTHREADS)
+                accJ[k / (3 + MAXSH)][k % (3 + MAXSH)] = 0.0;
+
+            // (1) shell pairs per type, type-major
+            int cnt[9];
+            const int j = jfirst + tid;
+            int nlj[3] = {0, 0, 0};
+            if (tid < nwin) {
+                const int spj = mt.at_sp[j];
+                for (int q = 0; q < sp.nsh[spj]; ++q) ++nlj[sp.ang[q + sp.msh * spj]];
+            }
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            wDi[k] -= w * p * A.vdp[3 * i + k];
-                            wDj[k] -= w * p * A.vdp[3 * j + k];
-                        }
+            for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
+            int incl[9];
 #pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            wQi[k] -= w * p * A.vqp[6 * i + k];
-                            wQj[k] -= w * p * A.vqp[6 * j + k];
-                        }
-                    } else {
-                        wS += w * (A.Sb[e1] + A.Sb[e2]);
-                        wH += w * (A.Hb[e1] + A.Hb[e2]);
-                        wHon += w * A.Hb[e1];
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            wDi[k] += w * A.Db[3 * e1 + k];
-                            wDj[k] += w * A.Db[3 * e2 + k];
-                        }
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            wQi[k] += w * A.Qb[6 * e1 + k];
-                            wQj[k] += w * A.Qb[6 * e2 + k];
+            for (int t = 0; t < 9; ++t) {
+                incl[t] = wave_scan(cnt[t]);
+                if (lane == 63) wsum[wv][t] = incl[t];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int so = 0, to = 0;
+                for (int t = 0; t < 9; ++t) {
+                    int tot = 0;
+                    for (int w = 0; w < NTHREADS / 64; ++w) tot += wsum[w][t];
+                    tsp[t] = so;
+                    toff[t] = to;
+                    so += tot;
+                    to += tot * ncart_of(t / 3) * ncart_of(t % 3);
+                }
+                tsp[9] = so;
+                toff[9] = to;
+            }
+            __syncthreads();
+            if (tid < nwin) {
+                const int spj = mt.at_sp[j];
+                for (int t = 0; t < 9; ++t) {
+                    if (cnt[t] == 0) continue;
+                    int off = tsp[t] + incl[t] - cnt[t];
+                    for (int w = 0; w < wv; ++w) off += wsum[w][t];
+                    for (int qi = 0; qi < nshi; ++qi) {
+                        if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
+                        for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
+                            if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
+                            spl[off++] = make_int4(j, qj, qi, t);
                         }
                     }
                 }
             }
-            // quadrupoles are contracted after trace removal: fold the
-            // (symmetric) traceless map into the weights
+            __syncthreads();
+
+            // (2) task records; ntask <= TW_GRAD = NTHREADS: one task per lane
+            const int ntask = toff[9];
+            int nch = 0;
+            if (tid < ntask) {
+                int t = 0;
+                while (t < 8 && tid >= toff[t + 1]) ++t;
+                const int ncj = ncart_of(t % 3);
+                const int ncp = ncart_of(t / 3) * ncj;
+                const int local = tid - toff[t];
+                const int4 e = spl[tsp[t] + local / ncp];
+                const int cp = local % ncp;
+                const int ki = cp / ncj, kj = cp - (cp / ncj) * ncj;
+                int ppoff = 0;
+                const int kept =
+                    task_setup<MODE>(A, i, e.x, itr, e.z, e.y, ki, kj, rec + tid, TW_GRAD, ppoff);
+                tinfo[tid] = t | (ki << 4) | (kj << 8) | ((e.x - jfirst) << 12) |
+                             ((e.z) << 24) | ((e.y) << 27);
+                tpp[tid] = ppoff;
+                tkept[tid] = kept;
+                nch = (kept + CHUNK - 1) / CHUNK;
+            }
+            // (3) chunk offsets: block-wide exclusive scan
             {
-                const double ti = 0.5 * (wQi[0] + wQi[2] + wQi[5]);
-                const double tj = 0.5 * (wQj[0] + wQj[2] + wQj[5]);
-#pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    wQi[k] *= 1.5;
-                    wQj[k] *= 1.5;
-                }
-                wQi[0] -= ti; wQi[2] -= ti; wQi[5] -= ti;
-                wQj[0] -= tj; wQj[2] -= tj; wQj[5] -= tj;
+                const int inc = wave_scan(nch);
+                if (lane == 63) wsum[wv][0] = inc;
+                __syncthreads();
+                int base = 0;
+                for (int w = 0; w < wv; ++w) base += wsum[w][0];
+                coff[tid] = base + inc - nch;
+                if (tid == NTHREADS - 1) coff[NTHREADS] = base + inc;
+                __syncthreads();
             }
+            const int nchunk = coff[NTHREADS];
 
-            const int npj = sp.nprim[slotj], npi = sp.nprim[sloti];
-            const double dx0 = xi - A.xyz[3 * j], dy0 = yi - A.xyz[3 * j + 1],
-                         dz0 = zi - A.xyz[3 * j + 2];
-            const double hsc = sp.hscale[slj + sp.msh * (sli + sp.msh * (spj + sp.nsp * spi))];
-            const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
-            const double radij = sp.rad[spi] + sp.rad[spj];
-            const double sesum = 0.5 * (A.se[mt.at_sh[i] + sli] + A.se[mt.at_sh[j] + slj]);
-            double dG[3] = {0, 0, 0}, dsi = 0.0, dsj = 0.0;
-            double sig[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            int nimg = 0;
-            const bool self = (j == i);
-            for (int itr = 0; itr < mt.ntrans; ++itr) {
-                const double vx = dx0 - mt.trans[3 * itr], vy = dy0 - mt.trans[3 * itr + 1],
-                             vz = dz0 - mt.trans[3 * itr + 2];
-                const double r2 = vx * vx + vy * vy + vz * vz;
-                const double vec[3] = {vx, vy, vz};
-                if (r2 < kEps) {
-                    if (MODE == GRAD_ADJOINT && self) {
-                        // on-site block: only dE/dse, H = S (se_i + se_j)/2
-                        double s = 0.0;
-                        for (int ip = 0; ip < npi; ++ip) {
-                            const double ai = sp.alpha[sloti * MAXG + ip];
-                            const double ci_ = sp.coeff[sloti * MAXG + ip];
-                            for (int jp = 0; jp < npj; ++jp) {
-                                const double aj = sp.alpha[slotj * MAXG + jp];
-                                const double oab = 1.0 / (ai + aj);
-                                const double so = sqrt(oab);
-                                const double cc = kSqrtPi3 * so * so * so * ci_ *
-                                                  sp.coeff[slotj * MAXG + jp];
-                                const Mom M = make_mom(0.5 * oab);
-                                double V[3][3];
-#pragma unroll
-                                for (int k = 0; k < 3; ++k) fwd1d(ei[k], ej[k], 0.0, 0.0, M, V[k]);
-                                s += cc * V[0][0] * V[1][0] * V[2][0];
-                            }
-                        }
-                        dsi += 0.5 * wHon * s;
-                        dsj += 0.5 * wHon * s;
-                    }
-                    continue;
+            // (4) work items
+            for (int c = tid; c < nchunk; c += NTHREADS) {
+                int lo2 = 0, hi2 = NTHREADS;  // last task with coff <= c
+                while (hi2 - lo2 > 1) {
+                    const int mid = (lo2 + hi2) >> 1;
+                    if (coff[mid] <= c) lo2 = mid; else hi2 = mid;
                 }
-                if (r2 > sp.cutoff2) continue;
-                ++nimg;
-                const double rr = sqrt(sqrt(r2) / radij);
-                const double shi = 1.0 + kpi * rr, shj = 1.0 + kpj * rr;
-                const double shp = shi * shj;
-                const double dshp = (shi * kpj + shj * kpi) * 0.5 * rr / r2;
-                const double hs = hsc * shp;
-                const double hij = sesum * hs;
-                double a3[3] = {0, 0, 0}, b3[3] = {0, 0, 0}, s = 0.0;
-                for (int ip = 0; ip < npi; ++ip) {
-                    const double ai = sp.alpha[sloti * MAXG + ip];
-                    const double ci_ = sp.coeff[sloti * MAXG + ip];
-                    for (int jp = 0; jp < npj; ++jp) {
-                        const double aj = sp.alpha[slotj * MAXG + jp];
-                        const double eab = ai + aj;
-                        const double oab = 1.0 / eab;
-                        const double est = ai * aj * r2 * oab;
-                        if (est > sp.intcut) continue;
-                        const double so = sqrt(oab);
-                        const double cc = exp(-est) * kSqrtPi3 * so * so * so * ci_ *
-                                          sp.coeff[slotj * MAXG + jp];
-                        const Mom M = make_mom(0.5 * oab);
-                        double Vi[3][3], Vj[3][3], Gi[3][3], Gj[3][3];
-#pragma unroll
-                        for (int k = 0; k < 3; ++k)
-                            grad1d(ei[k], ej[k], -vec[k] * aj * oab, vec[k] * ai * oab, ai, M,
-                                   Vi[k], Vj[k], Gi[k], Gj[k]);
-                        s += cc * wH * (Vi[0][0] * Vi[1][0] * Vi[2][0]);
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            // f_k(m): direction d carries the derivative
-                            double fi[3][3], fj[3][3];
-#pragma unroll
-                            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                                for (int mm = 0; mm < 3; ++mm) {
-                                    fi[k][mm] = (k == d) ? Gi[k][mm] : Vi[k][mm];
-                                    fj[k][mm] = (k == d) ? Gj[k][mm] : Vj[k][mm];
-                                }
-                            const double ds = fi[0][0] * fi[1][0] * fi[2][0];
-                            double acc = wS * ds;
-                            acc += wDi[0] * fi[0][1] * fi[1][0] * fi[2][0];
-                            acc += wDi[1] * fi[0][0] * fi[1][1] * fi[2][0];
-                            acc += wDi[2] * fi[0][0] * fi[1][0] * fi[2][1];
-                            acc += wDj[0] * fj[0][1] * fj[1][0] * fj[2][0];
-                            acc += wDj[1] * fj[0][0] * fj[1][1] * fj[2][0];
-                            acc += wDj[2] * fj[0][0] * fj[1][0] * fj[2][1];
-#pragma unroll
-                            for (int q = 0; q < 6; ++q) {
-                                acc += wQi[q] * fi[0][kQm[q][0]] * fi[1][kQm[q][1]] * fi[2][kQm[q][2]];
-                                acc += wQj[q] * fj[0][kQm[q][0]] * fj[1][kQm[q][1]] * fj[2][kQm[q][2]];
-                            }
-                            a3[d] += cc * acc;
-                            b3[d] += cc * wH * ds;
-                        }
-                    }
+                const int tk = lo2;
+                const int info = tinfo[tk];
+                const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
+                const int jw = (info >> 12) & 4095;
+                const int ish = (info >> 24) & 7, jsh = (info >> 27) & 7;
+                const int q = c - coff[tk];
+                const int p0 = tpp[tk] + q * CHUNK;
+                const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
+                const double *r = rec + tk;
+                double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
+                switch (t) {
+                case 0: chunk_run<0, 0>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 1: chunk_run<0, 1>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 2: chunk_run<0, 2>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 3: chunk_run<1, 0>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 4: chunk_run<1, 1>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 5: chunk_run<1, 2>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 6: chunk_run<2, 0>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                case 7: chunk_run<2, 1>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
+                default: chunk_run<2, 2>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
                 }
-                double g[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) g[d] = a3[d] + hij * (b3[d] + s * dshp / shp * vec[d]);
-                const double fself = self ? 0.5 : 1.0;
-                dsi += fself * 0.5 * hs * s;
-                dsj += fself * 0.5 * hs * s;
+                // explicit v dependence: S ex + 2 Wj D_i
+                const double *wq = r + R_WQ * TW_GRAD;
+                const double wq0 = wq[0], wq1 = wq[TW_GRAD], wq2 = wq[2 * TW_GRAD],
+                             wq3 = wq[3 * TW_GRAD], wq4 = wq[4 * TW_GRAD], wq5 = wq[5 * TW_GRAD];
+                const double WD0 = wq0 * dacc[0] + 0.5 * (wq1 * dacc[1] + wq3 * dacc[2]);
+                const double WD1 = wq2 * dacc[1] + 0.5 * (wq1 * dacc[0] + wq4 * dacc[2]);
+                const double WD2 = wq5 * dacc[2] + 0.5 * (wq3 * dacc[0] + wq4 * dacc[1]);
+                const double g0 = a3[0] + s * r[R_EX * TW_GRAD] + 2.0 * WD0;
+                const double g1 = a3[1] + s * r[(R_EX + 1) * TW_GRAD] + 2.0 * WD1;
+                const double g2 = a3[2] + s * r[(R_EX + 2) * TW_GRAD] + 2.0 * WD2;
+                const double vx = r[R_V * TW_GRAD], vy = r[(R_V + 1) * TW_GRAD],
+                             vz = r[(R_V + 2) * TW_GRAD];
+                const bool self = (jfirst + jw == i);
+                const double ds = s * r[R_HSW * TW_GRAD];
                 const double fs = self ? 0.25 : 0.5;
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) sig[a + 3 * b] += fs * (vec[a] * g[b] + g[a] * vec[b]);
-                if (!self) {
-                    dG[0] += g[0];
-                    dG[1] += g[1];
-                    dG[2] += g[2];
+                double *ai_ = accI[wv];
+                atomicAdd(&ai_[3 + MAXSH + 0], fs * 2.0 * vx * g0);
+                atomicAdd(&ai_[3 + MAXSH + 1], fs * (vx * g1 + g0 * vy));
+                atomicAdd(&ai_[3 + MAXSH + 2], fs * 2.0 * vy * g1);
+                atomicAdd(&ai_[3 + MAXSH + 3], fs * (vx * g2 + g0 * vz));
+                atomicAdd(&ai_[3 + MAXSH + 4], fs * (vy * g2 + g1 * vz));
+                atomicAdd(&ai_[3 + MAXSH + 5], fs * 2.0 * vz * g2);
+                atomicAdd(&ai_[3 + ish], ds);
+                if (self) {
+                    atomicAdd(&ai_[3 + jsh], ds);
+                } else {
+                    atomicAdd(&ai_[0], g0);
+                    atomicAdd(&ai_[1], g1);
+                    atomicAdd(&ai_[2], g2);
+                    atomicAdd(&accJ[jw][0], -g0);
+                    atomicAdd(&accJ[jw][1], -g1);
+                    atomicAdd(&accJ[jw][2], -g2);
+                    atomicAdd(&accJ[jw][3 + jsh], ds);
                 }
             }
-            if (A.nnl && ci == 0 && cj == 0 && nimg > 0) atomicAdd(&s_count, nimg);
-            // scatter to LDS accumulators
-            const int jw = j - jfirst;
-            if (dG[0] != 0.0 || dG[1] != 0.0 || dG[2] != 0.0) {
-                atomicAdd(&accJ[jw][0], -dG[0]);
-                atomicAdd(&accJ[jw][1], -dG[1]);
-                atomicAdd(&accJ[jw][2], -dG[2]);
-                atomicAdd(&accI[0], dG[0]);
-                atomicAdd(&accI[1], dG[1]);
-                atomicAdd(&accI[2], dG[2]);
+            __syncthreads();
+            // (5) flush the window's atom-j sums
+            for (int k = tid; k < nwin; k += NTHREADS) {
+                const int jj = jfirst + k;
+                if (jj == i) continue;  // self images were added to atom i directly
+                for (int q = 0; q < 3; ++q)
+                    if (accJ[k][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * jj + q], accJ[k][q]);
+                const int nshj = sp.nsh[mt.at_sp[jj]];
+                for (int q = 0; q < nshj; ++q)
+                    if (accJ[k][3 + q] != 0.0)
+                        unsafeAtomicAdd(&A.dse[mt.at_sh[jj] + q], accJ[k][3 + q]);
             }
-            if (dsj != 0.0) atomicAdd(&accJ[jw][3 + slj], dsj);
-            if (dsi != 0.0) atomicAdd(&accI[3 + sli], dsi);
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-                if (sig[k] != 0.0) atomicAdd(&accI[3 + MAXSH + k], sig[k]);
+            c0 = c1;
         }
-        __syncthreads();
-        // flush window atoms to global
-        for (int k = threadIdx.x; k < nwin; k += NTHREADS) {
-            const int j = jfirst + k;
-            if (j == i) {
-                // self-images / on-site: merge into atom i's accumulators
-                for (int q = 0; q < 3 + MAXSH; ++q) atomicAdd(&accI[q], accJ[k][q]);
-                continue;
+    }
+    // neighbour count for the list check: pairs (j <= i, image) within the cutoff
+    if (A.nnl) {
+        int n = 0;
+        for (int jj = a0 + tid; jj < jend; jj += NTHREADS) {
+            for (int itr = 0; itr < mt.ntrans; ++itr) {
+                const double vx = A.xyz[3 * i] - A.xyz[3 * jj] - mt.trans[3 * itr];
+                const double vy = A.xyz[3 * i + 1] - A.xyz[3 * jj + 1] - mt.trans[3 * itr + 1];
+                const double vz = A.xyz[3 * i + 2] - A.xyz[3 * jj + 2] - mt.trans[3 * itr + 2];
+                const double r2 = vx * vx + vy * vy + vz * vz;
+                if (r2 >= kEps && r2 <= sp.cutoff2) ++n;
             }
-            for (int q = 0; q < 3; ++q)
-                if (accJ[k][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * j + q], accJ[k][q]);
-            const int nshj = sp.nsh[mt.at_sp[j]];
-            for (int q = 0; q < nshj; ++q)
-                if (accJ[k][3 + q] != 0.0) unsafeAtomicAdd(&A.dse[mt.at_sh[j] + q], accJ[k][3 + q]);
         }
-        c0 = c1;
+        atomicAdd(&s_count, n);
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
-        if (accI[threadIdx.x] != 0.0) unsafeAtomicAdd(&A.grad[3 * i + threadIdx.x], accI[threadIdx.x]);
-    } else if (threadIdx.x < 3 + MAXSH) {
-        const int q = threadIdx.x - 3;
-        if (q < sp.nsh[spi] && accI[threadIdx.x] != 0.0)
-            unsafeAtomicAdd(&A.dse[mt.at_sh[i] + q], accI[threadIdx.x]);
-    } else if (threadIdx.x < 3 + MAXSH + 9) {
-        const int q = threadIdx.x - 3 - MAXSH;
-        if (accI[threadIdx.x] != 0.0) unsafeAtomicAdd(&A.sigma[9 * m + q], accI[threadIdx.x]);
+    if (tid < 3 + MAXSH + 9) {
+        // sum the per-wavefront copies; sigma is expanded from the packed form to 3x3
+        const int k9 = tid - 3 - MAXSH;
+        const int kp = (tid < 3 + MAXSH) ? tid : 3 + MAXSH + kSigPack[k9];
+        double s = 0.0;
+        for (int w = 0; w < NTHREADS / 64; ++w) s += accI[w][kp];
+        if (s != 0.0) {
+            if (tid < 3)
+                unsafeAtomicAdd(&A.grad[3 * i + tid], s);
+            else if (tid < 3 + MAXSH) {
+                if (tid - 3 < nshi) unsafeAtomicAdd(&A.dse[mt.at_sh[i] + tid - 3], s);
+            } else
+                unsafeAtomicAdd(&A.sigma[9 * m + k9], s);
+        }
     }
-    if (A.nnl && threadIdx.x == 0 && s_count != A.nnl[i]) atomicOr(A.flag, 1);
+    if (A.nnl && tid == 0 && s_count != A.nnl[i]) atomicOr(A.flag, 1);
 }
 
 // dEdcn (plain mode): sum over shells of dsedcn * dE/dse plus the on-site

@@ -5,6 +5,17 @@
 
 #define TBX_MAXCART 24 /* Cartesian functions per species: <= 4 shells, l <= 2 */
 #define TBX_MAXAO 20   /* spherical functions per species */
+#define TBX_MAXPRIM 2048 /* primitives over all shells of all species (staged in LDS) */
+
+/* fields of a primitive-pair entry (ai, aj: exponents; ci, cj: contraction
+   coefficients incl. normalisation) */
+#define TBX_PP_MU 0   /* ai aj / (ai + aj) */
+#define TBX_PP_AJO 1  /* aj / (ai + aj) */
+#define TBX_PP_AIO 2  /* ai / (ai + aj) */
+#define TBX_PP_H 3    /* 1 / (2 (ai + aj)) */
+#define TBX_PP_AI 4   /* ai */
+#define TBX_PP_PRE 5  /* pi^(3/2) (ai + aj)^(-3/2) ci cj */
+#define TBX_PP_N 6
 
 // Species tables (device pointers).  Shell slot = ish + msh*isp.
 struct TbxSpecies {
@@ -25,6 +36,15 @@ struct TbxSpecies {
     const int *ao_m;      // [nsp*TBX_MAXAO]  m index (0..2l) within the shell
     const int *sh_cart;   // [nsp*msh] first Cartesian function of the shell within the atom
     const int *sh_ao;     // [nsp*msh] first spherical function of the shell within the atom
+    const double *prim;   // [2*nprim_tot] (alpha, coeff) of every primitive, shells packed
+    const int *prim_off;  // [nsp*msh] first primitive of the shell in prim
+    int nprim_tot;
+    // primitive pairs of every ordered shell-slot pair (si, sj), sorted by the
+    // reduced exponent mu = ai aj / (ai + aj): entries pp_off[si*nslot+sj] ..
+    // pp_off[si*nslot+sj+1], 6 doubles each (TBX_PP_* fields)
+    int nslot;            // nsp*msh
+    const int *pp_off;    // [nslot*nslot+1]
+    const double *pp;     // [TBX_PP_N * entries]
     double intcut;
     double cutoff2;
 };
