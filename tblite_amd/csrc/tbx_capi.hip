@@ -229,9 +229,49 @@ struct HostMols {
     int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, order, sh_off;
+        cart_atom, ao_atom, order, sh_off, unit_seg;
     std::vector<long long> mol_dense;
+    std::vector<TbxSeg> useg;
 };
+
+// Gradient work units (see TbxSeg): the half list j < i (j <= i with lattice
+// images) of every molecule, cut into units of bounded task count.
+static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans)
+{
+    hm.useg.clear();
+    hm.unit_seg.assign(1, 0);
+    int tasks = 0, pairs = 0, segs = 0;
+    bool open_seg = false;
+    auto close_unit = [&]() {
+        if (segs > 0) hm.unit_seg.push_back((int)hm.useg.size());
+        tasks = pairs = segs = 0;
+        open_seg = false;
+    };
+    for (int m = 0; m < hm.nmol; ++m) {
+        const int a0 = hm.mol_atom[m], a1 = hm.mol_atom[m + 1];
+        for (int itr = 0; itr < ntrans; ++itr)
+            for (int i = a0; i < a1; ++i) {
+                const int jend = ntrans > 1 ? i + 1 : i;
+                const int nci = hs.ncart[hm.at_sp[i]];
+                open_seg = false;
+                for (int j = a0; j < jend; ++j) {
+                    const int c = nci * hs.ncart[hm.at_sp[j]];
+                    if (tasks + c > TBX_UNIT_TASKS || pairs == TBX_UNIT_PAIRS ||
+                        (!open_seg && segs == TBX_UNIT_SEGS))
+                        close_unit();
+                    if (!open_seg) {
+                        hm.useg.push_back(TbxSeg{i, j, j, itr});
+                        ++segs;
+                        open_seg = true;
+                    }
+                    hm.useg.back().j1 = j + 1;
+                    tasks += c;
+                    ++pairs;
+                }
+            }
+        close_unit();  // units never span molecules
+    }
+}
 
 static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const int *id,
                       HostMols &hm)
@@ -300,7 +340,7 @@ struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
         ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, mol_ao, order;
+        cart_atom, ao_atom, mol_ao, order, unit_seg, useg;
     TbxSpecies sp{};
     TbxMols mt{};
 
@@ -311,7 +351,7 @@ struct DevTables {
                          &pp, &pp_off,
                          &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
-                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &order};
+                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &order, &unit_seg, &useg};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -360,7 +400,8 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     hipError_t e;
 #define UP(name) if ((e = upload(d.name, hm.name, s)) != hipSuccess) return e
     UP(mol_atom); UP(mol_nao); UP(mol_cart); UP(mol_dense); UP(at_sp); UP(at_mol); UP(at_sh);
-    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(order);
+    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(order); UP(unit_seg);
+    UP(useg);
 #undef UP
     if ((e = upload(d.trans, trans, s)) != hipSuccess) return e;
     TbxMols &mt = d.mt;
@@ -380,6 +421,9 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     mt.cart_atom = d.cart_atom.as<int>();
     mt.ao_atom = d.ao_atom.as<int>();
     mt.mol_ao = d.mol_ao.as<int>();
+    mt.nunit = (int)hm.unit_seg.size() - 1;
+    mt.unit_seg = d.unit_seg.as<int>();
+    mt.seg = d.useg.as<TbxSeg>();
     return hipSuccess;
 }
 
@@ -497,6 +541,7 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
     hipStream_t s = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
+    build_units(hs, hm, geom->ntrans);
     HIP_TRY(upload_mols(ctx->tab, hm, trans, s));
     HIP_TRY(ctx->xyz.reserve(sizeof(double) * 3 * geom->nat));
     HIP_TRY(hipMemcpyAsync(ctx->xyz.p, geom->xyz, sizeof(double) * 3 * geom->nat,
@@ -635,6 +680,7 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     hipError_t e = hipSetDevice(ctx->device);
     std::vector<double> trans(3, 0.0);
     if (e == hipSuccess) e = upload_species(b->tab, b->hs, desc->cutoff, ctx->stream);
+    build_units(b->hs, b->hm, 1);
     if (e == hipSuccess) e = upload_mols(b->tab, b->hm, trans, ctx->stream);
     if (e == hipSuccess) e = b->flag.reserve(sizeof(int));
     // scratch of the gradient / adjoint launches: dE/dse (plain) or sigma (adjoint)

@@ -492,15 +492,22 @@ static constexpr int kSigPack[9] = {0, 1, 3, 1, 2, 4, 3, 4, 5};
 // wavefront): dE/dR_i (3), dE/dse of its shells (MAXSH), sigma packed (6)
 #define ACCI_N (3 + MAXSH + 6)
 #define CHUNK 8    // primitive pairs per work item
-#define NREC 23    // doubles of a task record
+#define NREC 20    // doubles of a task record
 
-// Task record fields (LDS, structure of arrays over the tasks of a window)
+// Task record fields (LDS, structure of arrays over the tasks of a unit)
 enum {
     R_W = 0,     // 10 contraction weights about centre i: S, D (3), Q (6)
-    R_V = 10,    // v = R_i - R_j - T (3)
-    R_EX = 13,   // weights of S in the explicit v-derivative (3)
-    R_WQ = 16,   // traceless-folded Q weights of the j-centred moments (6)
-    R_HSW = 22,  // dE/dse_i = dE/dse_j per unit S
+    R_EX = 10,   // weights of S in the explicit v-derivative (3)
+    R_WQ = 13,   // traceless-folded Q weights of the j-centred moments (6)
+    R_HSW = 19,  // dE/dse_i = dE/dse_j per unit S
+};
+
+// Atom pair of a unit, staged in LDS
+struct PairInfo {
+    int i, j;
+    int spi, spj;
+    int aoi, aoj;  // first AO of the atoms, molecule-local
+    int shi, shj;  // first shell of the atoms (global)
 };
 
 // inclusive prefix sum over the 64 lanes of a wavefront
@@ -532,17 +539,15 @@ __device__ __forceinline__ int kept_pairs(const double *pp, int n, double r2, do
 // shifts the j-centred multipole weights to centre i and stores the record.
 // Returns the number of primitive pairs kept by the screening.
 template <int MODE>
-__device__ int task_setup(const GradArgs &A, int i, int j, int itr, int ish, int jsh, int ki,
-                          int kj, double *rec, int ld, int &ppoff)
+__device__ int task_setup(const GradArgs &A, const PairInfo &pr, const double v[3], int ish,
+                          int jsh, int ki, int kj, int nao, long long dense0, int ao_m,
+                          double *rec, int ld, int &ppoff)
 {
     const TbxSpecies &sp = A.sp;
-    const TbxMols &mt = A.mol;
-    const int spi = mt.at_sp[i], spj = mt.at_sp[j];
+    const int i = pr.i, j = pr.j, spi = pr.spi, spj = pr.spj;
     const int sloti = ish + sp.msh * spi, slotj = jsh + sp.msh * spj;
     const int li = sp.ang[sloti], lj = sp.ang[slotj];
-    const double vx = A.xyz[3 * i] - A.xyz[3 * j] - mt.trans[3 * itr];
-    const double vy = A.xyz[3 * i + 1] - A.xyz[3 * j + 1] - mt.trans[3 * itr + 1];
-    const double vz = A.xyz[3 * i + 2] - A.xyz[3 * j + 2] - mt.trans[3 * itr + 2];
+    const double vx = v[0], vy = v[1], vz = v[2];
     const double r2 = vx * vx + vy * vy + vz * vz;
     if (r2 < kEps || r2 > sp.cutoff2) return 0;
     const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
@@ -551,12 +556,8 @@ __device__ int task_setup(const GradArgs &A, int i, int j, int itr, int ish, int
     ppoff = pi0;
     if (kept == 0) return 0;
 
-    const int m = mt.at_mol[i];
-    const int nao = mt.mol_nao[m];
-    const long long dense0 = mt.mol_dense[m];
-    const int ao_m = mt.mol_ao[m];
-    const int muj0 = mt.at_ao[j] + sp.sh_ao[slotj];
-    const int nui0 = mt.at_ao[i] + sp.sh_ao[sloti];
+    const int muj0 = pr.aoj + sp.sh_ao[slotj];
+    const int nui0 = pr.aoi + sp.sh_ao[sloti];
     const int na = kCartN[lj][kj], nb = kCartN[li][ki];
     double wS = 0.0, wH = 0.0;
     double wDi[3] = {0, 0, 0}, wDj[3] = {0, 0, 0};
@@ -632,15 +633,14 @@ __device__ int task_setup(const GradArgs &A, int i, int j, int itr, int ish, int
                           wQj[2] * vy + 0.5 * (wQj[1] * vx + wQj[4] * vz),
                           wQj[5] * vz + 0.5 * (wQj[3] * vx + wQj[4] * vy)};
     // H0 = S * (se_i + se_j)/2 * hscale * shpoly(R); shpoly's R dependence is explicit
-    const int spi_ = spi, spj_ = spj;
-    const double hsc = sp.hscale[jsh + sp.msh * (ish + sp.msh * (spj_ + sp.nsp * spi_))];
+    const double hsc = sp.hscale[jsh + sp.msh * (ish + sp.msh * (spj + sp.nsp * spi))];
     const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
     const double rr = sqrt(sqrt(r2) / (sp.rad[spi] + sp.rad[spj]));
     const double shi = 1.0 + kpi * rr, shj = 1.0 + kpj * rr;
     const double shp = shi * shj;
     const double dshp = (shi * kpj + shj * kpi) * 0.5 * rr / r2;  // (d shp / d r) / r
     const double hs = hsc * shp;
-    const double sesum = 0.5 * (A.se[mt.at_sh[i] + ish] + A.se[mt.at_sh[j] + jsh]);
+    const double sesum = 0.5 * (A.se[pr.shi + ish] + A.se[pr.shj + jsh]);
     const double hw = sesum * hs * wH;
     rec[(R_W + 0) * ld] = wS + hw + wDj[0] * vx + wDj[1] * vy + wDj[2] * vz + Wv[0] * vx +
                           Wv[1] * vy + Wv[2] * vz;
@@ -649,9 +649,6 @@ __device__ int task_setup(const GradArgs &A, int i, int j, int itr, int ish, int
     rec[(R_W + 3) * ld] = wDi[2] + wDj[2] + 2.0 * Wv[2];
 #pragma unroll
     for (int k = 0; k < 6; ++k) rec[(R_W + 4 + k) * ld] = wQi[k] + wQj[k];
-    rec[(R_V + 0) * ld] = vx;
-    rec[(R_V + 1) * ld] = vy;
-    rec[(R_V + 2) * ld] = vz;
     const double hd = hw * dshp / shp;
     rec[(R_EX + 0) * ld] = wDj[0] + 2.0 * Wv[0] + hd * vx;
     rec[(R_EX + 1) * ld] = wDj[1] + 2.0 * Wv[1] + hd * vy;
@@ -667,15 +664,14 @@ __device__ int task_setup(const GradArgs &A, int i, int j, int itr, int ish, int
 // dipole moments dacc.
 template <int LI, int LJ>
 __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const double *rec, int ld,
-                                          int ki, int kj, int p0, int p1, double a3[3],
-                                          double &s, double dacc[3])
+                                          const double vec[3], int ki, int kj, int p0, int p1,
+                                          double a3[3], double &s, double dacc[3])
 {
     const int ei[3] = {kCartExp[LI][ki][0], kCartExp[LI][ki][1], kCartExp[LI][ki][2]};
     const int ej[3] = {kCartExp[LJ][kj][0], kCartExp[LJ][kj][1], kCartExp[LJ][kj][2]};
     double w[10];
 #pragma unroll
     for (int k = 0; k < 10; ++k) w[k] = rec[(R_W + k) * ld];
-    const double vec[3] = {rec[R_V * ld], rec[(R_V + 1) * ld], rec[(R_V + 2) * ld]};
     const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
     for (int p = p0; p < p1; ++p) {
         const double *e = pp + TBX_PP_N * p;
@@ -697,49 +693,262 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
     }
 }
 
-// Workgroup of atom i.  For every lattice vector, the atoms j <= i of the
-// molecule are walked in windows.  In a window: (1) the shell pairs are laid
-// out by type (l_i, l_j), (2) one lane per Cartesian-pair task builds the
-// task record and counts its kept primitive pairs, (3) the kept pairs are cut
-// into chunks of CHUNK and a block scan turns them into a dense list of work
-// items, (4) every lane runs work items (type-major, so a wavefront mostly
-// runs one specialised code path) and adds its results to the LDS sums of
-// atoms i and j, (5) the window's atom-j sums go to HBM with FP64 atomics.
+// One workgroup per work unit (up to TBX_UNIT_PAIRS atom pairs of one
+// molecule, at most TBX_UNIT_TASKS Cartesian-pair tasks).  (1) The unit's
+// pairs are staged in LDS and their shell pairs laid out by type (l_i, l_j);
+// (2) one lane per task builds the task record and counts its kept primitive
+// pairs; (3) the kept pairs are cut into chunks of CHUNK and a block scan
+// turns them into a dense list of work items; (4) every lane runs work items
+// (type-major, so a wavefront mostly runs one specialised code path) and adds
+// the results to its pair's LDS sums; (5) the pair sums go to atoms j, the
+// segment sums to atoms i and the unit sum to sigma, with FP64 atomics.
 template <int MODE>
 __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
 {
-    __shared__ double rec[NREC * TW_GRAD];
-    __shared__ double accJ[TW_GRAD][3 + MAXSH];
-    __shared__ double accI[NTHREADS / 64][ACCI_N];
-    __shared__ int4 spl[TW_GRAD];          // (j, jsh, ish, type), type-major
-    __shared__ int tinfo[TW_GRAD];         // type | ki << 4 | kj << 8 | jw << 12
-    __shared__ int tpp[TW_GRAD];           // first primitive pair of the task
-    __shared__ int tkept[TW_GRAD];
-    __shared__ int coff[TW_GRAD + 1];      // first chunk of each task
+    __shared__ double rec[NREC * TBX_UNIT_TASKS];
+    __shared__ double accP[TBX_UNIT_PAIRS][3 + 2 * MAXSH];  // G (3), dse_i, dse_j per shell
+    __shared__ double pv[TBX_UNIT_PAIRS][3];
+    __shared__ PairInfo pinf[TBX_UNIT_PAIRS];
+    __shared__ int pseg[TBX_UNIT_PAIRS];
+    __shared__ double accS[TBX_UNIT_SEGS][3 + MAXSH];      // atom-i sums per segment
+    __shared__ double sig[6];
+    __shared__ int sfirst[TBX_UNIT_SEGS + 1];
+    __shared__ int4 spl[TBX_UNIT_TASKS];  // (pair, jsh, ish, type), type-major
+    __shared__ int tinfo[TBX_UNIT_TASKS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
+    __shared__ int tpp[TBX_UNIT_TASKS];    // first primitive pair of the task
+    __shared__ int tkept[TBX_UNIT_TASKS];
+    __shared__ int coff[TBX_UNIT_TASKS + 1];  // first chunk of each task
     __shared__ int wsum[NTHREADS / 64][9];
-    __shared__ int toff[10], tsp[10];      // task / shell-pair offsets per type
-    __shared__ int s_count;
+    __shared__ int toff[10], tsp[10];         // task / shell-pair offsets per type
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
-    const int i = A.order[blockIdx.x];
-    const int m = mt.at_mol[i];
-    const int a0 = mt.mol_atom[m];
-    const int spi = mt.at_sp[i];
-    const int nci = sp.ncart[spi];
-    const int nshi = sp.nsh[spi];
-    const int cart_m = mt.mol_cart[m];
-    const int rows_cap = TW_GRAD / nci;
-    // half list: j < i, plus j == i for the lattice self-images
-    const int jend = (mt.ntrans > 1) ? i + 1 : i;
-    const int cend = (jend > a0) ? (mt.at_cart[jend - 1] + sp.ncart[mt.at_sp[jend - 1]]) : 0;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid == 0) s_count = 0;
-    for (int k = tid; k < (NTHREADS / 64) * ACCI_N; k += NTHREADS) (&accI[0][0])[k] = 0.0;
+    const int u = blockIdx.x;
+    const int s0 = mt.unit_seg[u], nseg = mt.unit_seg[u + 1] - s0;
+    const int m = mt.at_mol[mt.seg[s0].i];
+    const int nao = mt.mol_nao[m];
+    const long long dense0 = mt.mol_dense[m];
+    const int ao_m = mt.mol_ao[m];
+
+    if (tid == 0) {
+        int o = 0;
+        for (int k = 0; k < nseg; ++k) {
+            sfirst[k] = o;
+            o += mt.seg[s0 + k].j1 - mt.seg[s0 + k].j0;
+        }
+        sfirst[nseg] = o;
+    }
+    for (int k = tid; k < TBX_UNIT_PAIRS * (3 + 2 * MAXSH); k += NTHREADS) (&accP[0][0])[k] = 0.0;
+    for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += NTHREADS) (&accS[0][0])[k] = 0.0;
+    if (tid < 6) sig[tid] = 0.0;
+    __syncthreads();
+    const int npair = sfirst[nseg];
+
+    // (1) stage the pairs; shell pairs per type, type-major
+    int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (tid < npair) {
+        int k = 0;
+        while (k + 1 < nseg && sfirst[k + 1] <= tid) ++k;
+        const TbxSeg g = mt.seg[s0 + k];
+        PairInfo pr;
+        pr.i = g.i;
+        pr.j = g.j0 + tid - sfirst[k];
+        pr.spi = mt.at_sp[pr.i];
+        pr.spj = mt.at_sp[pr.j];
+        pr.aoi = mt.at_ao[pr.i];
+        pr.aoj = mt.at_ao[pr.j];
+        pr.shi = mt.at_sh[pr.i];
+        pr.shj = mt.at_sh[pr.j];
+        pinf[tid] = pr;
+        pseg[tid] = k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            pv[tid][c] = A.xyz[3 * pr.i + c] - A.xyz[3 * pr.j + c] - mt.trans[3 * g.itr + c];
+        int nli[3] = {0, 0, 0}, nlj[3] = {0, 0, 0};
+        for (int q = 0; q < sp.nsh[pr.spi]; ++q) ++nli[sp.ang[q + sp.msh * pr.spi]];
+        for (int q = 0; q < sp.nsh[pr.spj]; ++q) ++nlj[sp.ang[q + sp.msh * pr.spj]];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
+    }
+    int incl[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        incl[t] = wave_scan(cnt[t]);
+        if (lane == 63) wsum[wv][t] = incl[t];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int so = 0, to = 0;
+        for (int t = 0; t < 9; ++t) {
+            int tot = 0;
+            for (int w = 0; w < NTHREADS / 64; ++w) tot += wsum[w][t];
+            tsp[t] = so;
+            toff[t] = to;
+            so += tot;
+            to += tot * ncart_of(t / 3) * ncart_of(t % 3);
+        }
+        tsp[9] = so;
+        toff[9] = to;
+    }
+    __syncthreads();
+    if (tid < npair) {
+        const int spi = pinf[tid].spi, spj = pinf[tid].spj;
+        for (int t = 0; t < 9; ++t) {
+            if (cnt[t] == 0) continue;
+            int off = tsp[t] + incl[t] - cnt[t];
+            for (int w = 0; w < wv; ++w) off += wsum[w][t];
+            for (int qi = 0; qi < sp.nsh[spi]; ++qi) {
+                if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
+                for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
+                    if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
+                    spl[off++] = make_int4(tid, qj, qi, t);
+                }
+            }
+        }
+    }
     __syncthreads();
 
-    // on-site block (adjoint only): H = S (se_i + se_j)/2 gives dE/dse only
-    if (MODE == GRAD_ADJOINT && tid < nci * nci) {
-        const int ci = tid / nci, cj = tid - ci * nci;
+    // (2) task records: one task per lane (ntask <= TBX_UNIT_TASKS = NTHREADS)
+    const int ntask = toff[9];
+    int nch = 0;
+    if (tid < ntask) {
+        int t = 0;
+        while (t < 8 && tid >= toff[t + 1]) ++t;
+        const int ncj = ncart_of(t % 3);
+        const int ncp = ncart_of(t / 3) * ncj;
+        const int local = tid - toff[t];
+        const int4 e = spl[tsp[t] + local / ncp];
+        const int cp = local % ncp;
+        const int ki = cp / ncj, kj = cp - (cp / ncj) * ncj;
+        const double v[3] = {pv[e.x][0], pv[e.x][1], pv[e.x][2]};
+        int ppoff = 0;
+        const int kept = task_setup<MODE>(A, pinf[e.x], v, e.z, e.y, ki, kj, nao, dense0, ao_m,
+                                          rec + tid, TBX_UNIT_TASKS, ppoff);
+        tinfo[tid] = t | (ki << 4) | (kj << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
+        tpp[tid] = ppoff;
+        tkept[tid] = kept;
+        nch = (kept + CHUNK - 1) / CHUNK;
+    }
+    // (3) chunk offsets: block-wide exclusive scan
+    {
+        const int inc = wave_scan(nch);
+        __syncthreads();  // wsum is reused
+        if (lane == 63) wsum[wv][0] = inc;
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wv; ++w) base += wsum[w][0];
+        coff[tid] = base + inc - nch;
+        if (tid == NTHREADS - 1) coff[NTHREADS] = base + inc;
+        __syncthreads();
+    }
+    const int nchunk = coff[NTHREADS];
+
+    // (4) work items
+    for (int c = tid; c < nchunk; c += NTHREADS) {
+        int lo = 0, hi = NTHREADS;  // last task with coff <= c
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (coff[mid] <= c) lo = mid; else hi = mid;
+        }
+        const int tk = lo;
+        const int info = tinfo[tk];
+        const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
+        const int p = (info >> 12) & 4095;
+        const int ish = (info >> 24) & 7, jsh = (info >> 27) & 7;
+        const int q = c - coff[tk];
+        const int p0 = tpp[tk] + q * CHUNK;
+        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
+        const double *r = rec + tk;
+        const double vec[3] = {pv[p][0], pv[p][1], pv[p][2]};
+        double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
+        const int L = TBX_UNIT_TASKS;
+        switch (t) {
+        case 0: chunk_run<0, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 1: chunk_run<0, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 2: chunk_run<0, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 3: chunk_run<1, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 4: chunk_run<1, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 5: chunk_run<1, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 6: chunk_run<2, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        case 7: chunk_run<2, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        default: chunk_run<2, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        }
+        // explicit v dependence: S ex + 2 Wj D_i
+        const double *wq = r + R_WQ * L;
+        const double wq0 = wq[0], wq1 = wq[L], wq2 = wq[2 * L], wq3 = wq[3 * L],
+                     wq4 = wq[4 * L], wq5 = wq[5 * L];
+        const double WD0 = wq0 * dacc[0] + 0.5 * (wq1 * dacc[1] + wq3 * dacc[2]);
+        const double WD1 = wq2 * dacc[1] + 0.5 * (wq1 * dacc[0] + wq4 * dacc[2]);
+        const double WD2 = wq5 * dacc[2] + 0.5 * (wq3 * dacc[0] + wq4 * dacc[1]);
+        atomicAdd(&accP[p][0], a3[0] + s * r[R_EX * L] + 2.0 * WD0);
+        atomicAdd(&accP[p][1], a3[1] + s * r[(R_EX + 1) * L] + 2.0 * WD1);
+        atomicAdd(&accP[p][2], a3[2] + s * r[(R_EX + 2) * L] + 2.0 * WD2);
+        const double ds = s * r[R_HSW * L];
+        atomicAdd(&accP[p][3 + ish], ds);
+        atomicAdd(&accP[p][3 + MAXSH + jsh], ds);
+    }
+    __syncthreads();
+
+    // (5) pair sums -> atom j (HBM), segment sums (atom i) and sigma (LDS)
+    if (tid < npair) {
+        const PairInfo pr = pinf[tid];
+        const int k = pseg[tid];
+        const bool self = (pr.i == pr.j);
+        const double G0 = accP[tid][0], G1 = accP[tid][1], G2 = accP[tid][2];
+        const double vx = pv[tid][0], vy = pv[tid][1], vz = pv[tid][2];
+        if (!self && (G0 != 0.0 || G1 != 0.0 || G2 != 0.0)) {
+            unsafeAtomicAdd(&A.grad[3 * pr.j + 0], -G0);
+            unsafeAtomicAdd(&A.grad[3 * pr.j + 1], -G1);
+            unsafeAtomicAdd(&A.grad[3 * pr.j + 2], -G2);
+            atomicAdd(&accS[k][0], G0);
+            atomicAdd(&accS[k][1], G1);
+            atomicAdd(&accS[k][2], G2);
+        }
+        const double fs = self ? 0.25 : 0.5;
+        if (G0 != 0.0 || G1 != 0.0 || G2 != 0.0) {
+            atomicAdd(&sig[0], fs * 2.0 * vx * G0);
+            atomicAdd(&sig[1], fs * (vx * G1 + G0 * vy));
+            atomicAdd(&sig[2], fs * 2.0 * vy * G1);
+            atomicAdd(&sig[3], fs * (vx * G2 + G0 * vz));
+            atomicAdd(&sig[4], fs * (vy * G2 + G1 * vz));
+            atomicAdd(&sig[5], fs * 2.0 * vz * G2);
+        }
+        for (int q = 0; q < sp.nsh[pr.spj]; ++q)
+            if (accP[tid][3 + MAXSH + q] != 0.0)
+                unsafeAtomicAdd(&A.dse[pr.shj + q], accP[tid][3 + MAXSH + q]);
+        for (int q = 0; q < sp.nsh[pr.spi]; ++q)
+            if (accP[tid][3 + q] != 0.0) atomicAdd(&accS[k][3 + q], accP[tid][3 + q]);
+    }
+    __syncthreads();
+    if (tid < nseg) {
+        const int i = mt.seg[s0 + tid].i;
+        for (int q = 0; q < 3; ++q)
+            if (accS[tid][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * i + q], accS[tid][q]);
+        const int nshi = sp.nsh[mt.at_sp[i]];
+        for (int q = 0; q < nshi; ++q)
+            if (accS[tid][3 + q] != 0.0) unsafeAtomicAdd(&A.dse[mt.at_sh[i] + q], accS[tid][3 + q]);
+    } else if (tid >= 64 && tid < 64 + 9) {
+        const int k9 = tid - 64;
+        const double v = sig[kSigPack[k9]];
+        if (v != 0.0) unsafeAtomicAdd(&A.sigma[9 * m + k9], v);
+    }
+}
+
+// On-site blocks of the adjoint: H = S (se_i + se_j)/2 contributes to dE/dse
+// only.  16 lanes per atom, looping over its Cartesian pairs.
+__global__ void __launch_bounds__(NTHREADS) k_onsite(GradArgs A)
+{
+    const TbxSpecies &sp = A.sp;
+    const TbxMols &mt = A.mol;
+    const int i = blockIdx.x * (NTHREADS / 16) + (threadIdx.x >> 4);
+    if (i >= mt.nat) return;
+    const int spi = mt.at_sp[i];
+    const int nci = sp.ncart[spi];
+    const int m = mt.at_mol[i];
+    const int nao = mt.mol_nao[m];
+    const long long dense0 = mt.mol_dense[m];
+    for (int tk = threadIdx.x & 15; tk < nci * nci; tk += 16) {
+        const int ci = tk / nci, cj = tk - ci * nci;
         const int ei_ = sp.cart_exp[spi * TBX_MAXCART + ci];
         const int ej_ = sp.cart_exp[spi * TBX_MAXCART + cj];
         const int ish = sp.cart_sh[spi * TBX_MAXCART + ci];
@@ -747,8 +956,6 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         const int ki = ei_ >> 12, kj = ej_ >> 12;
         const int sloti = ish + sp.msh * spi, slotj = jsh + sp.msh * spi;
         const int li = sp.ang[sloti], lj = sp.ang[slotj];
-        const int nao = mt.mol_nao[m];
-        const long long dense0 = mt.mol_dense[m];
         const int muj0 = mt.at_ao[i] + sp.sh_ao[slotj];
         const int nui0 = mt.at_ao[i] + sp.sh_ao[sloti];
         double wHon = 0.0;
@@ -757,233 +964,41 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
                 wHon += kCartW[lj][kj][a] * kCartW[li][ki][b] *
                         A.Hb[dense0 + (long long)(nui0 + kCartM[li][ki][b]) * nao + muj0 +
                              kCartM[lj][kj][a]];
+        if (wHon == 0.0) continue;
         const int e0 = (ei_ & 15) + (ej_ & 15), e1 = ((ei_ >> 4) & 15) + ((ej_ >> 4) & 15),
                   e2 = ((ei_ >> 8) & 15) + ((ej_ >> 8) & 15);
-        const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
-        const int pi1 = sp.pp_off[sloti * sp.nslot + slotj + 1];
+        const int p0 = sp.pp_off[sloti * sp.nslot + slotj];
+        const int p1 = sp.pp_off[sloti * sp.nslot + slotj + 1];
         double s = 0.0;
-        for (int p = pi0; p < pi1; ++p) {
+        for (int p = p0; p < p1; ++p) {
             const Mom M = make_mom(sp.pp[TBX_PP_N * p + TBX_PP_H]);
             s += sp.pp[TBX_PP_N * p + TBX_PP_PRE] * mom(M, e0) * mom(M, e1) * mom(M, e2);
         }
-        if (wHon * s != 0.0) {
-            atomicAdd(&accI[wv][3 + ish], 0.5 * wHon * s);
-            atomicAdd(&accI[wv][3 + jsh], 0.5 * wHon * s);
-        }
-    }
-
-    // shells of atom i per angular momentum
-    int nli[3] = {0, 0, 0};
-    for (int q = 0; q < nshi; ++q) ++nli[sp.ang[q + sp.msh * spi]];
-
-    for (int itr = 0; itr < mt.ntrans; ++itr) {
-        int c0 = 0;
-        while (c0 < cend) {
-            const int jfirst = mt.cart_atom[cart_m + c0];
-            int lo = jfirst, hi = jend;
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                int end = mt.at_cart[mid] + sp.ncart[mt.at_sp[mid]];
-                if (end <= c0 + rows_cap) lo = mid; else hi = mid;
-            }
-            const int jlast = lo;
-            const int c1 = mt.at_cart[jlast] + sp.ncart[mt.at_sp[jlast]];
-            const int nwin = jlast - jfirst + 1;
-            __syncthreads();
-            for (int k = tid; k < nwin * (3 + MAXSH); k += NTHREADS)
-                accJ[k / (3 + MAXSH)][k % (3 + MAXSH)] = 0.0;
-
-            // (1) shell pairs per type, type-major
-            int cnt[9];
-            const int j = jfirst + tid;
-            int nlj[3] = {0, 0, 0};
-            if (tid < nwin) {
-                const int spj = mt.at_sp[j];
-                for (int q = 0; q < sp.nsh[spj]; ++q) ++nlj[sp.ang[q + sp.msh * spj]];
-            }
-#pragma unroll
-            for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
-            int incl[9];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                incl[t] = wave_scan(cnt[t]);
-                if (lane == 63) wsum[wv][t] = incl[t];
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int so = 0, to = 0;
-                for (int t = 0; t < 9; ++t) {
-                    int tot = 0;
-                    for (int w = 0; w < NTHREADS / 64; ++w) tot += wsum[w][t];
-                    tsp[t] = so;
-                    toff[t] = to;
-                    so += tot;
-                    to += tot * ncart_of(t / 3) * ncart_of(t % 3);
-                }
-                tsp[9] = so;
-                toff[9] = to;
-            }
-            __syncthreads();
-            if (tid < nwin) {
-                const int spj = mt.at_sp[j];
-                for (int t = 0; t < 9; ++t) {
-                    if (cnt[t] == 0) continue;
-                    int off = tsp[t] + incl[t] - cnt[t];
-                    for (int w = 0; w < wv; ++w) off += wsum[w][t];
-                    for (int qi = 0; qi < nshi; ++qi) {
-                        if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
-                        for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
-                            if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
-                            spl[off++] = make_int4(j, qj, qi, t);
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-
-            // (2) task records; ntask <= TW_GRAD = NTHREADS: one task per lane
-            const int ntask = toff[9];
-            int nch = 0;
-            if (tid < ntask) {
-                int t = 0;
-                while (t < 8 && tid >= toff[t + 1]) ++t;
-                const int ncj = ncart_of(t % 3);
-                const int ncp = ncart_of(t / 3) * ncj;
-                const int local = tid - toff[t];
-                const int4 e = spl[tsp[t] + local / ncp];
-                const int cp = local % ncp;
-                const int ki = cp / ncj, kj = cp - (cp / ncj) * ncj;
-                int ppoff = 0;
-                const int kept =
-                    task_setup<MODE>(A, i, e.x, itr, e.z, e.y, ki, kj, rec + tid, TW_GRAD, ppoff);
-                tinfo[tid] = t | (ki << 4) | (kj << 8) | ((e.x - jfirst) << 12) |
-                             ((e.z) << 24) | ((e.y) << 27);
-                tpp[tid] = ppoff;
-                tkept[tid] = kept;
-                nch = (kept + CHUNK - 1) / CHUNK;
-            }
-            // (3) chunk offsets: block-wide exclusive scan
-            {
-                const int inc = wave_scan(nch);
-                if (lane == 63) wsum[wv][0] = inc;
-                __syncthreads();
-                int base = 0;
-                for (int w = 0; w < wv; ++w) base += wsum[w][0];
-                coff[tid] = base + inc - nch;
-                if (tid == NTHREADS - 1) coff[NTHREADS] = base + inc;
-                __syncthreads();
-            }
-            const int nchunk = coff[NTHREADS];
-
-            // (4) work items
-            for (int c = tid; c < nchunk; c += NTHREADS) {
-                int lo2 = 0, hi2 = NTHREADS;  // last task with coff <= c
-                while (hi2 - lo2 > 1) {
-                    const int mid = (lo2 + hi2) >> 1;
-                    if (coff[mid] <= c) lo2 = mid; else hi2 = mid;
-                }
-                const int tk = lo2;
-                const int info = tinfo[tk];
-                const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
-                const int jw = (info >> 12) & 4095;
-                const int ish = (info >> 24) & 7, jsh = (info >> 27) & 7;
-                const int q = c - coff[tk];
-                const int p0 = tpp[tk] + q * CHUNK;
-                const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
-                const double *r = rec + tk;
-                double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
-                switch (t) {
-                case 0: chunk_run<0, 0>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 1: chunk_run<0, 1>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 2: chunk_run<0, 2>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 3: chunk_run<1, 0>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 4: chunk_run<1, 1>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 5: chunk_run<1, 2>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 6: chunk_run<2, 0>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                case 7: chunk_run<2, 1>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                default: chunk_run<2, 2>(sp.pp, r, TW_GRAD, ki, kj, p0, p1, a3, s, dacc); break;
-                }
-                // explicit v dependence: S ex + 2 Wj D_i
-                const double *wq = r + R_WQ * TW_GRAD;
-                const double wq0 = wq[0], wq1 = wq[TW_GRAD], wq2 = wq[2 * TW_GRAD],
-                             wq3 = wq[3 * TW_GRAD], wq4 = wq[4 * TW_GRAD], wq5 = wq[5 * TW_GRAD];
-                const double WD0 = wq0 * dacc[0] + 0.5 * (wq1 * dacc[1] + wq3 * dacc[2]);
-                const double WD1 = wq2 * dacc[1] + 0.5 * (wq1 * dacc[0] + wq4 * dacc[2]);
-                const double WD2 = wq5 * dacc[2] + 0.5 * (wq3 * dacc[0] + wq4 * dacc[1]);
-                const double g0 = a3[0] + s * r[R_EX * TW_GRAD] + 2.0 * WD0;
-                const double g1 = a3[1] + s * r[(R_EX + 1) * TW_GRAD] + 2.0 * WD1;
-                const double g2 = a3[2] + s * r[(R_EX + 2) * TW_GRAD] + 2.0 * WD2;
-                const double vx = r[R_V * TW_GRAD], vy = r[(R_V + 1) * TW_GRAD],
-                             vz = r[(R_V + 2) * TW_GRAD];
-                const bool self = (jfirst + jw == i);
-                const double ds = s * r[R_HSW * TW_GRAD];
-                const double fs = self ? 0.25 : 0.5;
-                double *ai_ = accI[wv];
-                atomicAdd(&ai_[3 + MAXSH + 0], fs * 2.0 * vx * g0);
-                atomicAdd(&ai_[3 + MAXSH + 1], fs * (vx * g1 + g0 * vy));
-                atomicAdd(&ai_[3 + MAXSH + 2], fs * 2.0 * vy * g1);
-                atomicAdd(&ai_[3 + MAXSH + 3], fs * (vx * g2 + g0 * vz));
-                atomicAdd(&ai_[3 + MAXSH + 4], fs * (vy * g2 + g1 * vz));
-                atomicAdd(&ai_[3 + MAXSH + 5], fs * 2.0 * vz * g2);
-                atomicAdd(&ai_[3 + ish], ds);
-                if (self) {
-                    atomicAdd(&ai_[3 + jsh], ds);
-                } else {
-                    atomicAdd(&ai_[0], g0);
-                    atomicAdd(&ai_[1], g1);
-                    atomicAdd(&ai_[2], g2);
-                    atomicAdd(&accJ[jw][0], -g0);
-                    atomicAdd(&accJ[jw][1], -g1);
-                    atomicAdd(&accJ[jw][2], -g2);
-                    atomicAdd(&accJ[jw][3 + jsh], ds);
-                }
-            }
-            __syncthreads();
-            // (5) flush the window's atom-j sums
-            for (int k = tid; k < nwin; k += NTHREADS) {
-                const int jj = jfirst + k;
-                if (jj == i) continue;  // self images were added to atom i directly
-                for (int q = 0; q < 3; ++q)
-                    if (accJ[k][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * jj + q], accJ[k][q]);
-                const int nshj = sp.nsh[mt.at_sp[jj]];
-                for (int q = 0; q < nshj; ++q)
-                    if (accJ[k][3 + q] != 0.0)
-                        unsafeAtomicAdd(&A.dse[mt.at_sh[jj] + q], accJ[k][3 + q]);
-            }
-            c0 = c1;
-        }
-    }
-    // neighbour count for the list check: pairs (j <= i, image) within the cutoff
-    if (A.nnl) {
-        int n = 0;
-        for (int jj = a0 + tid; jj < jend; jj += NTHREADS) {
-            for (int itr = 0; itr < mt.ntrans; ++itr) {
-                const double vx = A.xyz[3 * i] - A.xyz[3 * jj] - mt.trans[3 * itr];
-                const double vy = A.xyz[3 * i + 1] - A.xyz[3 * jj + 1] - mt.trans[3 * itr + 1];
-                const double vz = A.xyz[3 * i + 2] - A.xyz[3 * jj + 2] - mt.trans[3 * itr + 2];
-                const double r2 = vx * vx + vy * vy + vz * vz;
-                if (r2 >= kEps && r2 <= sp.cutoff2) ++n;
-            }
-        }
-        atomicAdd(&s_count, n);
-    }
-    __syncthreads();
-    if (tid < 3 + MAXSH + 9) {
-        // sum the per-wavefront copies; sigma is expanded from the packed form to 3x3
-        const int k9 = tid - 3 - MAXSH;
-        const int kp = (tid < 3 + MAXSH) ? tid : 3 + MAXSH + kSigPack[k9];
-        double s = 0.0;
-        for (int w = 0; w < NTHREADS / 64; ++w) s += accI[w][kp];
         if (s != 0.0) {
-            if (tid < 3)
-                unsafeAtomicAdd(&A.grad[3 * i + tid], s);
-            else if (tid < 3 + MAXSH) {
-                if (tid - 3 < nshi) unsafeAtomicAdd(&A.dse[mt.at_sh[i] + tid - 3], s);
-            } else
-                unsafeAtomicAdd(&A.sigma[9 * m + k9], s);
+            unsafeAtomicAdd(&A.dse[mt.at_sh[i] + ish], 0.5 * wHon * s);
+            unsafeAtomicAdd(&A.dse[mt.at_sh[i] + jsh], 0.5 * wHon * s);
         }
     }
-    if (A.nnl && tid == 0 && s_count != A.nnl[i]) atomicOr(A.flag, 1);
+}
+
+// Neighbour-list check of the single-structure entry points: pairs (j <= i,
+// image) within the cutoff must match the caller's half-list counts.
+__global__ void k_listcheck(TbxSpecies sp, TbxMols mt, const double *xyz, const int *nnl,
+                            int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mt.nat) return;
+    const int a0 = mt.mol_atom[mt.at_mol[i]];
+    int n = 0;
+    for (int j = a0; j <= i; ++j)
+        for (int itr = 0; itr < mt.ntrans; ++itr) {
+            const double vx = xyz[3 * i] - xyz[3 * j] - mt.trans[3 * itr];
+            const double vy = xyz[3 * i + 1] - xyz[3 * j + 1] - mt.trans[3 * itr + 1];
+            const double vz = xyz[3 * i + 2] - xyz[3 * j + 2] - mt.trans[3 * itr + 2];
+            const double r2 = vx * vx + vy * vy + vz * vz;
+            if (r2 >= kEps && r2 <= sp.cutoff2) ++n;
+        }
+    if (n != nnl[i]) atomicOr(flag, 1);
 }
 
 // dEdcn (plain mode): sum over shells of dsedcn * dE/dse plus the on-site
@@ -1043,10 +1058,21 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
     GradArgs a{*sp, *mol, xyz, se, order, P, W, vao, vdp, vqp, nspin, pstride, vstride,
                Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag};
     (void)hipGetLastError();
-    if (mode == GRAD_PLAIN)
-        hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nat), dim3(NTHREADS), 0, stream, a);
-    else
-        hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nat), dim3(NTHREADS), 0, stream, a);
+    if (mol->nunit > 0) {
+        if (mode == GRAD_PLAIN)
+            hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nunit), dim3(NTHREADS), 0, stream,
+                               a);
+        else
+            hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nunit), dim3(NTHREADS), 0,
+                               stream, a);
+    }
+    const int per = NTHREADS / 16;
+    if (mode == GRAD_ADJOINT)
+        hipLaunchKernelGGL(k_onsite, dim3((mol->nat + per - 1) / per), dim3(NTHREADS), 0, stream,
+                           a);
+    if (nnl)
+        hipLaunchKernelGGL(k_listcheck, dim3((mol->nat + 255) / 256), dim3(256), 0, stream, *sp,
+                           *mol, xyz, nnl, flag);
     return hipGetLastError();
 }
 

@@ -49,6 +49,17 @@ struct TbxSpecies {
     double cutoff2;
 };
 
+// Gradient work units: consecutive (i, j, image) atom pairs of one molecule,
+// j in [j0, j1) for row atom i at lattice vector itr, packed so that a unit
+// holds at most TBX_UNIT_TASKS Cartesian-pair tasks, TBX_UNIT_PAIRS atom pairs
+// and TBX_UNIT_SEGS segments.  One workgroup runs one unit.
+#define TBX_UNIT_TASKS 256
+#define TBX_UNIT_PAIRS 128
+#define TBX_UNIT_SEGS 32
+struct TbxSeg {
+    int i, j0, j1, itr;
+};
+
 // Molecule / atom tables of a batch (device pointers).  "local" indices
 // count from the start of the atom's molecule.
 struct TbxMols {
@@ -67,4 +78,7 @@ struct TbxMols {
     const int *cart_atom;          // [total Cartesian] atom (global)
     const int *ao_atom;            // [total AO] atom (global)
     const int *mol_ao;             // [nmol+1] first AO (global)
+    int nunit;
+    const int *unit_seg;           // [nunit+1] first segment of each gradient unit
+    const TbxSeg *seg;             // gradient unit segments
 };
