@@ -20,11 +20,11 @@
 #include "tbx_tables.h"
 
 extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
-                                         const double *xyz, const double *se, const int *order,
-                                         double *S, double *D, double *Q, double *H,
-                                         const int *nnl, int *flag, hipStream_t stream);
+                                         const double *xyz, const double *se, double *S, double *D,
+                                         double *Q, double *H, long long ndense, const int *nnl,
+                                         int *flag, hipStream_t stream);
 extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const TbxMols *mol,
-                                          const double *xyz, const double *se, const int *order,
+                                          const double *xyz, const double *se,
                                           const double *P, const double *W, const double *vao,
                                           const double *vdp, const double *vqp, int nspin,
                                           long long pstride, long long vstride,
@@ -229,21 +229,23 @@ struct HostMols {
     int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, order, sh_off, unit_seg;
+        cart_atom, ao_atom, sh_off, unit_seg, funit_seg;
     std::vector<long long> mol_dense;
-    std::vector<TbxSeg> useg;
+    std::vector<TbxSeg> useg, fseg;
 };
 
-// Gradient work units (see TbxSeg): the half list j < i (j <= i with lattice
-// images) of every molecule, cut into units of bounded task count.
-static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans)
+// Work units (see TbxSeg): the half list of every molecule, cut into units of
+// bounded task count.  Gradient: j < i (j <= i with lattice images); forward:
+// j <= i, so that the on-site blocks are written by the same kernel.
+static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntrans, bool diag,
+                             std::vector<TbxSeg> &useg, std::vector<int> &unit_seg)
 {
-    hm.useg.clear();
-    hm.unit_seg.assign(1, 0);
+    useg.clear();
+    unit_seg.assign(1, 0);
     int tasks = 0, pairs = 0, segs = 0;
     bool open_seg = false;
     auto close_unit = [&]() {
-        if (segs > 0) hm.unit_seg.push_back((int)hm.useg.size());
+        if (segs > 0) unit_seg.push_back((int)useg.size());
         tasks = pairs = segs = 0;
         open_seg = false;
     };
@@ -251,7 +253,7 @@ static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans)
         const int a0 = hm.mol_atom[m], a1 = hm.mol_atom[m + 1];
         for (int itr = 0; itr < ntrans; ++itr)
             for (int i = a0; i < a1; ++i) {
-                const int jend = ntrans > 1 ? i + 1 : i;
+                const int jend = (diag || ntrans > 1) ? i + 1 : i;
                 const int nci = hs.ncart[hm.at_sp[i]];
                 open_seg = false;
                 for (int j = a0; j < jend; ++j) {
@@ -260,17 +262,23 @@ static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans)
                         (!open_seg && segs == TBX_UNIT_SEGS))
                         close_unit();
                     if (!open_seg) {
-                        hm.useg.push_back(TbxSeg{i, j, j, itr});
+                        useg.push_back(TbxSeg{i, j, j, itr});
                         ++segs;
                         open_seg = true;
                     }
-                    hm.useg.back().j1 = j + 1;
+                    useg.back().j1 = j + 1;
                     tasks += c;
                     ++pairs;
                 }
             }
         close_unit();  // units never span molecules
     }
+}
+
+static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans)
+{
+    build_units_list(hs, hm, ntrans, false, hm.useg, hm.unit_seg);
+    build_units_list(hs, hm, ntrans, true, hm.fseg, hm.funit_seg);
 }
 
 static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const int *id,
@@ -324,14 +332,6 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
     hm.nao = aog;
     hm.ncart = cg;
     hm.ndense = hm.mol_dense[nmol];
-    // panels ordered by work, heaviest first (rows x Cartesian columns)
-    std::vector<long long> cost(nat);
-    for (int a = 0; a < nat; ++a)
-        cost[a] = (long long)hm.mol_nao[hm.at_mol[a]] * hs.ncart[hm.at_sp[a]];
-    hm.order.resize(nat);
-    std::iota(hm.order.begin(), hm.order.end(), 0);
-    std::stable_sort(hm.order.begin(), hm.order.end(),
-                     [&](int x, int y) { return cost[x] > cost[y]; });
     return TBX_OK;
 }
 
@@ -340,7 +340,7 @@ struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
         ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, mol_ao, order, unit_seg, useg;
+        cart_atom, ao_atom, mol_ao, unit_seg, useg, funit_seg, fseg;
     TbxSpecies sp{};
     TbxMols mt{};
 
@@ -351,7 +351,8 @@ struct DevTables {
                          &pp, &pp_off,
                          &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
-                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &order, &unit_seg, &useg};
+                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &unit_seg, &useg,
+                         &funit_seg, &fseg};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -400,8 +401,8 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     hipError_t e;
 #define UP(name) if ((e = upload(d.name, hm.name, s)) != hipSuccess) return e
     UP(mol_atom); UP(mol_nao); UP(mol_cart); UP(mol_dense); UP(at_sp); UP(at_mol); UP(at_sh);
-    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(order); UP(unit_seg);
-    UP(useg);
+    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(unit_seg);
+    UP(useg); UP(funit_seg); UP(fseg);
 #undef UP
     if ((e = upload(d.trans, trans, s)) != hipSuccess) return e;
     TbxMols &mt = d.mt;
@@ -424,6 +425,9 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     mt.nunit = (int)hm.unit_seg.size() - 1;
     mt.unit_seg = d.unit_seg.as<int>();
     mt.seg = d.useg.as<TbxSeg>();
+    mt.nfunit = (int)hm.funit_seg.size() - 1;
+    mt.funit_seg = d.funit_seg.as<int>();
+    mt.fseg = d.fseg.as<TbxSeg>();
     return hipSuccess;
 }
 
@@ -588,10 +592,9 @@ extern "C" int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const
     HIP_TRY(ctx->Q.reserve(sizeof(double) * 6 * n2));
     HIP_TRY(ctx->H.reserve(sizeof(double) * n2));
     HIP_TRY(tbx_launch_forward(&ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
-                               ctx->se.as<double>(), ctx->tab.order.as<int>(),
-                               ctx->S.as<double>(), ctx->D.as<double>(), ctx->Q.as<double>(),
-                               ctx->H.as<double>(), list ? ctx->nnl.as<int>() : nullptr,
-                               ctx->flag.as<int>(), s));
+                               ctx->se.as<double>(), ctx->S.as<double>(), ctx->D.as<double>(),
+                               ctx->Q.as<double>(), ctx->H.as<double>(), hm.ndense,
+                               list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), s));
     HIP_TRY(hipMemcpyAsync(overlap, ctx->S.p, sizeof(double) * n2, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(dpint, ctx->D.p, sizeof(double) * 3 * n2, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(qpint, ctx->Q.p, sizeof(double) * 6 * n2, hipMemcpyDeviceToHost, s));
@@ -641,8 +644,7 @@ extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *ge
     HIP_TRY(tbx_launch_fill(ctx->dse.as<double>(), hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_fill(ctx->sigma.as<double>(), 9, 0.0, s));
     HIP_TRY(tbx_launch_gradient(0, &ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
-                                ctx->se.as<double>(), ctx->tab.order.as<int>(),
-                                ctx->P.as<double>(), ctx->W.as<double>(), ctx->vao.as<double>(),
+                                ctx->se.as<double>(), ctx->P.as<double>(), ctx->W.as<double>(), ctx->vao.as<double>(),
                                 ctx->vdp.as<double>(), ctx->vqp.as<double>(), nspin, (long long)n2,
                                 nao, nullptr, nullptr, nullptr, nullptr, ctx->grad.as<double>(),
                                 ctx->dse.as<double>(), ctx->sigma.as<double>(),
@@ -758,8 +760,8 @@ extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double
     if (!b || !d_xyz || !d_se || !d_S || !d_D || !d_Q || !d_H) return fail(TBX_ERR_ARG, "null argument");
     hipStream_t s = pick_stream(b, stream);
     HIP_TRY(mark(b, 0, s));
-    HIP_TRY(tbx_launch_forward(&b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(), d_S,
-                               d_D, d_Q, d_H, nullptr, b->flag.as<int>(), s));
+    HIP_TRY(tbx_launch_forward(&b->tab.sp, &b->tab.mt, d_xyz, d_se, d_S, d_D, d_Q, d_H,
+                               b->hm.ndense, nullptr, b->flag.as<int>(), s));
     HIP_TRY(mark(b, 1, s));
     return TBX_OK;
 }
@@ -777,8 +779,7 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
     HIP_TRY(tbx_launch_fill(d_grad, 3LL * b->hm.nat, 0.0, s));
     HIP_TRY(tbx_launch_fill(b->dse_tmp.as<double>(), b->hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
-    HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(),
-                                d_P, d_W, d_vao, d_vdp, d_vqp, 1, b->hm.ndense, b->hm.nao,
+    HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp, d_vqp, 1, b->hm.ndense, b->hm.nao,
                                 nullptr, nullptr, nullptr, nullptr, d_grad,
                                 b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(), s));
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
@@ -797,8 +798,7 @@ extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double
     HIP_TRY(mark(b, 6, s));
     HIP_TRY(tbx_launch_fill(d_dxyz, 3LL * b->hm.nat, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_dse, b->hm.nsh, 0.0, s));
-    HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, b->tab.order.as<int>(),
-                                nullptr, nullptr, nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar,
+    HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar,
                                 d_Dbar, d_Qbar, d_Hbar, d_dxyz, d_dse, b->dse_tmp.as<double>(),
                                 nullptr, b->flag.as<int>(), s));
     HIP_TRY(mark(b, 7, s));

@@ -201,186 +201,6 @@ __device__ __forceinline__ void unpack_exp(int e, int x[3])
 static constexpr int kQm[6][3] = {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}};
 
 // ---------------------------------------------------------------------------
-// Forward: dense S, D, Q, H0 column panels.
-
-struct FwdArgs {
-    TbxSpecies sp;
-    TbxMols mol;
-    const double *xyz;   // [3*nat]
-    const double *se;    // [nsh]
-    const int *order;    // [nat] panel order (heaviest first)
-    double *S, *D, *Q, *H;
-    const int *nnl;      // optional [nat]: expected half-list counts (NULL: no check)
-    int *flag;           // set to 1 on a list mismatch
-};
-
-__global__ void __launch_bounds__(NTHREADS) k_forward(FwdArgs A)
-{
-    __shared__ double cart[TW_FWD * 11];
-    __shared__ int s_count;
-    const TbxSpecies &sp = A.sp;
-    const TbxMols &mt = A.mol;
-    const int i = A.order[blockIdx.x];
-    const int m = mt.at_mol[i];
-    const int a0 = mt.mol_atom[m], a1 = mt.mol_atom[m + 1];
-    const int nao = mt.mol_nao[m];
-    const int spi = mt.at_sp[i];
-    const int nci = sp.ncart[spi], naoi = sp.nao[spi];
-    const double xi = A.xyz[3 * i], yi = A.xyz[3 * i + 1], zi = A.xyz[3 * i + 2];
-    const long long dense0 = mt.mol_dense[m];
-    const int cart_m = mt.mol_cart[m];
-    const int ncart_mol = mt.mol_cart[m + 1] - cart_m;
-    const int ao_m = mt.mol_ao[m];
-    const int iao0 = mt.at_ao[i];
-    const int rows_cap = TW_FWD / nci;  // Cartesian rows per window
-    if (threadIdx.x == 0) s_count = 0;
-
-    int c0 = 0;
-    while (c0 < ncart_mol) {
-        // window = atoms j whose Cartesian rows fit in [c0, c0 + rows_cap)
-        const int jfirst = mt.cart_atom[cart_m + c0];
-        int lo = jfirst, hi = a1;  // last atom with at_cart <= c0 + rows_cap - ncart(j) -> bsearch on start
-        // largest jl in [jfirst, a1) with at_cart[jl] + ncart[jl] <= c0 + rows_cap
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            int end = mt.at_cart[mid] + sp.ncart[mt.at_sp[mid]];
-            if (end <= c0 + rows_cap) lo = mid; else hi = mid;
-        }
-        const int jlast = lo;
-        const int c1 = mt.at_cart[jlast] + sp.ncart[mt.at_sp[jlast]];
-        const int ntask = (c1 - c0) * nci;
-        __syncthreads();
-
-        // Phase A: Cartesian pair integrals into LDS
-        for (int t = threadIdx.x; t < ntask; t += NTHREADS) {
-            const int cjl = t / nci, ci = t - cjl * nci;
-            const int j = mt.cart_atom[cart_m + c0 + cjl];
-            const int spj = mt.at_sp[j];
-            const int cj = c0 + cjl - mt.at_cart[j];
-            const int slj = sp.cart_sh[spj * TBX_MAXCART + cj];
-            const int sli = sp.cart_sh[spi * TBX_MAXCART + ci];
-            int ej[3], ei[3];
-            unpack_exp(sp.cart_exp[spj * TBX_MAXCART + cj], ej);
-            unpack_exp(sp.cart_exp[spi * TBX_MAXCART + ci], ei);
-            const int slotj = slj + sp.msh * spj, sloti = sli + sp.msh * spi;
-            const int npj = sp.nprim[slotj], npi = sp.nprim[sloti];
-            const double dx0 = xi - A.xyz[3 * j], dy0 = yi - A.xyz[3 * j + 1],
-                         dz0 = zi - A.xyz[3 * j + 2];
-            const double hsc = sp.hscale[slj + sp.msh * (sli + sp.msh * (spj + sp.nsp * spi))];
-            const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
-            const double radij = sp.rad[spi] + sp.rad[spj];
-            double acc[11];
-#pragma unroll
-            for (int q = 0; q < 11; ++q) acc[q] = 0.0;
-            int nimg = 0;
-            for (int itr = 0; itr < mt.ntrans; ++itr) {
-                const double vx = dx0 - mt.trans[3 * itr], vy = dy0 - mt.trans[3 * itr + 1],
-                             vz = dz0 - mt.trans[3 * itr + 2];
-                const double r2 = vx * vx + vy * vy + vz * vz;
-                double wH;
-                if (r2 < kEps) {
-                    if (j != i) continue;  // coinciding distinct atoms: not in the list
-                    wH = 1.0;             // on-site block: H = S (se_i + se_j)/2
-                } else {
-                    if (r2 > sp.cutoff2) continue;
-                    ++nimg;
-                    const double rr = sqrt(sqrt(r2) / radij);
-                    wH = hsc * (1.0 + kpi * rr) * (1.0 + kpj * rr);
-                }
-                const double vec[3] = {vx, vy, vz};
-                for (int ip = 0; ip < npi; ++ip) {
-                    const double ai = sp.alpha[sloti * MAXG + ip];
-                    const double ci_ = sp.coeff[sloti * MAXG + ip];
-                    for (int jp = 0; jp < npj; ++jp) {
-                        const double aj = sp.alpha[slotj * MAXG + jp];
-                        const double eab = ai + aj;
-                        const double oab = 1.0 / eab;
-                        const double est = ai * aj * r2 * oab;
-                        if (est > sp.intcut) continue;
-                        const double so = sqrt(oab);
-                        const double cc = exp(-est) * kSqrtPi3 * so * so * so * ci_ *
-                                          sp.coeff[slotj * MAXG + jp];
-                        const Mom M = make_mom(0.5 * oab);
-                        double V[3][3];
-#pragma unroll
-                        for (int k = 0; k < 3; ++k)
-                            fwd1d(ei[k], ej[k], -vec[k] * aj * oab, vec[k] * ai * oab, M, V[k]);
-                        const double s = V[0][0] * V[1][0] * V[2][0];
-                        acc[0] += cc * s;
-                        acc[1] += cc * V[0][1] * V[1][0] * V[2][0];
-                        acc[2] += cc * V[0][0] * V[1][1] * V[2][0];
-                        acc[3] += cc * V[0][0] * V[1][0] * V[2][1];
-#pragma unroll
-                        for (int q = 0; q < 6; ++q)
-                            acc[4 + q] += cc * V[0][kQm[q][0]] * V[1][kQm[q][1]] * V[2][kQm[q][2]];
-                        acc[10] += wH * cc * s;
-                    }
-                }
-            }
-            if (A.nnl && ci == 0 && cj == 0 && j <= i && nimg > 0)
-                atomicAdd(&s_count, nimg);
-#pragma unroll
-            for (int q = 0; q < 11; ++q) cart[t * 11 + q] = acc[q];
-        }
-        __syncthreads();
-
-        // Phase B: spherical transform and column writes of rows [r0, r1)
-        const int r0 = mt.at_ao[jfirst];
-        const int r1 = mt.at_ao[jlast] + sp.nao[mt.at_sp[jlast]];
-        const int nrow = r1 - r0;
-        for (int e = threadIdx.x; e < nrow * naoi; e += NTHREADS) {
-            const int col = e / nrow, rl = e - col * nrow;
-            const int mu = r0 + rl;
-            const int j = mt.ao_atom[ao_m + mu];
-            const int spj = mt.at_sp[j];
-            const int aoj = mu - mt.at_ao[j];
-            const int shj = sp.ao_sh[spj * TBX_MAXAO + aoj], mj = sp.ao_m[spj * TBX_MAXAO + aoj];
-            const int shi = sp.ao_sh[spi * TBX_MAXAO + col], mi = sp.ao_m[spi * TBX_MAXAO + col];
-            const int lj = sp.ang[shj + sp.msh * spj], li = sp.ang[shi + sp.msh * spi];
-            const int cbase_j = mt.at_cart[j] - c0 + sp.sh_cart[shj + sp.msh * spj];
-            const int cbase_i = sp.sh_cart[shi + sp.msh * spi];
-            double v[11];
-#pragma unroll
-            for (int q = 0; q < 11; ++q) v[q] = 0.0;
-            const int nj = kSphN[lj][mj], ni = kSphN[li][mi];
-            for (int a = 0; a < nj; ++a) {
-                const int rowc = cbase_j + kSphC[lj][mj][a];
-                const double wa = kSphW[lj][mj][a];
-                for (int b = 0; b < ni; ++b) {
-                    const double w = wa * kSphW[li][mi][b];
-                    const double *src = &cart[(rowc * nci + cbase_i + kSphC[li][mi][b]) * 11];
-#pragma unroll
-                    for (int q = 0; q < 11; ++q) v[q] += w * src[q];
-                }
-            }
-            const double tr = 0.5 * (v[4] + v[6] + v[9]);
-            const int nu = iao0 + col;
-            const long long idx = dense0 + (long long)nu * nao + mu;
-            const double hij = 0.5 * (A.se[mt.at_sh[i] + shi] + A.se[mt.at_sh[j] + shj]);
-            A.S[idx] = v[0];
-            A.H[idx] = hij * v[10];
-            double *dp = A.D + 3 * idx;
-            dp[0] = v[1];
-            dp[1] = v[2];
-            dp[2] = v[3];
-            double *qp = A.Q + 6 * idx;
-            qp[0] = 1.5 * v[4] - tr;
-            qp[1] = 1.5 * v[5];
-            qp[2] = 1.5 * v[6] - tr;
-            qp[3] = 1.5 * v[7];
-            qp[4] = 1.5 * v[8];
-            qp[5] = 1.5 * v[9] - tr;
-        }
-        c0 = c1;
-    }
-    if (A.nnl) {
-        __syncthreads();
-        if (threadIdx.x == 0 && s_count != A.nnl[i]) atomicOr(A.flag, 1);
-    }
-    (void)a0;
-}
-
-// ---------------------------------------------------------------------------
 // Gradient / adjoint contraction.
 
 enum { GRAD_PLAIN = 0, GRAD_ADJOINT = 1 };
@@ -389,7 +209,6 @@ struct GradArgs {
     TbxSpecies sp;
     TbxMols mol;
     const double *xyz, *se;
-    const int *order;
     // plain: P, W dense per molecule (spin planes at +spin*pstride), vao [nao_tot*nspin],
     // vdp [3*nat*nspin], vqp [6*nat*nspin]
     const double *P, *W, *vao, *vdp, *vqp;
@@ -531,6 +350,403 @@ __device__ __forceinline__ int kept_pairs(const double *pp, int n, double r2, do
         if (pp[TBX_PP_N * mid + TBX_PP_MU] * r2 <= intcut) lo = mid + 1; else hi = mid;
     }
     return lo;
+}
+
+// Unit staging shared by the forward and gradient kernels (LDS)
+struct UnitStage {
+    PairInfo pinf[TBX_UNIT_PAIRS];
+    double pv[TBX_UNIT_PAIRS][3];  // v = R_i - R_j - T
+    int pseg[TBX_UNIT_PAIRS];      // segment of each pair
+    int sfirst[TBX_UNIT_SEGS + 1]; // first pair of each segment
+    int4 spl[TBX_UNIT_TASKS];      // shell pairs (pair, jsh, ish, type), type-major
+    int wsum[NTHREADS / 64][9];
+    int toff[10], tsp[10];         // task / shell-pair offsets per type
+};
+
+// Stage the pairs of unit segments [s0, s0 + nseg) and lay out their shell
+// pairs by type.  spidx (optional): shell-pair index of (pair, ish, jsh).
+__device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double *xyz,
+                           const TbxSeg *seg, int s0, int nseg, UnitStage &U, int *spidx)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) {
+        int o = 0;
+        for (int k = 0; k < nseg; ++k) {
+            U.sfirst[k] = o;
+            o += seg[s0 + k].j1 - seg[s0 + k].j0;
+        }
+        U.sfirst[nseg] = o;
+    }
+    __syncthreads();
+    const int npair = U.sfirst[nseg];
+    int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (tid < npair) {
+        int k = 0;
+        while (k + 1 < nseg && U.sfirst[k + 1] <= tid) ++k;
+        const TbxSeg g = seg[s0 + k];
+        PairInfo pr;
+        pr.i = g.i;
+        pr.j = g.j0 + tid - U.sfirst[k];
+        pr.spi = mt.at_sp[pr.i];
+        pr.spj = mt.at_sp[pr.j];
+        pr.aoi = mt.at_ao[pr.i];
+        pr.aoj = mt.at_ao[pr.j];
+        pr.shi = mt.at_sh[pr.i];
+        pr.shj = mt.at_sh[pr.j];
+        U.pinf[tid] = pr;
+        U.pseg[tid] = k;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            U.pv[tid][c] = xyz[3 * pr.i + c] - xyz[3 * pr.j + c] - mt.trans[3 * g.itr + c];
+        int nli[3] = {0, 0, 0}, nlj[3] = {0, 0, 0};
+        for (int q = 0; q < sp.nsh[pr.spi]; ++q) ++nli[sp.ang[q + sp.msh * pr.spi]];
+        for (int q = 0; q < sp.nsh[pr.spj]; ++q) ++nlj[sp.ang[q + sp.msh * pr.spj]];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
+    }
+    int incl[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        incl[t] = wave_scan(cnt[t]);
+        if (lane == 63) U.wsum[wv][t] = incl[t];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int so = 0, to = 0;
+        for (int t = 0; t < 9; ++t) {
+            int tot = 0;
+            for (int w = 0; w < NTHREADS / 64; ++w) tot += U.wsum[w][t];
+            U.tsp[t] = so;
+            U.toff[t] = to;
+            so += tot;
+            to += tot * ncart_of(t / 3) * ncart_of(t % 3);
+        }
+        U.tsp[9] = so;
+        U.toff[9] = to;
+    }
+    __syncthreads();
+    if (tid < npair) {
+        const int spi = U.pinf[tid].spi, spj = U.pinf[tid].spj;
+        for (int t = 0; t < 9; ++t) {
+            if (cnt[t] == 0) continue;
+            int off = U.tsp[t] + incl[t] - cnt[t];
+            for (int w = 0; w < wv; ++w) off += U.wsum[w][t];
+            for (int qi = 0; qi < sp.nsh[spi]; ++qi) {
+                if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
+                for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
+                    if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
+                    if (spidx) spidx[tid * MAXSH * MAXSH + qi * MAXSH + qj] = off;
+                    U.spl[off++] = make_int4(tid, qj, qi, t);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Decode task tid of a staged unit: shell pair index, type and Cartesian pair.
+__device__ __forceinline__ int4 decode_task(const UnitStage &U, int task, int &spx)
+{
+    int t = 0;
+    while (t < 8 && task >= U.toff[t + 1]) ++t;
+    const int ncj = ncart_of(t % 3);
+    const int ncp = ncart_of(t / 3) * ncj;
+    const int local = task - U.toff[t];
+    spx = U.tsp[t] + local / ncp;
+    const int cp = local % ncp;
+    return make_int4(t, cp / ncj, cp % ncj, cp);  // (type, ki, kj, cp)
+}
+
+// block-wide exclusive scan of one int per lane; returns the total
+__device__ __forceinline__ int block_scan(int v, int *out, int (*wsum)[9])
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int inc = wave_scan(v);
+    __syncthreads();  // wsum may still be read by the caller's previous phase
+    if (lane == 63) wsum[wv][0] = inc;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wv; ++w) base += wsum[w][0];
+    out[tid] = base + inc - v;
+    if (tid == NTHREADS - 1) out[NTHREADS] = base + inc;
+    __syncthreads();
+    return out[NTHREADS];
+}
+
+// last index k in [0, n) with off[k] <= c (off non-decreasing, off[0] <= c)
+__device__ __forceinline__ int find_last_le(const int *off, int n, int c)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= c) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Forward: dense S, D, Q, H0 over work units of the half list (j <= i).
+
+struct FwdArgs {
+    TbxSpecies sp;
+    TbxMols mol;
+    const double *xyz;   // [3*nat]
+    const double *se;    // [nsh]
+    double *S, *D, *Q, *H;
+    int accumulate;      // 1: add into zeroed outputs (lattice images), 0: plain stores
+};
+
+// 1D factors V[m] = int (y+pa)^(a+m) (y+pb)^b, m = 0..2 (moments about A)
+template <int LI, int LJ>
+__device__ __forceinline__ void fwd1d_t(int a, int b, double pa, double pb, const Mom &M,
+                                        double V[3])
+{
+    double A0[LI + 1], B[LJ + 1], P[LI + LJ + 1];
+    pow_poly<LI + 1>(a, pa, A0);
+    pow_poly<LJ + 1>(b, pb, B);
+#pragma unroll
+    for (int k = 0; k < LI + LJ + 1; ++k) P[k] = 0.0;
+#pragma unroll
+    for (int x = 0; x <= LI; ++x)
+#pragma unroll
+        for (int y = 0; y <= LJ; ++y) P[x + y] += A0[x] * B[y];
+    mom3<LI + LJ + 1>(P, pa, M, V[0], V[1], V[2]);
+}
+
+// Primitive pairs [p0, p1) of one Cartesian pair: S, D (3), Q (6, raw) about i
+template <int LI, int LJ>
+__device__ __forceinline__ void fchunk_run(const double *__restrict__ pp, const double vec[3],
+                                           int ki, int kj, int p0, int p1, double o[10])
+{
+    const int ei[3] = {kCartExp[LI][ki][0], kCartExp[LI][ki][1], kCartExp[LI][ki][2]};
+    const int ej[3] = {kCartExp[LJ][kj][0], kCartExp[LJ][kj][1], kCartExp[LJ][kj][2]};
+    const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    for (int p = p0; p < p1; ++p) {
+        const double *e = pp + TBX_PP_N * p;
+        const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
+        const double h = e[TBX_PP_H], pre = e[TBX_PP_PRE];
+        const double cc = exp(-(mu * r2)) * pre;
+        const Mom M = make_mom(h);
+        double V[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd1d_t<LI, LJ>(ei[k], ej[k], -vec[k] * ajo, vec[k] * aio, M, V[k]);
+        const double yz = V[1][0] * V[2][0];
+        o[0] += cc * (V[0][0] * yz);
+        o[1] += cc * (V[0][1] * yz);
+        o[2] += cc * (V[0][0] * V[1][1] * V[2][0]);
+        o[3] += cc * (V[0][0] * V[1][0] * V[2][1]);
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            o[4 + q] += cc * (V[0][kQm[q][0]] * V[1][kQm[q][1]] * V[2][kQm[q][2]]);
+    }
+}
+
+__device__ __forceinline__ void put(double *p, double v, bool acc)
+{
+    if (acc) unsafeAtomicAdd(p, v); else *p = v;
+}
+
+// One workgroup per forward unit.  (1) stage pairs and shell pairs; (2) one
+// lane per Cartesian-pair task counts its kept primitive pairs, the first
+// lane of each shell pair also forms its H0 factor; (3) chunks of kept pairs
+// are scanned into work items; (4) work items accumulate Cartesian S, D, Q
+// about i in LDS; (5) one lane per dense element applies the spherical
+// transform and writes the block (rows j, column i: operator on i) and its
+// mirror (rows i, column j: operator shifted to j, xtb/h0.f90:704-747) --
+// every element of the molecule's matrices is written exactly once.
+__global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
+{
+    __shared__ UnitStage U;
+    __shared__ double acc[10][TBX_UNIT_TASKS];
+    __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
+    __shared__ int spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
+    __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | pair << 12
+    __shared__ int tpp[TBX_UNIT_TASKS];
+    __shared__ int tkept[TBX_UNIT_TASKS];
+    __shared__ int coff[TBX_UNIT_TASKS + 1];
+    __shared__ int eoff[2 * TBX_UNIT_SEGS + 1];  // element offsets: A blocks, then mirrors
+    __shared__ int erow[TBX_UNIT_SEGS][3];       // first row, rows of A, rows of the mirror
+    const TbxSpecies &sp = A.sp;
+    const TbxMols &mt = A.mol;
+    const int tid = threadIdx.x;
+    const int u = blockIdx.x;
+    const int s0 = mt.funit_seg[u], nseg = mt.funit_seg[u + 1] - s0;
+    const int m = mt.at_mol[mt.fseg[s0].i];
+    const int nao = mt.mol_nao[m];
+    const long long dense0 = mt.mol_dense[m];
+    const int ao_m = mt.mol_ao[m];
+    const bool accum = A.accumulate != 0;
+    for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
+    stage_unit(sp, mt, A.xyz, mt.fseg, s0, nseg, U, spidx);
+
+    // (2) tasks
+    const int ntask = U.toff[9];
+    int nch = 0;
+    if (tid < ntask) {
+        int spx;
+        const int4 d = decode_task(U, tid, spx);
+        const int4 e = U.spl[spx];
+        const PairInfo pr = U.pinf[e.x];
+        const double vx = U.pv[e.x][0], vy = U.pv[e.x][1], vz = U.pv[e.x][2];
+        const double r2 = vx * vx + vy * vy + vz * vz;
+        const int ish = e.z, jsh = e.y;
+        const int sloti = ish + sp.msh * pr.spi, slotj = jsh + sp.msh * pr.spj;
+        const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
+        const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
+        const bool onsite = (pr.i == pr.j) && r2 < kEps;
+        const bool inrange = onsite || (r2 >= kEps && r2 <= sp.cutoff2);
+        const int kept = inrange ? kept_pairs(sp.pp + (long long)TBX_PP_N * pi0, npp,
+                                              onsite ? 0.0 : r2, sp.intcut)
+                                 : 0;
+        if (d.w == 0) {
+            const double sesum = 0.5 * (A.se[pr.shi + ish] + A.se[pr.shj + jsh]);
+            double hf = 0.0;
+            if (onsite) {
+                hf = sesum;
+            } else if (inrange) {
+                const double hsc =
+                    sp.hscale[jsh + sp.msh * (ish + sp.msh * (pr.spj + sp.nsp * pr.spi))];
+                const double rr = sqrt(sqrt(r2) / (sp.rad[pr.spi] + sp.rad[pr.spj]));
+                hf = sesum * hsc * (1.0 + sp.shpoly[sloti] * rr) * (1.0 + sp.shpoly[slotj] * rr);
+            }
+            hfac[spx] = hf;
+        }
+        tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12);
+        tpp[tid] = pi0;
+        tkept[tid] = kept;
+        nch = (kept + CHUNK - 1) / CHUNK;
+    }
+    // (3) chunk offsets
+    const int nchunk = block_scan(nch, coff, U.wsum);
+
+    // (4) work items
+    for (int c = tid; c < nchunk; c += NTHREADS) {
+        const int tk = find_last_le(coff, NTHREADS, c);
+        const int info = tinfo[tk];
+        const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
+        const int p = (info >> 12) & 4095;
+        const int q = c - coff[tk];
+        const int p0 = tpp[tk] + q * CHUNK;
+        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
+        const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
+        double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        switch (t) {
+        case 0: fchunk_run<0, 0>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 1: fchunk_run<0, 1>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 2: fchunk_run<0, 2>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 3: fchunk_run<1, 0>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 4: fchunk_run<1, 1>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 5: fchunk_run<1, 2>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 6: fchunk_run<2, 0>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        case 7: fchunk_run<2, 1>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        default: fchunk_run<2, 2>(sp.pp, vec, ki, kj, p0, p1, o); break;
+        }
+#pragma unroll
+        for (int k = 0; k < 10; ++k) atomicAdd(&acc[k][tk], o[k]);
+    }
+
+    // (5) element layout per segment: block A (rows of atoms j0..j1-1, the
+    // columns of atom i) and the mirror (rows of atom i, columns of the j != i)
+    if (tid == 0) {
+        int o = 0;
+        for (int k = 0; k < nseg; ++k) {
+            const TbxSeg g = mt.fseg[s0 + k];
+            const PairInfo &first = U.pinf[U.sfirst[k]];
+            const PairInfo &last = U.pinf[U.sfirst[k + 1] - 1];
+            const int r0 = first.aoj;
+            const int r1 = last.aoj + sp.nao[last.spj];
+            const int jb = min(g.j1, g.i);  // mirrored atoms: j < i
+            const int rb1 = (jb > g.j0) ? U.pinf[U.sfirst[k] + (jb - 1 - g.j0)].aoj +
+                                              sp.nao[U.pinf[U.sfirst[k] + (jb - 1 - g.j0)].spj]
+                                        : r0;
+            erow[k][0] = r0;
+            erow[k][1] = r1 - r0;
+            erow[k][2] = rb1 - r0;
+            eoff[k] = o;
+            o += sp.nao[first.spi] * (r1 - r0);
+        }
+        for (int k = 0; k < nseg; ++k) {
+            eoff[nseg + k] = o;
+            o += sp.nao[U.pinf[U.sfirst[k]].spi] * erow[k][2];
+        }
+        eoff[2 * nseg] = o;
+    }
+    __syncthreads();
+    const int nel = eoff[2 * nseg];
+    for (int e = tid; e < nel; e += NTHREADS) {
+        const int kk = find_last_le(eoff, 2 * nseg, e);
+        const bool mirror = kk >= nseg;
+        const int k = mirror ? kk - nseg : kk;
+        const int local = e - eoff[kk];
+        const int spi = U.pinf[U.sfirst[k]].spi;
+        const int naoi = sp.nao[spi];
+        int mu, nul;
+        if (!mirror) {
+            nul = local / erow[k][1];
+            mu = erow[k][0] + local - nul * erow[k][1];
+        } else {
+            mu = erow[k][0] + local / naoi;
+            nul = local - (local / naoi) * naoi;
+        }
+        const int jj = mt.ao_atom[ao_m + mu];
+        const int p = U.sfirst[k] + (jj - mt.fseg[s0 + k].j0);
+        const PairInfo pr = U.pinf[p];
+        const int mul = mu - pr.aoj;
+        const int ish = sp.ao_sh[spi * TBX_MAXAO + nul], mi = sp.ao_m[spi * TBX_MAXAO + nul];
+        const int jsh = sp.ao_sh[pr.spj * TBX_MAXAO + mul], mj = sp.ao_m[pr.spj * TBX_MAXAO + mul];
+        const int li = sp.ang[ish + sp.msh * spi], lj = sp.ang[jsh + sp.msh * pr.spj];
+        const int spx = spidx[p * MAXSH * MAXSH + ish * MAXSH + jsh];
+        const int t = li * 3 + lj;
+        const int ncj = ncart_of(lj);
+        const int task0 = U.toff[t] + (spx - U.tsp[t]) * ncart_of(li) * ncj;
+        double v[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) v[q] = 0.0;
+        for (int a = 0; a < kSphN[lj][mj]; ++a) {
+            const double wa = kSphW[lj][mj][a];
+            const int cj = kSphC[lj][mj][a];
+            for (int b = 0; b < kSphN[li][mi]; ++b) {
+                const double w = wa * kSphW[li][mi][b];
+                const int tk = task0 + kSphC[li][mi][b] * ncj + cj;
+#pragma unroll
+                for (int q = 0; q < 10; ++q) v[q] += w * acc[q][tk];
+            }
+        }
+        const double s = v[0];
+        const double h = hfac[spx] * s;
+        const double tr = 0.5 * (v[4] + v[6] + v[9]);
+        double d[3] = {v[1], v[2], v[3]};
+        double qv[6] = {1.5 * v[4] - tr, 1.5 * v[5], 1.5 * v[6] - tr,
+                        1.5 * v[7], 1.5 * v[8], 1.5 * v[9] - tr};
+        long long idx;
+        const int nu = pr.aoi + nul;
+        if (!mirror) {
+            idx = dense0 + (long long)nu * nao + mu;
+        } else {
+            // operator on j: shift the moments from i to j (x - R_j = (x - R_i) + v)
+            idx = dense0 + (long long)mu * nao + nu;
+            const double vv[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
+            double sq[6];
+            sq[0] = 2 * vv[0] * d[0] + vv[0] * vv[0] * s;
+            sq[2] = 2 * vv[1] * d[1] + vv[1] * vv[1] * s;
+            sq[5] = 2 * vv[2] * d[2] + vv[2] * vv[2] * s;
+            sq[1] = vv[0] * d[1] + vv[1] * d[0] + vv[0] * vv[1] * s;
+            sq[3] = vv[0] * d[2] + vv[2] * d[0] + vv[0] * vv[2] * s;
+            sq[4] = vv[1] * d[2] + vv[2] * d[1] + vv[1] * vv[2] * s;
+            const double ts = 0.5 * (sq[0] + sq[2] + sq[5]);
+            qv[0] += 1.5 * sq[0] - ts;
+            qv[1] += 1.5 * sq[1];
+            qv[2] += 1.5 * sq[2] - ts;
+            qv[3] += 1.5 * sq[3];
+            qv[4] += 1.5 * sq[4];
+            qv[5] += 1.5 * sq[5] - ts;
+            for (int c = 0; c < 3; ++c) d[c] += vv[c] * s;
+        }
+        put(A.S + idx, s, accum);
+        put(A.H + idx, h, accum);
+        for (int c = 0; c < 3; ++c) put(A.D + 3 * idx + c, d[c], accum);
+        for (int c = 0; c < 6; ++c) put(A.Q + 6 * idx + c, qv[c], accum);
+    }
 }
 
 // Task set-up: one Cartesian function pair (ki of shell ish on atom i, kj of
@@ -705,152 +921,53 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
 template <int MODE>
 __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
 {
+    __shared__ UnitStage U;
     __shared__ double rec[NREC * TBX_UNIT_TASKS];
     __shared__ double accP[TBX_UNIT_PAIRS][3 + 2 * MAXSH];  // G (3), dse_i, dse_j per shell
-    __shared__ double pv[TBX_UNIT_PAIRS][3];
-    __shared__ PairInfo pinf[TBX_UNIT_PAIRS];
-    __shared__ int pseg[TBX_UNIT_PAIRS];
     __shared__ double accS[TBX_UNIT_SEGS][3 + MAXSH];      // atom-i sums per segment
     __shared__ double sig[6];
-    __shared__ int sfirst[TBX_UNIT_SEGS + 1];
-    __shared__ int4 spl[TBX_UNIT_TASKS];  // (pair, jsh, ish, type), type-major
     __shared__ int tinfo[TBX_UNIT_TASKS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
     __shared__ int tpp[TBX_UNIT_TASKS];    // first primitive pair of the task
     __shared__ int tkept[TBX_UNIT_TASKS];
     __shared__ int coff[TBX_UNIT_TASKS + 1];  // first chunk of each task
-    __shared__ int wsum[NTHREADS / 64][9];
-    __shared__ int toff[10], tsp[10];         // task / shell-pair offsets per type
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x;
     const int u = blockIdx.x;
     const int s0 = mt.unit_seg[u], nseg = mt.unit_seg[u + 1] - s0;
     const int m = mt.at_mol[mt.seg[s0].i];
     const int nao = mt.mol_nao[m];
     const long long dense0 = mt.mol_dense[m];
     const int ao_m = mt.mol_ao[m];
-
-    if (tid == 0) {
-        int o = 0;
-        for (int k = 0; k < nseg; ++k) {
-            sfirst[k] = o;
-            o += mt.seg[s0 + k].j1 - mt.seg[s0 + k].j0;
-        }
-        sfirst[nseg] = o;
-    }
     for (int k = tid; k < TBX_UNIT_PAIRS * (3 + 2 * MAXSH); k += NTHREADS) (&accP[0][0])[k] = 0.0;
     for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += NTHREADS) (&accS[0][0])[k] = 0.0;
     if (tid < 6) sig[tid] = 0.0;
-    __syncthreads();
-    const int npair = sfirst[nseg];
-
     // (1) stage the pairs; shell pairs per type, type-major
-    int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (tid < npair) {
-        int k = 0;
-        while (k + 1 < nseg && sfirst[k + 1] <= tid) ++k;
-        const TbxSeg g = mt.seg[s0 + k];
-        PairInfo pr;
-        pr.i = g.i;
-        pr.j = g.j0 + tid - sfirst[k];
-        pr.spi = mt.at_sp[pr.i];
-        pr.spj = mt.at_sp[pr.j];
-        pr.aoi = mt.at_ao[pr.i];
-        pr.aoj = mt.at_ao[pr.j];
-        pr.shi = mt.at_sh[pr.i];
-        pr.shj = mt.at_sh[pr.j];
-        pinf[tid] = pr;
-        pseg[tid] = k;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            pv[tid][c] = A.xyz[3 * pr.i + c] - A.xyz[3 * pr.j + c] - mt.trans[3 * g.itr + c];
-        int nli[3] = {0, 0, 0}, nlj[3] = {0, 0, 0};
-        for (int q = 0; q < sp.nsh[pr.spi]; ++q) ++nli[sp.ang[q + sp.msh * pr.spi]];
-        for (int q = 0; q < sp.nsh[pr.spj]; ++q) ++nlj[sp.ang[q + sp.msh * pr.spj]];
-#pragma unroll
-        for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
-    }
-    int incl[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        incl[t] = wave_scan(cnt[t]);
-        if (lane == 63) wsum[wv][t] = incl[t];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int so = 0, to = 0;
-        for (int t = 0; t < 9; ++t) {
-            int tot = 0;
-            for (int w = 0; w < NTHREADS / 64; ++w) tot += wsum[w][t];
-            tsp[t] = so;
-            toff[t] = to;
-            so += tot;
-            to += tot * ncart_of(t / 3) * ncart_of(t % 3);
-        }
-        tsp[9] = so;
-        toff[9] = to;
-    }
-    __syncthreads();
-    if (tid < npair) {
-        const int spi = pinf[tid].spi, spj = pinf[tid].spj;
-        for (int t = 0; t < 9; ++t) {
-            if (cnt[t] == 0) continue;
-            int off = tsp[t] + incl[t] - cnt[t];
-            for (int w = 0; w < wv; ++w) off += wsum[w][t];
-            for (int qi = 0; qi < sp.nsh[spi]; ++qi) {
-                if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
-                for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
-                    if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
-                    spl[off++] = make_int4(tid, qj, qi, t);
-                }
-            }
-        }
-    }
-    __syncthreads();
+    stage_unit(sp, mt, A.xyz, mt.seg, s0, nseg, U, nullptr);
+    const int npair = U.sfirst[nseg];
 
     // (2) task records: one task per lane (ntask <= TBX_UNIT_TASKS = NTHREADS)
-    const int ntask = toff[9];
+    const int ntask = U.toff[9];
     int nch = 0;
     if (tid < ntask) {
-        int t = 0;
-        while (t < 8 && tid >= toff[t + 1]) ++t;
-        const int ncj = ncart_of(t % 3);
-        const int ncp = ncart_of(t / 3) * ncj;
-        const int local = tid - toff[t];
-        const int4 e = spl[tsp[t] + local / ncp];
-        const int cp = local % ncp;
-        const int ki = cp / ncj, kj = cp - (cp / ncj) * ncj;
-        const double v[3] = {pv[e.x][0], pv[e.x][1], pv[e.x][2]};
+        int spx;
+        const int4 d = decode_task(U, tid, spx);
+        const int4 e = U.spl[spx];
+        const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
         int ppoff = 0;
-        const int kept = task_setup<MODE>(A, pinf[e.x], v, e.z, e.y, ki, kj, nao, dense0, ao_m,
+        const int kept = task_setup<MODE>(A, U.pinf[e.x], v, e.z, e.y, d.y, d.z, nao, dense0, ao_m,
                                           rec + tid, TBX_UNIT_TASKS, ppoff);
-        tinfo[tid] = t | (ki << 4) | (kj << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
+        tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
         tpp[tid] = ppoff;
         tkept[tid] = kept;
         nch = (kept + CHUNK - 1) / CHUNK;
     }
     // (3) chunk offsets: block-wide exclusive scan
-    {
-        const int inc = wave_scan(nch);
-        __syncthreads();  // wsum is reused
-        if (lane == 63) wsum[wv][0] = inc;
-        __syncthreads();
-        int base = 0;
-        for (int w = 0; w < wv; ++w) base += wsum[w][0];
-        coff[tid] = base + inc - nch;
-        if (tid == NTHREADS - 1) coff[NTHREADS] = base + inc;
-        __syncthreads();
-    }
-    const int nchunk = coff[NTHREADS];
+    const int nchunk = block_scan(nch, coff, U.wsum);
 
     // (4) work items
     for (int c = tid; c < nchunk; c += NTHREADS) {
-        int lo = 0, hi = NTHREADS;  // last task with coff <= c
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (coff[mid] <= c) lo = mid; else hi = mid;
-        }
-        const int tk = lo;
+        const int tk = find_last_le(coff, NTHREADS, c);
         const int info = tinfo[tk];
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
         const int p = (info >> 12) & 4095;
@@ -859,7 +976,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         const int p0 = tpp[tk] + q * CHUNK;
         const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
         const double *r = rec + tk;
-        const double vec[3] = {pv[p][0], pv[p][1], pv[p][2]};
+        const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
         double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
         const int L = TBX_UNIT_TASKS;
         switch (t) {
@@ -891,11 +1008,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
 
     // (5) pair sums -> atom j (HBM), segment sums (atom i) and sigma (LDS)
     if (tid < npair) {
-        const PairInfo pr = pinf[tid];
-        const int k = pseg[tid];
+        const PairInfo pr = U.pinf[tid];
+        const int k = U.pseg[tid];
         const bool self = (pr.i == pr.j);
         const double G0 = accP[tid][0], G1 = accP[tid][1], G2 = accP[tid][2];
-        const double vx = pv[tid][0], vy = pv[tid][1], vz = pv[tid][2];
+        const double vx = U.pv[tid][0], vy = U.pv[tid][1], vz = U.pv[tid][2];
         if (!self && (G0 != 0.0 || G1 != 0.0 || G2 != 0.0)) {
             unsafeAtomicAdd(&A.grad[3 * pr.j + 0], -G0);
             unsafeAtomicAdd(&A.grad[3 * pr.j + 1], -G1);
@@ -1034,19 +1151,34 @@ __global__ void k_dedcn(CnArgs A)
 // ---------------------------------------------------------------------------
 // launchers (host side of this translation unit)
 
+extern "C" hipError_t tbx_launch_fill(double *p, long long n, double v, hipStream_t stream);
+
 extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
-                                         const double *xyz, const double *se, const int *order,
-                                         double *S, double *D, double *Q, double *H,
-                                         const int *nnl, int *flag, hipStream_t stream)
+                                         const double *xyz, const double *se, double *S, double *D,
+                                         double *Q, double *H, long long ndense, const int *nnl,
+                                         int *flag, hipStream_t stream)
 {
-    FwdArgs a{*sp, *mol, xyz, se, order, S, D, Q, H, nnl, flag};
+    const int accumulate = mol->ntrans > 1;
+    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate};
     (void)hipGetLastError();  // launch errors are read below; drop stale ones
-    hipLaunchKernelGGL(k_forward, dim3(mol->nat), dim3(NTHREADS), 0, stream, a);
+    if (accumulate) {  // lattice images add into the same elements
+        double *outs[4] = {S, H, D, Q};
+        const long long n[4] = {ndense, ndense, 3 * ndense, 6 * ndense};
+        for (int k = 0; k < 4; ++k) {
+            const hipError_t e = tbx_launch_fill(outs[k], n[k], 0.0, stream);
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (mol->nfunit > 0)
+        hipLaunchKernelGGL(k_forward, dim3(mol->nfunit), dim3(NTHREADS), 0, stream, a);
+    if (nnl)
+        hipLaunchKernelGGL(k_listcheck, dim3((mol->nat + 255) / 256), dim3(256), 0, stream, *sp,
+                           *mol, xyz, nnl, flag);
     return hipGetLastError();
 }
 
 extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const TbxMols *mol,
-                                          const double *xyz, const double *se, const int *order,
+                                          const double *xyz, const double *se,
                                           const double *P, const double *W, const double *vao,
                                           const double *vdp, const double *vqp, int nspin,
                                           long long pstride, long long vstride,
@@ -1055,7 +1187,7 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                                           double *sigma, const int *nnl, int *flag,
                                           hipStream_t stream)
 {
-    GradArgs a{*sp, *mol, xyz, se, order, P, W, vao, vdp, vqp, nspin, pstride, vstride,
+    GradArgs a{*sp, *mol, xyz, se, P, W, vao, vdp, vqp, nspin, pstride, vstride,
                Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag};
     (void)hipGetLastError();
     if (mol->nunit > 0) {

@@ -80,5 +80,8 @@ struct TbxMols {
     const int *mol_ao;             // [nmol+1] first AO (global)
     int nunit;
     const int *unit_seg;           // [nunit+1] first segment of each gradient unit
-    const TbxSeg *seg;             // gradient unit segments
+    const TbxSeg *seg;             // gradient unit segments (j < i; j <= i with images)
+    int nfunit;
+    const int *funit_seg;          // [nfunit+1] first segment of each forward unit
+    const TbxSeg *fseg;            // forward unit segments (j <= i, on-site pairs included)
 };
