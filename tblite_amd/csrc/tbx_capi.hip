@@ -100,8 +100,8 @@ hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s)
 struct HostSpecies {
     int nsp = 0, msh = 0, maxnc = 0;
     std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao,
-        prim_off, pp_off;
-    std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp;
+        prim_off, pp_off, ao_info;
+    std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp, pp_mu;
     double intcut = 25.0;
 };
 
@@ -222,6 +222,14 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
             for (const auto &e : ent) hs.pp.insert(hs.pp.end(), e.begin(), e.end());
         }
     hs.pp_off[(size_t)nslot * nslot] = (int)(hs.pp.size() / TBX_PP_N);
+    hs.pp_mu.resize(hs.pp.size() / TBX_PP_N);
+    for (size_t k = 0; k < hs.pp_mu.size(); ++k) hs.pp_mu[k] = hs.pp[TBX_PP_N * k + TBX_PP_MU];
+    hs.ao_info.assign((size_t)nsp * TBX_MAXAO, 0);
+    for (size_t k = 0; k < hs.ao_info.size(); ++k) {
+        const int isp = (int)(k / TBX_MAXAO);
+        const int sh = hs.ao_sh[k];
+        hs.ao_info[k] = sh | (hs.ao_m[k] << 4) | (hs.ang[sh + msh * isp] << 8);
+    }
     return TBX_OK;
 }
 
@@ -338,7 +346,7 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
 // Device copies of the tables
 struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
-        ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off;
+        ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off, pp_mu, ao_info;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
         cart_atom, ao_atom, mol_ao, unit_seg, useg, funit_seg, fseg;
     TbxSpecies sp{};
@@ -348,7 +356,7 @@ struct DevTables {
     {
         DevBuf *all[] = {&nsh, &ang, &nprim, &alpha, &coeff, &shpoly, &rad, &hscale, &ncart, &nao,
                          &cart_sh, &cart_exp, &ao_sh, &ao_m, &sh_cart, &sh_ao, &prim, &prim_off,
-                         &pp, &pp_off,
+                         &pp, &pp_off, &pp_mu, &ao_info,
                          &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
                          &at_cart, &cart_atom, &ao_atom, &mol_ao, &unit_seg, &useg,
@@ -363,7 +371,7 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
 #define UP(name) if ((e = upload(d.name, hs.name, s)) != hipSuccess) return e
     UP(nsh); UP(ang); UP(nprim); UP(alpha); UP(coeff); UP(shpoly); UP(rad); UP(hscale); UP(ncart);
     UP(nao); UP(cart_sh); UP(cart_exp); UP(ao_sh); UP(ao_m); UP(sh_cart); UP(sh_ao); UP(prim);
-    UP(prim_off); UP(pp); UP(pp_off);
+    UP(prim_off); UP(pp); UP(pp_off); UP(pp_mu); UP(ao_info);
 #undef UP
     TbxSpecies &sp = d.sp;
     sp.nsp = hs.nsp;
@@ -390,6 +398,8 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     sp.nslot = hs.nsp * hs.msh;
     sp.pp_off = d.pp_off.as<int>();
     sp.pp = d.pp.as<double>();
+    sp.pp_mu = d.pp_mu.as<double>();
+    sp.ao_info = d.ao_info.as<int>();
     sp.intcut = hs.intcut;
     sp.cutoff2 = cutoff * cutoff;
     return hipSuccess;

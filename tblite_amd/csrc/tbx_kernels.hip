@@ -39,23 +39,70 @@ static constexpr double kEps = 2.220446049250313e-16;
 // Cartesian -> spherical: sph(l, m) = sum_k w * cart(l, c_k), CCA Cartesian
 // order; l=1 (y, z, x); l=2 m=-2..2: sqrt3 xy, sqrt3 yz, zz-(xx+yy)/2,
 // sqrt3 xz, sqrt3/2 (xx-yy).
-__constant__ int kSphN[3][5] = {{1, 0, 0, 0, 0}, {1, 1, 1, 0, 0}, {1, 1, 3, 1, 2}};
-__constant__ int kSphC[3][5][3] = {{{0}},
+static constexpr int kSphN[3][5] = {{1, 0, 0, 0, 0}, {1, 1, 1, 0, 0}, {1, 1, 3, 1, 2}};
+static constexpr int kSphC[3][5][3] = {{{0}},
                                    {{1}, {2}, {0}},
                                    {{1}, {4}, {5, 0, 3}, {2}, {0, 3}}};
-__constant__ double kSphW[3][5][3] = {
+static constexpr double kSphW[3][5][3] = {
     {{1.0}},
     {{1.0}, {1.0}, {1.0}},
     {{kS3}, {kS3}, {1.0, -0.5, -0.5}, {kS3}, {0.5 * kS3, -0.5 * kS3}}};
 // Transpose of the same map: Cartesian c -> (m, w) pairs.
-__constant__ int kCartN[3][6] = {{1, 0, 0, 0, 0, 0}, {1, 1, 1, 0, 0, 0}, {2, 1, 1, 2, 1, 1}};
-__constant__ int kCartM[3][6][2] = {{{0}},
+static constexpr int kCartN[3][6] = {{1, 0, 0, 0, 0, 0}, {1, 1, 1, 0, 0, 0}, {2, 1, 1, 2, 1, 1}};
+static constexpr int kCartM[3][6][2] = {{{0}},
                                     {{2}, {0}, {1}},
                                     {{2, 4}, {0}, {3}, {2, 4}, {1}, {2}}};
-__constant__ double kCartW[3][6][2] = {
+static constexpr double kCartW[3][6][2] = {
     {{1.0}},
     {{1.0}, {1.0}, {1.0}},
     {{-0.5, 0.5 * kS3}, {kS3}, {kS3}, {-0.5, -0.5 * kS3}, {kS3}, {1.0}}};
+
+
+// The same maps as arithmetic on packed constants (no memory access), for
+// use with lane-dependent l, m and Cartesian indices in the kernels.
+// sph(l, m) = sum_{a < sph_n} sph_w * cart(l, sph_c)
+__host__ __device__ __forceinline__ int sph_n(int l, int m)
+{
+    return (l == 2) ? (m == 2 ? 3 : (m == 4 ? 2 : 1)) : 1;
+}
+__host__ __device__ __forceinline__ int sph_c(int l, int m, int a)
+{
+    if (l == 0) return 0;
+    if (l == 1) return (0x021 >> (4 * m)) & 15;
+    return (int)((0x30002305004001ULL >> (4 * (3 * m + a))) & 15);
+}
+__host__ __device__ __forceinline__ double sph_w(int l, int m, int a)
+{
+    if (l < 2) return 1.0;
+    if (m == 2) return a == 0 ? 1.0 : -0.5;
+    if (m == 4) return a == 0 ? 0.5 * kS3 : -0.5 * kS3;
+    return kS3;
+}
+// transpose: cart(l, c) enters sph(l, cart_m) with weight cart_w
+__host__ __device__ __forceinline__ int cart_n(int l, int c)
+{
+    return (l == 2 && (c == 0 || c == 3)) ? 2 : 1;
+}
+__host__ __device__ __forceinline__ int cart_m(int l, int c, int k)
+{
+    if (l == 0) return 0;
+    if (l == 1) return (0x102 >> (4 * c)) & 15;
+    return (int)((0x20142030042ULL >> (4 * (2 * c + k))) & 15);
+}
+__host__ __device__ __forceinline__ double cart_w(int l, int c, int k)
+{
+    if (l < 2 || c == 5) return 1.0;
+    if (c == 0) return k == 0 ? -0.5 : 0.5 * kS3;
+    if (c == 3) return k == 0 ? -0.5 : -0.5 * kS3;
+    return kS3;
+}
+// exponent of direction k (x, y, z) in Cartesian function c (CCA order) of shell l
+__host__ __device__ __forceinline__ int cart_exp(int l, int c, int k)
+{
+    if (l == 0) return 0;
+    if (l == 1) return c == k ? 1 : 0;
+    return (int)((0x814211142ULL >> (6 * c + 2 * k)) & 3);
+}
 
 // ---------------------------------------------------------------------------
 // 1D Gaussian moment factors.  In the product-centre coordinate y = x - P the
@@ -341,13 +388,14 @@ __device__ __forceinline__ int wave_scan(int v)
     return v;
 }
 
-// Number of leading entries of a sorted primitive-pair list kept at r2.
-__device__ __forceinline__ int kept_pairs(const double *pp, int n, double r2, double intcut)
+// Number of primitive pairs of a slot pair kept by the screening at r2: the
+// list is sorted by mu, so the kept pairs are its leading entries.
+__device__ __forceinline__ int kept_pairs(const double *mu, int n, double r2, double intcut)
 {
     int lo = 0, hi = n;  // invariant: entries < lo kept, entries >= hi dropped
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (pp[TBX_PP_N * mid + TBX_PP_MU] * r2 <= intcut) lo = mid + 1; else hi = mid;
+        if (mu[mid] * r2 <= intcut) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
@@ -358,24 +406,30 @@ struct UnitStage {
     double pv[TBX_UNIT_PAIRS][3];  // v = R_i - R_j - T
     int pseg[TBX_UNIT_PAIRS];      // segment of each pair
     int sfirst[TBX_UNIT_SEGS + 1]; // first pair of each segment
-    int4 spl[TBX_UNIT_TASKS];      // shell pairs (pair, jsh, ish, type), type-major
+    int spl[TBX_UNIT_TASKS];       // shell pairs: pair | jsh << 8 | ish << 12 | type << 16
+                                   // (type-major)
     int wsum[NTHREADS / 64][9];
     int toff[10], tsp[10];         // task / shell-pair offsets per type
 };
 
+// (pair, jsh, ish, type) of a packed shell-pair entry
+__device__ __forceinline__ int4 unpack_spl(int v)
+{
+    return make_int4(v & 255, (v >> 8) & 15, (v >> 12) & 15, v >> 16);
+}
+
 // Stage the pairs of unit segments [s0, s0 + nseg) and lay out their shell
 // pairs by type.  spidx (optional): shell-pair index of (pair, ish, jsh).
 __device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double *xyz,
-                           const TbxSeg *seg, int s0, int nseg, UnitStage &U, int *spidx)
+                           const TbxSeg *seg, int s0, int nseg, UnitStage &U,
+                           unsigned char *spidx)
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid == 0) {
-        int o = 0;
-        for (int k = 0; k < nseg; ++k) {
-            U.sfirst[k] = o;
-            o += seg[s0 + k].j1 - seg[s0 + k].j0;
-        }
-        U.sfirst[nseg] = o;
+    if (wv == 0) {  // nseg <= TBX_UNIT_SEGS <= 64: one wavefront scans the segment lengths
+        const int len = (lane < nseg) ? seg[s0 + lane].j1 - seg[s0 + lane].j0 : 0;
+        const int inc = wave_scan(len);
+        if (lane < nseg) U.sfirst[lane] = inc - len;
+        if (lane == 63) U.sfirst[nseg] = inc;
     }
     __syncthreads();
     const int npair = U.sfirst[nseg];
@@ -435,8 +489,8 @@ __device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double
                 if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
                 for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
                     if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
-                    if (spidx) spidx[tid * MAXSH * MAXSH + qi * MAXSH + qj] = off;
-                    U.spl[off++] = make_int4(tid, qj, qi, t);
+                    if (spidx) spidx[tid * MAXSH * MAXSH + qi * MAXSH + qj] = (unsigned char)off;
+                    U.spl[off++] = tid | (qj << 8) | (qi << 12) | (t << 16);
                 }
             }
         }
@@ -518,13 +572,20 @@ template <int LI, int LJ>
 __device__ __forceinline__ void fchunk_run(const double *__restrict__ pp, const double vec[3],
                                            int ki, int kj, int p0, int p1, double o[10])
 {
-    const int ei[3] = {kCartExp[LI][ki][0], kCartExp[LI][ki][1], kCartExp[LI][ki][2]};
-    const int ej[3] = {kCartExp[LJ][kj][0], kCartExp[LJ][kj][1], kCartExp[LJ][kj][2]};
+    const int ei[3] = {cart_exp(LI, ki, 0), cart_exp(LI, ki, 1), cart_exp(LI, ki, 2)};
+    const int ej[3] = {cart_exp(LJ, kj, 0), cart_exp(LJ, kj, 1), cart_exp(LJ, kj, 2)};
     const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    // entries are loaded one iteration ahead
+    double nx[TBX_PP_N];
+#pragma unroll
+    for (int f = 0; f < TBX_PP_N; ++f) nx[f] = pp[TBX_PP_N * p0 + f];
     for (int p = p0; p < p1; ++p) {
-        const double *e = pp + TBX_PP_N * p;
-        const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
-        const double h = e[TBX_PP_H], pre = e[TBX_PP_PRE];
+        const double mu = nx[TBX_PP_MU], ajo = nx[TBX_PP_AJO], aio = nx[TBX_PP_AIO];
+        const double h = nx[TBX_PP_H], pre = nx[TBX_PP_PRE];
+        if (p + 1 < p1) {
+#pragma unroll
+            for (int f = 0; f < TBX_PP_N; ++f) nx[f] = pp[TBX_PP_N * (p + 1) + f];
+        }
         const double cc = exp(-(mu * r2)) * pre;
         const Mom M = make_mom(h);
         double V[3][3];
@@ -559,13 +620,13 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ UnitStage U;
     __shared__ double acc[10][TBX_UNIT_TASKS];
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
-    __shared__ int spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
+    __shared__ unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | pair << 12
     __shared__ int tpp[TBX_UNIT_TASKS];
     __shared__ int tkept[TBX_UNIT_TASKS];
     __shared__ int coff[TBX_UNIT_TASKS + 1];
     __shared__ int eoff[2 * TBX_UNIT_SEGS + 1];  // element offsets: A blocks, then mirrors
-    __shared__ int erow[TBX_UNIT_SEGS][3];       // first row, rows of A, rows of the mirror
+    __shared__ int erow[TBX_UNIT_SEGS][3];       // first row, rows of block A, AOs of atom i
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -585,7 +646,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     if (tid < ntask) {
         int spx;
         const int4 d = decode_task(U, tid, spx);
-        const int4 e = U.spl[spx];
+        const int4 e = unpack_spl(U.spl[spx]);
         const PairInfo pr = U.pinf[e.x];
         const double vx = U.pv[e.x][0], vy = U.pv[e.x][1], vz = U.pv[e.x][2];
         const double r2 = vx * vx + vy * vy + vz * vz;
@@ -595,7 +656,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
         const bool onsite = (pr.i == pr.j) && r2 < kEps;
         const bool inrange = onsite || (r2 >= kEps && r2 <= sp.cutoff2);
-        const int kept = inrange ? kept_pairs(sp.pp + (long long)TBX_PP_N * pi0, npp,
+        const int kept = inrange ? kept_pairs(sp.pp_mu + pi0, npp,
                                               onsite ? 0.0 : r2, sp.intcut)
                                  : 0;
         if (d.w == 0) {
@@ -647,107 +708,117 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 
     // (5) element layout per segment: block A (rows of atoms j0..j1-1, the
     // columns of atom i) and the mirror (rows of atom i, columns of the j != i)
-    if (tid == 0) {
-        int o = 0;
-        for (int k = 0; k < nseg; ++k) {
-            const TbxSeg g = mt.fseg[s0 + k];
-            const PairInfo &first = U.pinf[U.sfirst[k]];
-            const PairInfo &last = U.pinf[U.sfirst[k + 1] - 1];
+    if (tid < 64) {  // one wavefront; nseg <= TBX_UNIT_SEGS <= 64
+        int na = 0, nb = 0;
+        if (tid < nseg) {
+            const TbxSeg g = mt.fseg[s0 + tid];
+            const PairInfo &first = U.pinf[U.sfirst[tid]];
+            const PairInfo &last = U.pinf[U.sfirst[tid + 1] - 1];
             const int r0 = first.aoj;
             const int r1 = last.aoj + sp.nao[last.spj];
             const int jb = min(g.j1, g.i);  // mirrored atoms: j < i
-            const int rb1 = (jb > g.j0) ? U.pinf[U.sfirst[k] + (jb - 1 - g.j0)].aoj +
-                                              sp.nao[U.pinf[U.sfirst[k] + (jb - 1 - g.j0)].spj]
-                                        : r0;
-            erow[k][0] = r0;
-            erow[k][1] = r1 - r0;
-            erow[k][2] = rb1 - r0;
-            eoff[k] = o;
-            o += sp.nao[first.spi] * (r1 - r0);
+            int rb1 = r0;
+            if (jb > g.j0) {
+                const PairInfo &lb = U.pinf[U.sfirst[tid] + (jb - 1 - g.j0)];
+                rb1 = lb.aoj + sp.nao[lb.spj];
+            }
+            const int naoi = sp.nao[first.spi];
+            erow[tid][0] = r0;
+            erow[tid][1] = r1 - r0;
+            erow[tid][2] = naoi;
+            na = naoi * (r1 - r0);
+            nb = naoi * (rb1 - r0);
         }
-        for (int k = 0; k < nseg; ++k) {
-            eoff[nseg + k] = o;
-            o += sp.nao[U.pinf[U.sfirst[k]].spi] * erow[k][2];
+        const int ia = wave_scan(na), ib = wave_scan(nb);
+        const int ta = __shfl(ia, 63, 64);
+        if (tid < nseg) {
+            eoff[tid] = ia - na;
+            eoff[nseg + tid] = ta + ib - nb;
         }
-        eoff[2 * nseg] = o;
+        if (tid == 63) eoff[2 * nseg] = ta + ib;
     }
     __syncthreads();
     const int nel = eoff[2 * nseg];
-    for (int e = tid; e < nel; e += NTHREADS) {
-        const int kk = find_last_le(eoff, 2 * nseg, e);
-        const bool mirror = kk >= nseg;
-        const int k = mirror ? kk - nseg : kk;
-        const int local = e - eoff[kk];
-        const int spi = U.pinf[U.sfirst[k]].spi;
-        const int naoi = sp.nao[spi];
-        int mu, nul;
-        if (!mirror) {
-            nul = local / erow[k][1];
-            mu = erow[k][0] + local - nul * erow[k][1];
-        } else {
-            mu = erow[k][0] + local / naoi;
-            nul = local - (local / naoi) * naoi;
-        }
-        const int jj = mt.ao_atom[ao_m + mu];
-        const int p = U.sfirst[k] + (jj - mt.fseg[s0 + k].j0);
-        const PairInfo pr = U.pinf[p];
-        const int mul = mu - pr.aoj;
-        const int ish = sp.ao_sh[spi * TBX_MAXAO + nul], mi = sp.ao_m[spi * TBX_MAXAO + nul];
-        const int jsh = sp.ao_sh[pr.spj * TBX_MAXAO + mul], mj = sp.ao_m[pr.spj * TBX_MAXAO + mul];
-        const int li = sp.ang[ish + sp.msh * spi], lj = sp.ang[jsh + sp.msh * pr.spj];
-        const int spx = spidx[p * MAXSH * MAXSH + ish * MAXSH + jsh];
-        const int t = li * 3 + lj;
-        const int ncj = ncart_of(lj);
-        const int task0 = U.toff[t] + (spx - U.tsp[t]) * ncart_of(li) * ncj;
-        double v[10];
-#pragma unroll
-        for (int q = 0; q < 10; ++q) v[q] = 0.0;
-        for (int a = 0; a < kSphN[lj][mj]; ++a) {
-            const double wa = kSphW[lj][mj][a];
-            const int cj = kSphC[lj][mj][a];
-            for (int b = 0; b < kSphN[li][mi]; ++b) {
-                const double w = wa * kSphW[li][mi][b];
-                const int tk = task0 + kSphC[li][mi][b] * ncj + cj;
-#pragma unroll
-                for (int q = 0; q < 10; ++q) v[q] += w * acc[q][tk];
+    for (int base = 0; base < nel; base += NTHREADS) {
+        const int e = base + tid;
+        if (e < nel) {
+            const int kk = find_last_le(eoff, 2 * nseg, e);
+            const bool mirror = kk >= nseg;
+            const int k = mirror ? kk - nseg : kk;
+            const int local = e - eoff[kk];
+            const int naoi = erow[k][2];
+            int mu, nul;
+            if (!mirror) {
+                nul = local / erow[k][1];
+                mu = erow[k][0] + local - nul * erow[k][1];
+            } else {
+                mu = erow[k][0] + local / naoi;
+                nul = local - (local / naoi) * naoi;
             }
+            const int jj = mt.ao_atom[ao_m + mu];
+            const int p = U.sfirst[k] + (jj - mt.fseg[s0 + k].j0);
+            const PairInfo pr = U.pinf[p];
+            const int infi = sp.ao_info[pr.spi * TBX_MAXAO + nul];
+            const int infj = sp.ao_info[pr.spj * TBX_MAXAO + mu - pr.aoj];
+            const int ish = infi & 15, mi = (infi >> 4) & 15, li = infi >> 8;
+            const int jsh = infj & 15, mj = (infj >> 4) & 15, lj = infj >> 8;
+            const int spx = spidx[p * MAXSH * MAXSH + ish * MAXSH + jsh];
+            const int t = li * 3 + lj;
+            const int ncj = ncart_of(lj);
+            const int task0 = U.toff[t] + (spx - U.tsp[t]) * ncart_of(li) * ncj;
+            double v[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) v[q] = 0.0;
+            const int nsj = sph_n(lj, mj), nsi = sph_n(li, mi);
+            for (int a = 0; a < nsj; ++a) {
+                const double wa = sph_w(lj, mj, a);
+                const int cj = sph_c(lj, mj, a);
+                for (int b = 0; b < nsi; ++b) {
+                    const double w = wa * sph_w(li, mi, b);
+                    const int tk = task0 + sph_c(li, mi, b) * ncj + cj;
+#pragma unroll
+                    for (int q = 0; q < 10; ++q) v[q] += w * acc[q][tk];
+                }
+            }
+            const double sv = v[0];
+            const double tr = 0.5 * (v[4] + v[6] + v[9]);
+            double d[3] = {v[1], v[2], v[3]};
+            double qv[6] = {1.5 * v[4] - tr, 1.5 * v[5], 1.5 * v[6] - tr,
+                            1.5 * v[7], 1.5 * v[8], 1.5 * v[9] - tr};
+            const int nu = pr.aoi + nul;
+            long long idx;
+            if (!mirror) {
+                idx = dense0 + (long long)nu * nao + mu;
+            } else {
+                // operator on j: shift the moments from i to j (x - R_j = (x - R_i) + v)
+                idx = dense0 + (long long)mu * nao + nu;
+                const double vv[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
+                double sq[6];
+                sq[0] = 2 * vv[0] * d[0] + vv[0] * vv[0] * sv;
+                sq[2] = 2 * vv[1] * d[1] + vv[1] * vv[1] * sv;
+                sq[5] = 2 * vv[2] * d[2] + vv[2] * vv[2] * sv;
+                sq[1] = vv[0] * d[1] + vv[1] * d[0] + vv[0] * vv[1] * sv;
+                sq[3] = vv[0] * d[2] + vv[2] * d[0] + vv[0] * vv[2] * sv;
+                sq[4] = vv[1] * d[2] + vv[2] * d[1] + vv[1] * vv[2] * sv;
+                const double ts = 0.5 * (sq[0] + sq[2] + sq[5]);
+                qv[0] += 1.5 * sq[0] - ts;
+                qv[1] += 1.5 * sq[1];
+                qv[2] += 1.5 * sq[2] - ts;
+                qv[3] += 1.5 * sq[3];
+                qv[4] += 1.5 * sq[4];
+                qv[5] += 1.5 * sq[5] - ts;
+                for (int c = 0; c < 3; ++c) d[c] += vv[c] * sv;
+            }
+            put(A.S + idx, sv, accum);
+            put(A.H + idx, hfac[spx] * sv, accum);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) put(A.D + 3 * idx + c, d[c], accum);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) put(A.Q + 6 * idx + c, qv[c], accum);
         }
-        const double s = v[0];
-        const double h = hfac[spx] * s;
-        const double tr = 0.5 * (v[4] + v[6] + v[9]);
-        double d[3] = {v[1], v[2], v[3]};
-        double qv[6] = {1.5 * v[4] - tr, 1.5 * v[5], 1.5 * v[6] - tr,
-                        1.5 * v[7], 1.5 * v[8], 1.5 * v[9] - tr};
-        long long idx;
-        const int nu = pr.aoi + nul;
-        if (!mirror) {
-            idx = dense0 + (long long)nu * nao + mu;
-        } else {
-            // operator on j: shift the moments from i to j (x - R_j = (x - R_i) + v)
-            idx = dense0 + (long long)mu * nao + nu;
-            const double vv[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
-            double sq[6];
-            sq[0] = 2 * vv[0] * d[0] + vv[0] * vv[0] * s;
-            sq[2] = 2 * vv[1] * d[1] + vv[1] * vv[1] * s;
-            sq[5] = 2 * vv[2] * d[2] + vv[2] * vv[2] * s;
-            sq[1] = vv[0] * d[1] + vv[1] * d[0] + vv[0] * vv[1] * s;
-            sq[3] = vv[0] * d[2] + vv[2] * d[0] + vv[0] * vv[2] * s;
-            sq[4] = vv[1] * d[2] + vv[2] * d[1] + vv[1] * vv[2] * s;
-            const double ts = 0.5 * (sq[0] + sq[2] + sq[5]);
-            qv[0] += 1.5 * sq[0] - ts;
-            qv[1] += 1.5 * sq[1];
-            qv[2] += 1.5 * sq[2] - ts;
-            qv[3] += 1.5 * sq[3];
-            qv[4] += 1.5 * sq[4];
-            qv[5] += 1.5 * sq[5] - ts;
-            for (int c = 0; c < 3; ++c) d[c] += vv[c] * s;
-        }
-        put(A.S + idx, s, accum);
-        put(A.H + idx, h, accum);
-        for (int c = 0; c < 3; ++c) put(A.D + 3 * idx + c, d[c], accum);
-        for (int c = 0; c < 6; ++c) put(A.Q + 6 * idx + c, qv[c], accum);
     }
 }
+
 
 // Task set-up: one Cartesian function pair (ki of shell ish on atom i, kj of
 // shell jsh on atom j) at lattice vector itr.  Folds the spherical weights of
@@ -768,23 +839,23 @@ __device__ int task_setup(const GradArgs &A, const PairInfo &pr, const double v[
     if (r2 < kEps || r2 > sp.cutoff2) return 0;
     const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
     const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
-    const int kept = kept_pairs(sp.pp + (long long)TBX_PP_N * pi0, npp, r2, sp.intcut);
+    const int kept = kept_pairs(sp.pp_mu + pi0, npp, r2, sp.intcut);
     ppoff = pi0;
     if (kept == 0) return 0;
 
     const int muj0 = pr.aoj + sp.sh_ao[slotj];
     const int nui0 = pr.aoi + sp.sh_ao[sloti];
-    const int na = kCartN[lj][kj], nb = kCartN[li][ki];
+    const int na = cart_n(lj, kj), nb = cart_n(li, ki);
     double wS = 0.0, wH = 0.0;
     double wDi[3] = {0, 0, 0}, wDj[3] = {0, 0, 0};
     double wQi[6] = {0, 0, 0, 0, 0, 0}, wQj[6] = {0, 0, 0, 0, 0, 0};
     if (MODE == GRAD_PLAIN) {
         double pt = 0.0;
         for (int a = 0; a < na; ++a) {
-            const int mu = muj0 + kCartM[lj][kj][a];
+            const int mu = muj0 + cart_m(lj, kj, a);
             for (int b = 0; b < nb; ++b) {
-                const int nu = nui0 + kCartM[li][ki][b];
-                const double w = kCartW[lj][kj][a] * kCartW[li][ki][b];
+                const int nu = nui0 + cart_m(li, ki, b);
+                const double w = cart_w(lj, kj, a) * cart_w(li, ki, b);
                 const long long e1 = dense0 + (long long)nu * nao + mu;  // (mu, nu)
                 double vs = 0.0;
                 for (int sgn = 0; sgn < A.nspin; ++sgn)
@@ -808,10 +879,10 @@ __device__ int task_setup(const GradArgs &A, const PairInfo &pr, const double v[
         }
     } else {
         for (int a = 0; a < na; ++a) {
-            const int mu = muj0 + kCartM[lj][kj][a];
+            const int mu = muj0 + cart_m(lj, kj, a);
             for (int b = 0; b < nb; ++b) {
-                const int nu = nui0 + kCartM[li][ki][b];
-                const double w = kCartW[lj][kj][a] * kCartW[li][ki][b];
+                const int nu = nui0 + cart_m(li, ki, b);
+                const double w = cart_w(lj, kj, a) * cart_w(li, ki, b);
                 const long long e1 = dense0 + (long long)nu * nao + mu;  // (mu, nu)
                 const long long e2 = dense0 + (long long)mu * nao + nu;  // (nu, mu)
                 wS += w * (A.Sb[e1] + A.Sb[e2]);
@@ -883,16 +954,23 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
                                           const double vec[3], int ki, int kj, int p0, int p1,
                                           double a3[3], double &s, double dacc[3])
 {
-    const int ei[3] = {kCartExp[LI][ki][0], kCartExp[LI][ki][1], kCartExp[LI][ki][2]};
-    const int ej[3] = {kCartExp[LJ][kj][0], kCartExp[LJ][kj][1], kCartExp[LJ][kj][2]};
+    const int ei[3] = {cart_exp(LI, ki, 0), cart_exp(LI, ki, 1), cart_exp(LI, ki, 2)};
+    const int ej[3] = {cart_exp(LJ, kj, 0), cart_exp(LJ, kj, 1), cart_exp(LJ, kj, 2)};
     double w[10];
 #pragma unroll
     for (int k = 0; k < 10; ++k) w[k] = rec[(R_W + k) * ld];
     const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    // entries are loaded one iteration ahead
+    double nx[TBX_PP_N];
+#pragma unroll
+    for (int f = 0; f < TBX_PP_N; ++f) nx[f] = pp[TBX_PP_N * p0 + f];
     for (int p = p0; p < p1; ++p) {
-        const double *e = pp + TBX_PP_N * p;
-        const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
-        const double h = e[TBX_PP_H], ai = e[TBX_PP_AI], pre = e[TBX_PP_PRE];
+        const double mu = nx[TBX_PP_MU], ajo = nx[TBX_PP_AJO], aio = nx[TBX_PP_AIO];
+        const double h = nx[TBX_PP_H], ai = nx[TBX_PP_AI], pre = nx[TBX_PP_PRE];
+        if (p + 1 < p1) {
+#pragma unroll
+            for (int f = 0; f < TBX_PP_N; ++f) nx[f] = pp[TBX_PP_N * (p + 1) + f];
+        }
         const double cc = exp(-(mu * r2)) * pre;
         const Mom M = make_mom(h);
         double V[3][3], G[3][3];
@@ -952,7 +1030,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     if (tid < ntask) {
         int spx;
         const int4 d = decode_task(U, tid, spx);
-        const int4 e = U.spl[spx];
+        const int4 e = unpack_spl(U.spl[spx]);
         const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
         int ppoff = 0;
         const int kept = task_setup<MODE>(A, U.pinf[e.x], v, e.z, e.y, d.y, d.z, nao, dense0, ao_m,
@@ -1076,11 +1154,11 @@ __global__ void __launch_bounds__(NTHREADS) k_onsite(GradArgs A)
         const int muj0 = mt.at_ao[i] + sp.sh_ao[slotj];
         const int nui0 = mt.at_ao[i] + sp.sh_ao[sloti];
         double wHon = 0.0;
-        for (int a = 0; a < kCartN[lj][kj]; ++a)
-            for (int b = 0; b < kCartN[li][ki]; ++b)
-                wHon += kCartW[lj][kj][a] * kCartW[li][ki][b] *
-                        A.Hb[dense0 + (long long)(nui0 + kCartM[li][ki][b]) * nao + muj0 +
-                             kCartM[lj][kj][a]];
+        for (int a = 0; a < cart_n(lj, kj); ++a)
+            for (int b = 0; b < cart_n(li, ki); ++b)
+                wHon += cart_w(lj, kj, a) * cart_w(li, ki, b) *
+                        A.Hb[dense0 + (long long)(nui0 + cart_m(li, ki, b)) * nao + muj0 +
+                             cart_m(lj, kj, a)];
         if (wHon == 0.0) continue;
         const int e0 = (ei_ & 15) + (ej_ & 15), e1 = ((ei_ >> 4) & 15) + ((ej_ >> 4) & 15),
                   e2 = ((ei_ >> 8) & 15) + ((ej_ >> 8) & 15);
@@ -1217,6 +1295,27 @@ extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
     const int nb = (mol->nat + 255) / 256;
     hipLaunchKernelGGL(k_dedcn, dim3(nb), dim3(256), 0, stream, a);
     return hipGetLastError();
+}
+
+// Host self-test: the packed maps reproduce the tables (returns the number
+// of mismatches; 0 on success).
+extern "C" int tbx_selftest_maps(void)
+{
+    int bad = 0;
+    for (int l = 0; l <= 2; ++l) {
+        for (int m = 0; m < 2 * l + 1; ++m) {
+            bad += sph_n(l, m) != kSphN[l][m];
+            for (int a = 0; a < kSphN[l][m]; ++a)
+                bad += (sph_c(l, m, a) != kSphC[l][m][a]) + (sph_w(l, m, a) != kSphW[l][m][a]);
+        }
+        for (int c = 0; c < (l + 1) * (l + 2) / 2; ++c) {
+            bad += cart_n(l, c) != kCartN[l][c];
+            for (int k = 0; k < kCartN[l][c]; ++k)
+                bad += (cart_m(l, c, k) != kCartM[l][c][k]) + (cart_w(l, c, k) != kCartW[l][c][k]);
+            for (int k = 0; k < 3; ++k) bad += cart_exp(l, c, k) != kCartExp[l][c][k];
+        }
+    }
+    return bad;
 }
 
 // Fill kernel (used instead of hipMemsetAsync so that callers may capture the

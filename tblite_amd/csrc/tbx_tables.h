@@ -45,6 +45,8 @@ struct TbxSpecies {
     int nslot;            // nsp*msh
     const int *pp_off;    // [nslot*nslot+1]
     const double *pp;     // [TBX_PP_N * entries]
+    const double *pp_mu;  // [entries] reduced exponents alone (screening counts)
+    const int *ao_info;   // [nsp*TBX_MAXAO] shell | m << 4 | l << 8 of each spherical function
     double intcut;
     double cutoff2;
 };
