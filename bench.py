@@ -37,6 +37,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel: str, nmol: int):
+    """HBM bytes per launch of ``kernel`` on this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel); None
+    when no profile of this workload exists."""
+    path = ROOT / "profiles" / f"batch{nmol}_hbm_traffic.md"
+    if not path.exists():
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def _cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -250,12 +264,14 @@ def main():
     adj_flop = roofline.FLOP_BWD_PER_KC * kc
     if adj_a >= fwd_a:
         ach = adj_flop / (adj_a * 1e-3) / 1e12
-        roof = roofline.roofline_entry("mfma", ach, roofline.PEAK_FP64_TFLOPS, "TFLOP/s")
+        roof = roofline.roofline_entry("mfma", ach, roofline.PEAK_FP64_TFLOPS, "TFLOP/s",
+                                       profiled_traffic("k_gradient<adjoint>", b.nmol))
         roof["kernel"] = "k_gradient<adjoint> (FP64 vector)"
         roof["kernel_ms"] = adj_a
     else:
         ach = fwd_bytes / (fwd_a * 1e-3) / 1e9
-        roof = roofline.roofline_entry("hbm", ach, roofline.PEAK_HBM_GBS, "GB/s")
+        roof = roofline.roofline_entry("hbm", ach, roofline.PEAK_HBM_GBS, "GB/s",
+                                       profiled_traffic("k_forward", b.nmol))
         roof["kernel"] = "k_forward"
         roof["kernel_ms"] = fwd_a
     line = {

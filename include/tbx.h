@@ -176,6 +176,12 @@ int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
 int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pairs,
                           double *ms_gradient, double *ms_adjoint);
 
+/* Measurement helper, no counterpart in the reference: copies n doubles
+ * d_src -> d_dst with a plain streaming kernel (k_copy) on `stream`.  Its
+ * hardware-counter readings (FETCH_SIZE, WRITE_SIZE) for a known byte count
+ * calibrate the HBM traffic reported for the integral kernels. */
+int tbx_probe_copy(const double *d_src, double *d_dst, long long n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

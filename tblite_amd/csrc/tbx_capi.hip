@@ -36,6 +36,8 @@ extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
                                        const double *dse, const double *dsedcn, const double *P,
                                        double *dEdcn, hipStream_t stream);
 extern "C" hipError_t tbx_launch_fill(double *p, long long n, double v, hipStream_t stream);
+extern "C" hipError_t tbx_launch_copy(const double *src, double *dst, long long n,
+                                      hipStream_t stream);
 
 namespace {
 
@@ -830,5 +832,12 @@ extern "C" int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, dou
     if (ms_gradient) *ms_gradient = el(4, 5);
     if (ms_adjoint) *ms_adjoint = el(6, 7);
     (void)hipGetLastError();  // a query of a never-recorded event is not an error here
+    return TBX_OK;
+}
+
+extern "C" int tbx_probe_copy(const double *d_src, double *d_dst, long long n, void *stream)
+{
+    if (!d_src || !d_dst || n < 0) return fail(TBX_ERR_ARG, "tbx_probe_copy: bad arguments");
+    HIP_TRY(tbx_launch_copy(d_src, d_dst, n, (hipStream_t)stream));
     return TBX_OK;
 }

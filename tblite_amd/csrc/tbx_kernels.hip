@@ -1318,6 +1318,23 @@ extern "C" int tbx_selftest_maps(void)
     return bad;
 }
 
+// Streaming copy (traffic calibration of the hardware counters)
+__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        dst[k] = src[k];
+}
+
+extern "C" hipError_t tbx_launch_copy(const double *src, double *dst, long long n,
+                                      hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
 // Fill kernel (used instead of hipMemsetAsync so that callers may capture the
 // launches in a HIP graph).
 __global__ void k_fill(double *p, long long n, double v)

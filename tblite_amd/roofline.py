@@ -84,3 +84,48 @@ def count_kc(spec, ids_per_mol, xyz_per_mol, cutoff, intcut=25.0):
 def roofline_entry(bound, achieved, peak, unit, traffic=None):
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": traffic}
+
+
+def copy_probe(gib: float = 1.0, reps: int = 3, device: int = 0):
+    """Run the library's streaming copy kernel (k_copy) ``reps`` times over
+    ``gib`` GiB and return (bytes read per launch, bytes written per launch,
+    measured GB/s).  Profiled with FETCH_SIZE / WRITE_SIZE, its per-dispatch
+    counter readings divided by these byte counts give the factors that turn
+    the integral kernels' readings into HBM bytes."""
+    import ctypes as ct
+
+    import torch
+
+    from . import lib as L
+
+    n = int(gib * (1 << 30)) // 8
+    dev = torch.device("cuda", device)
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream(dev)
+    h = ct.c_void_p(stream.cuda_stream)
+    L.check(L.lib().tbx_probe_copy(ct.c_void_p(src.data_ptr()), ct.c_void_p(dst.data_ptr()), n, h))
+    torch.cuda.synchronize(dev)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(reps):
+        L.check(L.lib().tbx_probe_copy(ct.c_void_p(src.data_ptr()), ct.c_void_p(dst.data_ptr()),
+                                       n, h))
+    t1.record(stream)
+    torch.cuda.synchronize(dev)
+    ms = t0.elapsed_time(t1) / reps
+    assert bool((dst == 1).all())
+    return 8 * n, 8 * n, 16 * n / (ms * 1e6)
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="HBM traffic calibration probe")
+    ap.add_argument("--gib", type=float, default=1.0)
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args()
+    rd, wr, gbs = copy_probe(a.gib, a.reps)
+    print(json.dumps({"probe": "k_copy", "bytes_read": rd, "bytes_written": wr,
+                      "launches": a.reps + 1, "GBs": gbs}))

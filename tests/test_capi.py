@@ -81,3 +81,13 @@ def test_batch_create_rejects_l3_shell_without_gpu():
     assert code == 2
     assert b"l=3" in L.lib().tbx_last_error()
     assert not out.value
+
+
+def test_packed_maps_match_tables():
+    """The kernels' register-only spherical/Cartesian maps reproduce the
+    transform tables (host self-test exported by the library)."""
+    assert L.lib() is not None
+    import ctypes as ct
+    f = ct.CDLL(str(L.LIB_PATH)).tbx_selftest_maps
+    f.restype = ct.c_int
+    assert f() == 0
