@@ -1,0 +1,275 @@
+!> Fortran binding of the MI355X GFN2-xTB integral / H0 path (include/tbx.h)
+!> through iso_c_binding.
+!>
+!> The procedures keep the argument meaning of tblite's module procedures
+!>   get_selfenergy            src/tblite/xtb/h0.f90:113-217
+!>   get_hamiltonian           src/tblite/xtb/h0.f90:220-429
+!>   get_hamiltonian_gradient  src/tblite/xtb/h0.f90:432-666
+!> with the derived types (structure_type, basis_type, tb_hamiltonian,
+!> adjacency_list, potential_type) flattened to arrays.  Atom, species and
+!> translation indices passed in are 1-based as in tblite and are converted to
+!> the 0-based C side here; the shell/AO offsets ish_at and iao_sh are tblite's
+!> 0-based offsets and pass through unchanged.  Dense outputs keep tblite's
+!> column-major layout: overlap(nao,nao), dpint(3,nao,nao), qpint(6,nao,nao),
+!> hamiltonian(nao,nao).  dEdcn, gradient and sigma are accumulated into.
+module tbx_h0
+   use, intrinsic :: iso_c_binding, only : c_int, c_double, c_ptr, c_null_ptr, c_loc, &
+      & c_associated, c_char, c_f_pointer
+   implicit none
+   private
+
+   public :: tbx_h0_tables, tbx_h0_context, tbx_h0_release, tbx_h0_last_error
+   public :: tbx_h0_get_selfenergy, tbx_h0_get_hamiltonian, tbx_h0_get_hamiltonian_gradient
+
+   integer, parameter, public :: tbx_maxg = 12, tbx_maxsh = 4
+
+   !> Flattened basis and H0 parameters of one parametrisation
+   type :: tbx_h0_tables
+      integer :: nid = 0, msh = 0
+      integer(c_int), allocatable :: nsh_id(:)          !< (nid)
+      integer(c_int), allocatable :: ang(:, :)          !< (msh, nid)
+      integer(c_int), allocatable :: nprim(:, :)        !< (msh, nid)
+      real(c_double), allocatable :: alpha(:, :, :)     !< (maxg, msh, nid)
+      real(c_double), allocatable :: coeff(:, :, :)     !< (maxg, msh, nid), normalised
+      real(c_double), allocatable :: hscale(:, :, :, :) !< (msh, msh, nid, nid)
+      real(c_double), allocatable :: shpoly(:, :)       !< (msh, nid)
+      real(c_double), allocatable :: rad(:)             !< (nid), bohr
+      real(c_double) :: intcut = 25.0_c_double
+   end type tbx_h0_tables
+
+   type, bind(C) :: basis_c
+      integer(c_int) :: nid, msh
+      type(c_ptr) :: nsh_id, ang, nprim, alpha, coeff
+      real(c_double) :: intcut
+   end type basis_c
+
+   type, bind(C) :: h0par_c
+      type(c_ptr) :: hscale, shpoly, rad
+   end type h0par_c
+
+   type, bind(C) :: geom_c
+      integer(c_int) :: nat
+      type(c_ptr) :: id, xyz
+      integer(c_int) :: ntrans
+      type(c_ptr) :: trans
+      real(c_double) :: cutoff
+   end type geom_c
+
+   type, bind(C) :: list_c
+      type(c_ptr) :: inl, nnl
+      integer(c_int) :: nnbr
+      type(c_ptr) :: nlat, nltr
+   end type list_c
+
+   interface
+      function c_context_new(device) result(ctx) bind(C, name="tbx_context_new")
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr) :: ctx
+      end function c_context_new
+
+      subroutine c_context_delete(ctx) bind(C, name="tbx_context_delete")
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine c_context_delete
+
+      function c_last_error() result(msg) bind(C, name="tbx_last_error")
+         import :: c_ptr
+         type(c_ptr) :: msg
+      end function c_last_error
+
+      function c_get_selfenergy(nat, id, ish_at, nsh_id, msh, se_sp, kcn, cn, se, dsedcn) &
+            & result(stat) bind(C, name="tbx_get_selfenergy")
+         import :: c_int, c_ptr
+         integer(c_int), value :: nat, msh
+         type(c_ptr), value :: id, ish_at, nsh_id, se_sp, kcn, cn, se, dsedcn
+         integer(c_int) :: stat
+      end function c_get_selfenergy
+
+      function c_get_hamiltonian(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, se, &
+            & overlap, dpint, qpint, hamiltonian) result(stat) bind(C, name="tbx_get_hamiltonian")
+         import :: c_int, c_ptr, geom_c, basis_c, h0par_c
+         type(c_ptr), value :: ctx
+         type(geom_c), intent(in) :: geom
+         type(c_ptr), value :: list
+         type(basis_c), intent(in) :: bas
+         type(h0par_c), intent(in) :: h0
+         type(c_ptr), value :: ish_at, iao_sh
+         integer(c_int), value :: nao
+         type(c_ptr), value :: se, overlap, dpint, qpint, hamiltonian
+         integer(c_int) :: stat
+      end function c_get_hamiltonian
+
+      function c_get_hamiltonian_gradient(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, se, &
+            & dsedcn, nspin, vao, vdp, vqp, pmat, xmat, dEdcn, gradient, sigma) result(stat) &
+            & bind(C, name="tbx_get_hamiltonian_gradient")
+         import :: c_int, c_ptr, geom_c, basis_c, h0par_c
+         type(c_ptr), value :: ctx
+         type(geom_c), intent(in) :: geom
+         type(c_ptr), value :: list
+         type(basis_c), intent(in) :: bas
+         type(h0par_c), intent(in) :: h0
+         type(c_ptr), value :: ish_at, iao_sh
+         integer(c_int), value :: nao
+         type(c_ptr), value :: se, dsedcn
+         integer(c_int), value :: nspin
+         type(c_ptr), value :: vao, vdp, vqp, pmat, xmat, dEdcn, gradient, sigma
+         integer(c_int) :: stat
+      end function c_get_hamiltonian_gradient
+
+      function c_strlen(s) result(n) bind(C, name="strlen")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: s
+         integer(c_int) :: n
+      end function c_strlen
+   end interface
+
+contains
+
+   !> Context on HIP device `device` (0-based); c_null_ptr on failure
+   function tbx_h0_context(device) result(ctx)
+      integer, intent(in) :: device
+      type(c_ptr) :: ctx
+      ctx = c_context_new(int(device, c_int))
+   end function tbx_h0_context
+
+   subroutine tbx_h0_release(ctx)
+      type(c_ptr), intent(inout) :: ctx
+      if (c_associated(ctx)) call c_context_delete(ctx)
+      ctx = c_null_ptr
+   end subroutine tbx_h0_release
+
+   !> Message of the last failing call ("" if none)
+   function tbx_h0_last_error() result(msg)
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: s(:)
+      integer :: n, k
+      p = c_last_error()
+      n = 0
+      if (c_associated(p)) n = c_strlen(p)
+      allocate(character(len=n) :: msg)
+      if (n == 0) return
+      call c_f_pointer(p, s, [n])
+      do k = 1, n
+         msg(k:k) = s(k)
+      end do
+   end function tbx_h0_last_error
+
+   !> get_selfenergy, CN branch (GFN2): se = selfenergy_sp - kcn*cn, dsedcn = -kcn
+   subroutine tbx_h0_get_selfenergy(tab, id, ish_at, selfenergy_sp, kcn, cn, selfenergy, &
+         & dsedcn, stat)
+      type(tbx_h0_tables), intent(in), target :: tab
+      integer, intent(in) :: id(:)
+      integer(c_int), intent(in), target, contiguous :: ish_at(:)
+      real(c_double), intent(in), target, contiguous :: selfenergy_sp(:, :), kcn(:, :), cn(:)
+      real(c_double), intent(out), target, contiguous :: selfenergy(:), dsedcn(:)
+      integer, intent(out) :: stat
+      integer(c_int), allocatable, target :: id0(:)
+      id0 = int(id - 1, c_int)
+      stat = c_get_selfenergy(int(size(id), c_int), c_loc(id0), c_loc(ish_at), &
+         & c_loc(tab%nsh_id), int(tab%msh, c_int), c_loc(selfenergy_sp), c_loc(kcn), c_loc(cn), &
+         & c_loc(selfenergy), c_loc(dsedcn))
+   end subroutine tbx_h0_get_selfenergy
+
+   !> get_hamiltonian: dense overlap, dipole, quadrupole and H0 (intent(out)).
+   !> The half list (tblite's adjacency_list: inl offsets, nnl counts, nlat
+   !> 1-based neighbour atoms, nltr 1-based translations) is optional; without
+   !> it the neighbours are all pairs within cutoff.
+   subroutine tbx_h0_get_hamiltonian(ctx, tab, id, xyz, trans, cutoff, ish_at, iao_sh, nao, &
+         & selfenergy, overlap, dpint, qpint, hamiltonian, stat, inl, nnl, nlat, nltr)
+      type(c_ptr), intent(in) :: ctx
+      type(tbx_h0_tables), intent(in), target :: tab
+      integer, intent(in) :: id(:)
+      real(c_double), intent(in), target, contiguous :: xyz(:, :), trans(:, :)
+      real(c_double), intent(in) :: cutoff
+      integer(c_int), intent(in), target, contiguous :: ish_at(:), iao_sh(:)
+      integer, intent(in) :: nao
+      real(c_double), intent(in), target, contiguous :: selfenergy(:)
+      real(c_double), intent(out), target, contiguous :: overlap(:, :), dpint(:, :, :), &
+         & qpint(:, :, :), hamiltonian(:, :)
+      integer, intent(out) :: stat
+      integer, intent(in), optional :: inl(:), nnl(:), nlat(:), nltr(:)
+      type(geom_c) :: geom
+      type(basis_c) :: bas
+      type(h0par_c) :: h0
+      type(list_c), target :: list
+      integer(c_int), allocatable, target :: id0(:), inl0(:), nnl0(:), nlat0(:), nltr0(:)
+      type(c_ptr) :: plist
+
+      id0 = int(id - 1, c_int)
+      call pack_tables(tab, bas, h0)
+      geom = geom_c(int(size(id), c_int), c_loc(id0), c_loc(xyz), int(size(trans, 2), c_int), &
+         & c_loc(trans), cutoff)
+      plist = c_null_ptr
+      if (present(inl) .and. present(nnl) .and. present(nlat) .and. present(nltr)) then
+         inl0 = int(inl, c_int)
+         nnl0 = int(nnl, c_int)
+         nlat0 = int(nlat - 1, c_int)
+         nltr0 = int(nltr - 1, c_int)
+         list = list_c(c_loc(inl0), c_loc(nnl0), int(sum(nnl), c_int), c_loc(nlat0), c_loc(nltr0))
+         plist = c_loc(list)
+      end if
+      stat = c_get_hamiltonian(ctx, geom, plist, bas, h0, c_loc(ish_at), c_loc(iao_sh), &
+         & int(nao, c_int), c_loc(selfenergy), c_loc(overlap), c_loc(dpint), c_loc(qpint), &
+         & c_loc(hamiltonian))
+   contains
+      subroutine pack_tables(t, b, p)
+         type(tbx_h0_tables), intent(in), target :: t
+         type(basis_c), intent(out) :: b
+         type(h0par_c), intent(out) :: p
+         b = basis_c(int(t%nid, c_int), int(t%msh, c_int), c_loc(t%nsh_id), c_loc(t%ang), &
+            & c_loc(t%nprim), c_loc(t%alpha), c_loc(t%coeff), t%intcut)
+         p = h0par_c(c_loc(t%hscale), c_loc(t%shpoly), c_loc(t%rad))
+      end subroutine pack_tables
+   end subroutine tbx_h0_get_hamiltonian
+
+   !> get_hamiltonian_gradient: contracts the integral derivatives with the
+   !> density pmat and energy-weighted density xmat (nao, nao, nspin, magnet
+   !> representation) and the potential (vao(nao,nspin), vdp(3,nat,nspin),
+   !> vqp(6,nat,nspin)); dEdcn(nat), gradient(3,nat), sigma(3,3) accumulate.
+   subroutine tbx_h0_get_hamiltonian_gradient(ctx, tab, id, xyz, trans, cutoff, ish_at, iao_sh, &
+         & nao, selfenergy, dsedcn, vao, vdp, vqp, pmat, xmat, dEdcn, gradient, sigma, stat, &
+         & inl, nnl, nlat, nltr)
+      type(c_ptr), intent(in) :: ctx
+      type(tbx_h0_tables), intent(in), target :: tab
+      integer, intent(in) :: id(:)
+      real(c_double), intent(in), target, contiguous :: xyz(:, :), trans(:, :)
+      real(c_double), intent(in) :: cutoff
+      integer(c_int), intent(in), target, contiguous :: ish_at(:), iao_sh(:)
+      integer, intent(in) :: nao
+      real(c_double), intent(in), target, contiguous :: selfenergy(:), dsedcn(:)
+      real(c_double), intent(in), target, contiguous :: vao(:, :), vdp(:, :, :), vqp(:, :, :)
+      real(c_double), intent(in), target, contiguous :: pmat(:, :, :), xmat(:, :, :)
+      real(c_double), intent(inout), target, contiguous :: dEdcn(:), gradient(:, :), sigma(:, :)
+      integer, intent(out) :: stat
+      integer, intent(in), optional :: inl(:), nnl(:), nlat(:), nltr(:)
+      type(geom_c) :: geom
+      type(basis_c) :: bas
+      type(h0par_c) :: h0
+      type(list_c), target :: list
+      integer(c_int), allocatable, target :: id0(:), inl0(:), nnl0(:), nlat0(:), nltr0(:)
+      type(c_ptr) :: plist
+
+      id0 = int(id - 1, c_int)
+      bas = basis_c(int(tab%nid, c_int), int(tab%msh, c_int), c_loc(tab%nsh_id), c_loc(tab%ang), &
+         & c_loc(tab%nprim), c_loc(tab%alpha), c_loc(tab%coeff), tab%intcut)
+      h0 = h0par_c(c_loc(tab%hscale), c_loc(tab%shpoly), c_loc(tab%rad))
+      geom = geom_c(int(size(id), c_int), c_loc(id0), c_loc(xyz), int(size(trans, 2), c_int), &
+         & c_loc(trans), cutoff)
+      plist = c_null_ptr
+      if (present(inl) .and. present(nnl) .and. present(nlat) .and. present(nltr)) then
+         inl0 = int(inl, c_int)
+         nnl0 = int(nnl, c_int)
+         nlat0 = int(nlat - 1, c_int)
+         nltr0 = int(nltr - 1, c_int)
+         list = list_c(c_loc(inl0), c_loc(nnl0), int(sum(nnl), c_int), c_loc(nlat0), c_loc(nltr0))
+         plist = c_loc(list)
+      end if
+      stat = c_get_hamiltonian_gradient(ctx, geom, plist, bas, h0, c_loc(ish_at), c_loc(iao_sh), &
+         & int(nao, c_int), c_loc(selfenergy), c_loc(dsedcn), int(size(pmat, 3), c_int), &
+         & c_loc(vao), c_loc(vdp), c_loc(vqp), c_loc(pmat), c_loc(xmat), c_loc(dEdcn), &
+         & c_loc(gradient), c_loc(sigma))
+   end subroutine tbx_h0_get_hamiltonian_gradient
+
+end module tbx_h0

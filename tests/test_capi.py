@@ -91,3 +91,26 @@ def test_packed_maps_match_tables():
     f = ct.CDLL(str(L.LIB_PATH)).tbx_selftest_maps
     f.restype = ct.c_int
     assert f() == 0
+
+
+def test_fortran_driver_reads_input_and_reports_missing_device(tmp_path):
+    """The Fortran binding is built, its driver parses the stream input written
+    by tests/fortran_io.py, and without a GPU it stops with the C-ABI's error
+    message marshalled through tbx_h0_last_error (exit status 2)."""
+    import subprocess
+
+    torch = pytest.importorskip("torch")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present: the parity test covers the driver")
+    from fortran_io import DRIVER, write_input
+
+    st = structure.setup(structure.Structure(*synthetic.octane()))
+    se, dse = st.selfenergy(np.full(st.mol.nat, 2.0))
+    nao, nat = st.bas.nao, st.mol.nat
+    P = np.asfortranarray(np.eye(nao)[:, :, None])
+    write_input(tmp_path / "in.bin", st, se, dse, np.zeros((nao, 1)), np.zeros((3, nat, 1)),
+                np.zeros((6, nat, 1)), P, P)
+    r = subprocess.run([str(DRIVER), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2, r.stdout + r.stderr
+    assert "no context" in r.stdout and len(r.stdout.strip()) > len("tbx_h0_driver: no context:")
