@@ -124,6 +124,7 @@ def main():
     ap.add_argument("--nmol", type=int, default=NMOL)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--cpu-baseline-child", type=float, default=None)
     args = ap.parse_args()
 
@@ -141,7 +142,7 @@ def main():
     import torch
 
     from tblite_amd import h0 as h0mod
-    from tblite_amd import params, roofline, structure, synthetic
+    from tblite_amd import params, roofline, structure, synthetic, tune
     from tblite_amd.batch import Batch, h0_integrals
 
     t0 = time.time()
@@ -196,6 +197,11 @@ def main():
              torch.empty(3 * b.nat, dtype=torch.float64, device=dev),
              torch.empty(9 * b.nmol, dtype=torch.float64, device=dev))
     log(f"inputs ready; Kc={kc:.3e} (off-site {kc_off:.3e})")
+    # tile shapes: timed choice before the warm-up (set-up, not part of the timed steps)
+    tuned = None
+    if not args.no_tune:
+        tuned = tune.tune(b, xyz, se, (Sb, Db, Qb, Hb), (dsedcn, P, W, vao, vdp, vqp), log=log)
+        log(f"tiles: {tuned['tiles']}")
 
     def step_autograd():
         x = xyz.detach().requires_grad_(True)
@@ -302,6 +308,7 @@ def main():
         "adjoint_fp64_TFLOPs": adj_flop / (adj_a * 1e-3) / 1e12,
         "gradient_fp64_TFLOPs": adj_flop / (grad_p * 1e-3) / 1e12,
         "autograd_vs_plain_max_abs_diff": agree,
+        "tiles": b.tiles(),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

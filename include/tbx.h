@@ -176,6 +176,21 @@ int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
 int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pairs,
                           double *ms_gradient, double *ms_adjoint);
 
+/* Tile shapes of the batched kernels, no counterpart in the reference: the
+ * half list of every molecule is cut into work units of at most
+ * *_unit_tasks Cartesian-pair tasks (<= 256, >= the largest atom pair) and
+ * *_unit_pairs atom pairs (<= 128); kept primitive pairs are processed in
+ * chunks of *_chunk (1..64) per work item.  Defaults 256, 128, 8.
+ * tbx_batch_set_tiles rebuilds the units (host) and waits for the context's
+ * stream; tblite_amd/tune.py chooses them by timing the kernels. */
+typedef struct {
+    int fwd_unit_tasks, fwd_unit_pairs, fwd_chunk;
+    int grad_unit_tasks, grad_unit_pairs, grad_chunk;
+} tbx_tiles;
+
+int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles);
+int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles);
+
 /* Measurement helper, no counterpart in the reference: copies n doubles
  * d_src -> d_dst with a plain streaming kernel (k_copy) on `stream`.  Its
  * hardware-counter readings (FETCH_SIZE, WRITE_SIZE) for a known byte count

@@ -160,6 +160,20 @@ class Batch:
                                           self._stream()))
         return dxyz, dse
 
+    def tiles(self) -> dict:
+        """Current tile shapes (work-unit caps and chunk lengths) of the kernels."""
+        t = L.Tiles()
+        L.check(L.lib().tbx_batch_get_tiles(self.handle, ct.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.Tiles._fields_}
+
+    def set_tiles(self, **kw):
+        """Change tile shapes (keys as returned by ``tiles()``); rebuilds the
+        work units on the host and waits for pending launches of the batch."""
+        cur = self.tiles()
+        cur.update(kw)
+        t = L.Tiles(**cur)
+        L.check(L.lib().tbx_batch_set_tiles(self.handle, ct.byref(t)))
+
     def last_timing(self):
         v = [ct.c_double() for _ in range(4)]
         L.check(L.lib().tbx_batch_last_timing(self.handle, *[ct.byref(x) for x in v]))

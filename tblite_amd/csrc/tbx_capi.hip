@@ -22,7 +22,7 @@
 extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
                                          const double *xyz, const double *se, double *S, double *D,
                                          double *Q, double *H, long long ndense, const int *nnl,
-                                         int *flag, hipStream_t stream);
+                                         int *flag, int chunk, hipStream_t stream);
 extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const TbxMols *mol,
                                           const double *xyz, const double *se,
                                           const double *P, const double *W, const double *vao,
@@ -30,7 +30,7 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                                           long long pstride, long long vstride,
                                           const double *Sb, const double *Db, const double *Qb,
                                           const double *Hb, double *grad, double *dse,
-                                          double *sigma, const int *nnl, int *flag,
+                                          double *sigma, const int *nnl, int *flag, int chunk,
                                           hipStream_t stream);
 extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
                                        const double *dse, const double *dsedcn, const double *P,
@@ -248,7 +248,8 @@ struct HostMols {
 // bounded task count.  Gradient: j < i (j <= i with lattice images); forward:
 // j <= i, so that the on-site blocks are written by the same kernel.
 static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntrans, bool diag,
-                             std::vector<TbxSeg> &useg, std::vector<int> &unit_seg)
+                             int max_tasks, int max_pairs, std::vector<TbxSeg> &useg,
+                             std::vector<int> &unit_seg)
 {
     useg.clear();
     unit_seg.assign(1, 0);
@@ -268,7 +269,7 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
                 open_seg = false;
                 for (int j = a0; j < jend; ++j) {
                     const int c = nci * hs.ncart[hm.at_sp[j]];
-                    if (tasks + c > TBX_UNIT_TASKS || pairs == TBX_UNIT_PAIRS ||
+                    if (tasks + c > max_tasks || pairs == max_pairs ||
                         (!open_seg && segs == TBX_UNIT_SEGS))
                         close_unit();
                     if (!open_seg) {
@@ -285,10 +286,34 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
     }
 }
 
-static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans)
+static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans, const tbx_tiles &t)
 {
-    build_units_list(hs, hm, ntrans, false, hm.useg, hm.unit_seg);
-    build_units_list(hs, hm, ntrans, true, hm.fseg, hm.funit_seg);
+    build_units_list(hs, hm, ntrans, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg,
+                     hm.unit_seg);
+    build_units_list(hs, hm, ntrans, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg,
+                     hm.funit_seg);
+}
+
+static const tbx_tiles kDefaultTiles = {TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 8,
+                                        TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 8};
+
+// Tile shapes must fit the kernels' LDS arrays and hold the largest atom pair.
+static int check_tiles(const HostSpecies &hs, const tbx_tiles *t)
+{
+    const int need = hs.maxnc * hs.maxnc;
+    const int tasks[2] = {t->fwd_unit_tasks, t->grad_unit_tasks};
+    const int pairs[2] = {t->fwd_unit_pairs, t->grad_unit_pairs};
+    const int chunk[2] = {t->fwd_chunk, t->grad_chunk};
+    for (int k = 0; k < 2; ++k) {
+        if (tasks[k] < need || tasks[k] > TBX_UNIT_TASKS)
+            return fail(TBX_ERR_ARG, "unit tasks %d outside %d..%d", tasks[k], need,
+                        TBX_UNIT_TASKS);
+        if (pairs[k] < 1 || pairs[k] > TBX_UNIT_PAIRS)
+            return fail(TBX_ERR_ARG, "unit pairs %d outside 1..%d", pairs[k], TBX_UNIT_PAIRS);
+        if (chunk[k] < 1 || chunk[k] > 64)
+            return fail(TBX_ERR_ARG, "chunk %d outside 1..64", chunk[k]);
+    }
+    return TBX_OK;
 }
 
 static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const int *id,
@@ -457,6 +482,7 @@ struct tbx_batch {
     tbx_context *ctx = nullptr;
     HostSpecies hs;
     HostMols hm;
+    tbx_tiles tiles = kDefaultTiles;
     DevTables tab;
     DevBuf flag, dse_tmp;
     hipEvent_t ev[8] = {};
@@ -557,7 +583,7 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
     hipStream_t s = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
-    build_units(hs, hm, geom->ntrans);
+    build_units(hs, hm, geom->ntrans, kDefaultTiles);
     HIP_TRY(upload_mols(ctx->tab, hm, trans, s));
     HIP_TRY(ctx->xyz.reserve(sizeof(double) * 3 * geom->nat));
     HIP_TRY(hipMemcpyAsync(ctx->xyz.p, geom->xyz, sizeof(double) * 3 * geom->nat,
@@ -606,7 +632,7 @@ extern "C" int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const
     HIP_TRY(tbx_launch_forward(&ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
                                ctx->se.as<double>(), ctx->S.as<double>(), ctx->D.as<double>(),
                                ctx->Q.as<double>(), ctx->H.as<double>(), hm.ndense,
-                               list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), s));
+                               list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), 0, s));
     HIP_TRY(hipMemcpyAsync(overlap, ctx->S.p, sizeof(double) * n2, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(dpint, ctx->D.p, sizeof(double) * 3 * n2, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(qpint, ctx->Q.p, sizeof(double) * 6 * n2, hipMemcpyDeviceToHost, s));
@@ -660,7 +686,7 @@ extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *ge
                                 ctx->vdp.as<double>(), ctx->vqp.as<double>(), nspin, (long long)n2,
                                 nao, nullptr, nullptr, nullptr, nullptr, ctx->grad.as<double>(),
                                 ctx->dse.as<double>(), ctx->sigma.as<double>(),
-                                list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), s));
+                                list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), 0, s));
     HIP_TRY(tbx_launch_dedcn(&ctx->tab.sp, &ctx->tab.mt, ctx->dse.as<double>(),
                              ctx->dsedcn.as<double>(), ctx->P.as<double>(),
                              ctx->dEdcn.as<double>(), s));
@@ -694,7 +720,7 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     hipError_t e = hipSetDevice(ctx->device);
     std::vector<double> trans(3, 0.0);
     if (e == hipSuccess) e = upload_species(b->tab, b->hs, desc->cutoff, ctx->stream);
-    build_units(b->hs, b->hm, 1);
+    build_units(b->hs, b->hm, 1, b->tiles);
     if (e == hipSuccess) e = upload_mols(b->tab, b->hm, trans, ctx->stream);
     if (e == hipSuccess) e = b->flag.reserve(sizeof(int));
     // scratch of the gradient / adjoint launches: dE/dse (plain) or sigma (adjoint)
@@ -708,6 +734,27 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
         return fail(TBX_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e));
     }
     *out = b;
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
+{
+    if (!b || !tiles) return fail(TBX_ERR_ARG, "null argument");
+    if (int st = check_tiles(b->hs, tiles)) return st;
+    b->tiles = *tiles;
+    build_units(b->hs, b->hm, 1, b->tiles);
+    std::vector<double> trans(3, 0.0);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));  // no launch may still read the old units
+    HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles)
+{
+    if (!b || !tiles) return fail(TBX_ERR_ARG, "null argument");
+    *tiles = b->tiles;
     return TBX_OK;
 }
 
@@ -773,7 +820,7 @@ extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double
     hipStream_t s = pick_stream(b, stream);
     HIP_TRY(mark(b, 0, s));
     HIP_TRY(tbx_launch_forward(&b->tab.sp, &b->tab.mt, d_xyz, d_se, d_S, d_D, d_Q, d_H,
-                               b->hm.ndense, nullptr, b->flag.as<int>(), s));
+                               b->hm.ndense, nullptr, b->flag.as<int>(), b->tiles.fwd_chunk, s));
     HIP_TRY(mark(b, 1, s));
     return TBX_OK;
 }
@@ -793,7 +840,8 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
     HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
     HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp, d_vqp, 1, b->hm.ndense, b->hm.nao,
                                 nullptr, nullptr, nullptr, nullptr, d_grad,
-                                b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(), s));
+                                b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(),
+                                b->tiles.grad_chunk, s));
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
                              d_dEdcn, s));
     HIP_TRY(mark(b, 5, s));
@@ -812,7 +860,7 @@ extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double
     HIP_TRY(tbx_launch_fill(d_dse, b->hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar,
                                 d_Dbar, d_Qbar, d_Hbar, d_dxyz, d_dse, b->dse_tmp.as<double>(),
-                                nullptr, b->flag.as<int>(), s));
+                                nullptr, b->flag.as<int>(), b->tiles.grad_chunk, s));
     HIP_TRY(mark(b, 7, s));
     return TBX_OK;
 }

@@ -270,6 +270,7 @@ struct GradArgs {
     double *sigma;  // [9*nmol]
     const int *nnl; // optional [nat] list check
     int *flag;
+    int chunk;      // primitive pairs per work item
 };
 
 // Cartesian exponents (CCA order) of the functions of a shell, l <= 2.
@@ -357,7 +358,7 @@ static constexpr int kSigPack[9] = {0, 1, 3, 1, 2, 4, 3, 4, 5};
 // LDS accumulators of the row atom i of a gradient workgroup (one copy per
 // wavefront): dE/dR_i (3), dE/dse of its shells (MAXSH), sigma packed (6)
 #define ACCI_N (3 + MAXSH + 6)
-#define CHUNK 8    // primitive pairs per work item
+#define CHUNK_DEFAULT 8  // primitive pairs per work item (runtime-tunable per launch)
 #define NREC 20    // doubles of a task record
 
 // Task record fields (LDS, structure of arrays over the tasks of a unit)
@@ -548,6 +549,7 @@ struct FwdArgs {
     const double *se;    // [nsh]
     double *S, *D, *Q, *H;
     int accumulate;      // 1: add into zeroed outputs (lattice images), 0: plain stores
+    int chunk;           // primitive pairs per work item
 };
 
 // 1D factors V[m] = int (y+pa)^(a+m) (y+pb)^b, m = 0..2 (moments about A)
@@ -637,6 +639,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const long long dense0 = mt.mol_dense[m];
     const int ao_m = mt.mol_ao[m];
     const bool accum = A.accumulate != 0;
+    const int chunk = A.chunk;
     for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
     stage_unit(sp, mt, A.xyz, mt.fseg, s0, nseg, U, spidx);
 
@@ -675,7 +678,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12);
         tpp[tid] = pi0;
         tkept[tid] = kept;
-        nch = (kept + CHUNK - 1) / CHUNK;
+        nch = (kept + chunk - 1) / chunk;
     }
     // (3) chunk offsets
     const int nchunk = block_scan(nch, coff, U.wsum);
@@ -687,8 +690,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
         const int p = (info >> 12) & 4095;
         const int q = c - coff[tk];
-        const int p0 = tpp[tk] + q * CHUNK;
-        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
+        const int p0 = tpp[tk] + q * chunk;
+        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * chunk);
         const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
         double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         switch (t) {
@@ -991,7 +994,7 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
 // molecule, at most TBX_UNIT_TASKS Cartesian-pair tasks).  (1) The unit's
 // pairs are staged in LDS and their shell pairs laid out by type (l_i, l_j);
 // (2) one lane per task builds the task record and counts its kept primitive
-// pairs; (3) the kept pairs are cut into chunks of CHUNK and a block scan
+// pairs; (3) the kept pairs are cut into chunks of A.chunk and a block scan
 // turns them into a dense list of work items; (4) every lane runs work items
 // (type-major, so a wavefront mostly runs one specialised code path) and adds
 // the results to its pair's LDS sums; (5) the pair sums go to atoms j, the
@@ -1017,6 +1020,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     const int nao = mt.mol_nao[m];
     const long long dense0 = mt.mol_dense[m];
     const int ao_m = mt.mol_ao[m];
+    const int chunk = A.chunk;
     for (int k = tid; k < TBX_UNIT_PAIRS * (3 + 2 * MAXSH); k += NTHREADS) (&accP[0][0])[k] = 0.0;
     for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += NTHREADS) (&accS[0][0])[k] = 0.0;
     if (tid < 6) sig[tid] = 0.0;
@@ -1038,7 +1042,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
         tpp[tid] = ppoff;
         tkept[tid] = kept;
-        nch = (kept + CHUNK - 1) / CHUNK;
+        nch = (kept + chunk - 1) / chunk;
     }
     // (3) chunk offsets: block-wide exclusive scan
     const int nchunk = block_scan(nch, coff, U.wsum);
@@ -1051,8 +1055,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         const int p = (info >> 12) & 4095;
         const int ish = (info >> 24) & 7, jsh = (info >> 27) & 7;
         const int q = c - coff[tk];
-        const int p0 = tpp[tk] + q * CHUNK;
-        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * CHUNK);
+        const int p0 = tpp[tk] + q * chunk;
+        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * chunk);
         const double *r = rec + tk;
         const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
         double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
@@ -1234,10 +1238,10 @@ extern "C" hipError_t tbx_launch_fill(double *p, long long n, double v, hipStrea
 extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
                                          const double *xyz, const double *se, double *S, double *D,
                                          double *Q, double *H, long long ndense, const int *nnl,
-                                         int *flag, hipStream_t stream)
+                                         int *flag, int chunk, hipStream_t stream)
 {
     const int accumulate = mol->ntrans > 1;
-    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate};
+    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate, chunk > 0 ? chunk : CHUNK_DEFAULT};
     (void)hipGetLastError();  // launch errors are read below; drop stale ones
     if (accumulate) {  // lattice images add into the same elements
         double *outs[4] = {S, H, D, Q};
@@ -1262,11 +1266,11 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                                           long long pstride, long long vstride,
                                           const double *Sb, const double *Db, const double *Qb,
                                           const double *Hb, double *grad, double *dse,
-                                          double *sigma, const int *nnl, int *flag,
+                                          double *sigma, const int *nnl, int *flag, int chunk,
                                           hipStream_t stream)
 {
     GradArgs a{*sp, *mol, xyz, se, P, W, vao, vdp, vqp, nspin, pstride, vstride,
-               Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag};
+               Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag, chunk > 0 ? chunk : CHUNK_DEFAULT};
     (void)hipGetLastError();
     if (mol->nunit > 0) {
         if (mode == GRAD_PLAIN)

@@ -59,6 +59,12 @@ class BatchSizes(ct.Structure):
 _lib = None
 
 _P = ct.c_void_p
+class Tiles(ct.Structure):
+    _fields_ = [("fwd_unit_tasks", ct.c_int), ("fwd_unit_pairs", ct.c_int), ("fwd_chunk", ct.c_int),
+                ("grad_unit_tasks", ct.c_int), ("grad_unit_pairs", ct.c_int),
+                ("grad_chunk", ct.c_int)]
+
+
 _SIGS = {
     "tbx_version": (ct.c_int, []),
     "tbx_last_error": (ct.c_char_p, []),
@@ -81,6 +87,8 @@ _SIGS = {
     "tbx_batch_last_timing": (ct.c_int, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
                                          ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "tbx_probe_copy": (ct.c_int, [_P, _P, ct.c_longlong, _P]),
+    "tbx_batch_set_tiles": (ct.c_int, [_P, ct.POINTER(Tiles)]),
+    "tbx_batch_get_tiles": (ct.c_int, [_P, ct.POINTER(Tiles)]),
 }
 
 

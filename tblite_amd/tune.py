@@ -1,0 +1,78 @@
+"""Autotuner of the batched kernels' tile shapes.
+
+A tile shape is the work-unit cap (Cartesian-pair tasks per workgroup) and
+the chunk length (kept primitive pairs per work item) of ``k_forward`` and of
+``k_gradient``; the two kernels use separate work units, so each is tuned on
+its own.  Every candidate is applied with ``Batch.set_tiles`` and timed with
+the library's HIP events on the launch stream; the fastest shape per kernel
+is kept (and optionally cached on disk, keyed by the batch's shape)."""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from pathlib import Path
+
+import torch
+
+UNIT_TASKS = (192, 256)
+CHUNKS = (1, 2, 3, 4, 6, 8)
+
+
+def batch_key(b) -> str:
+    return f"nmol{b.nmol}-nat{b.nat}-nao{b.nao}-nd{b.ndense}-sp{'.'.join(map(str, b.nums))}"
+
+
+def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, unit_tasks=UNIT_TASKS,
+         chunks=CHUNKS, cache: str | os.PathLike | None = None, log=None):
+    """Choose tile shapes for batch ``b`` on the given inputs.
+
+    ``adjoints`` = (Sb, Db, Qb, Hb) times the adjoint kernel; ``plain`` =
+    (dsedcn, P, W, vao, vdp, vqp) additionally times the plain gradient
+    kernel (the gradient tile is then chosen on the sum of both).  Returns
+    ``{"tiles": {...}, "table": [...]}`` and leaves the best tiles applied."""
+    key = batch_key(b)
+    if cache is not None and Path(cache).exists():
+        known = json.loads(Path(cache).read_text())
+        if key in known:
+            b.set_tiles(**known[key]["tiles"])
+            return known[key]
+    min_tasks = int(max(sum((c.ang + 1) * (c.ang + 2) // 2 for c in sh)
+                        for sh in b.spec.basis_cgto)) ** 2
+    table = []
+    for tasks, chunk in itertools.product(unit_tasks, chunks):
+        if tasks < min_tasks:
+            continue
+        b.set_tiles(fwd_unit_tasks=tasks, fwd_chunk=chunk, grad_unit_tasks=tasks,
+                    grad_chunk=chunk)
+        f = a = g = 0.0
+        for _ in range(reps):
+            b.forward(xyz, se)
+            b.adjoint(xyz, se, *adjoints)
+            torch.cuda.synchronize()
+            t = b.last_timing()
+            f += t["forward_ms"] / reps
+            a += t["adjoint_ms"] / reps
+            if plain is not None:
+                b.gradient(xyz, se, *plain)
+                torch.cuda.synchronize()
+                g += b.last_timing()["gradient_ms"] / reps
+        row = {"unit_tasks": tasks, "chunk": chunk, "forward_ms": f, "adjoint_ms": a,
+               "gradient_ms": g}
+        table.append(row)
+        if log:
+            log(f"tune: tasks={tasks} chunk={chunk}: forward {f:.3f} ms, adjoint {a:.3f} ms"
+                + (f", gradient {g:.3f} ms" if plain is not None else ""))
+    bf = min(table, key=lambda r: r["forward_ms"])
+    bg = min(table, key=lambda r: r["adjoint_ms"] + r["gradient_ms"])
+    tiles = {"fwd_unit_tasks": bf["unit_tasks"], "fwd_chunk": bf["chunk"],
+             "grad_unit_tasks": bg["unit_tasks"], "grad_chunk": bg["chunk"]}
+    b.set_tiles(**tiles)
+    result = {"tiles": b.tiles(), "table": table}
+    if cache is not None:
+        p = Path(cache)
+        known = json.loads(p.read_text()) if p.exists() else {}
+        known[key] = result
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(known, indent=1))
+    return result

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tblite_amd import structure, synthetic
+from tblite_amd import lib as L, structure, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -231,3 +231,31 @@ def test_graph_replay_matches_eager(batch_case):
         del junk
         for o, r in zip(outs_f + outs_g, ref):
             assert torch.abs(o - r).max().item() < 1e-12, rep
+
+
+def test_tile_shapes_keep_results(batch_case):
+    """Other work-unit caps and chunk lengths (the autotuner's search space)
+    give the same integrals and adjoint gradient; invalid shapes are refused."""
+    mols, b, cn, se, dse = batch_case
+    _say("[gpu] tile shapes")
+    rng = np.random.default_rng(5)
+    adj = [torch.tensor(rng.normal(size=k * b.ndense), dtype=torch.float64, device="cuda")
+           for k in (1, 3, 6, 1)]
+    xyz = b.tensor_xyz()
+    ref_f = [t.clone() for t in b.forward(xyz, se)]
+    ref_a = [t.clone() for t in b.adjoint(xyz, se, *adj)]
+    default = b.tiles()
+    try:
+        for tasks, chunk in [(128, 4), (192, 12), (256, 1), (256, 16)]:
+            b.set_tiles(fwd_unit_tasks=tasks, fwd_chunk=chunk, grad_unit_tasks=tasks,
+                        grad_chunk=chunk)
+            for r, g in zip(ref_f, b.forward(xyz, se)):
+                assert torch.abs(r - g).max().item() < 1e-12, (tasks, chunk)
+            for r, g in zip(ref_a, b.adjoint(xyz, se, *adj)):
+                assert torch.abs(r - g).max().item() < 1e-10, (tasks, chunk)
+        with pytest.raises(L.TbxError):
+            b.set_tiles(fwd_unit_tasks=512)
+        with pytest.raises(L.TbxError):
+            b.set_tiles(grad_chunk=0)
+    finally:
+        b.set_tiles(**default)
