@@ -180,7 +180,8 @@ int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pai
  * half list of every molecule is cut into work units of at most
  * *_unit_tasks Cartesian-pair tasks (<= 256, >= the largest atom pair) and
  * *_unit_pairs atom pairs (<= 128); kept primitive pairs are processed in
- * chunks of *_chunk (1..64) per work item.  Defaults 256, 128, 8.
+ * chunks of *_chunk (1..64) per work item.  Defaults 256, 128 and chunks of
+ * 3 (forward) and 1 (gradient).
  * tbx_batch_set_tiles rebuilds the units (host) and waits for the context's
  * stream; tblite_amd/tune.py chooses them by timing the kernels. */
 typedef struct {

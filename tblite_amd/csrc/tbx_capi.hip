@@ -294,8 +294,9 @@ static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans, const t
                      hm.funit_seg);
 }
 
-static const tbx_tiles kDefaultTiles = {TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 8,
-                                        TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 8};
+// defaults: the autotuner's choice on the 2048-molecule batch
+static const tbx_tiles kDefaultTiles = {TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 3,
+                                        TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 1};
 
 // Tile shapes must fit the kernels' LDS arrays and hold the largest atom pair.
 static int check_tiles(const HostSpecies &hs, const tbx_tiles *t)

@@ -358,7 +358,9 @@ static constexpr int kSigPack[9] = {0, 1, 3, 1, 2, 4, 3, 4, 5};
 // LDS accumulators of the row atom i of a gradient workgroup (one copy per
 // wavefront): dE/dR_i (3), dE/dse of its shells (MAXSH), sigma packed (6)
 #define ACCI_N (3 + MAXSH + 6)
-#define CHUNK_DEFAULT 8  // primitive pairs per work item (runtime-tunable per launch)
+// primitive pairs per work item when the caller passes none (runtime-tunable per launch)
+#define FWD_CHUNK_DEFAULT 3
+#define GRAD_CHUNK_DEFAULT 1
 #define NREC 20    // doubles of a task record
 
 // Task record fields (LDS, structure of arrays over the tasks of a unit)
@@ -990,6 +992,93 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
     }
 }
 
+// 1D overlap table of one direction for a primitive pair (normalised
+// moments): O[a][b] = int (y+pa)^a (y+pb)^b for a < NA, b < NB, from the
+// recurrence O(a+1,0) = pa O(a,0) + a h O(a-1,0) (h = 1/(2 eta)) and the
+// transfer O(a,b+1) = O(a+1,b) + (pb-pa) O(a,b).
+template <int NA, int NB>
+__device__ __forceinline__ void os_table(double pa, double pb, double h,
+                                         double O[NA + NB - 1][NB])
+{
+    constexpr int N0 = NA + NB - 1;
+    O[0][0] = 1.0;
+    if (N0 > 1) O[1][0] = pa;
+#pragma unroll
+    for (int a = 1; a + 1 < N0; ++a) O[a + 1][0] = pa * O[a][0] + (a * h) * O[a - 1][0];
+    const double d = pb - pa;
+#pragma unroll
+    for (int b = 0; b + 1 < NB; ++b)
+#pragma unroll
+        for (int a = 0; a + 1 < N0 - b; ++a) O[a][b + 1] = O[a + 1][b] + d * O[a][b];
+}
+
+// Primitive pairs [p0, p1) of an s/p shell pair (LI, LJ <= 1): the 1D tables
+// are built once per primitive pair and shared by the Cartesian pairs of one
+// row ki = ROW (ROW < 0: every row), with compile-time exponents; each
+// Cartesian pair reads its task record (rec + cp, rec = the row's first
+// Cartesian pair) and adds its contracted derivative with the explicit v
+// terms to g and its H0 share to ds.
+template <int LI, int LJ, int ROW>
+__device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, const double *rec,
+                                              int ld, const double vec[3], int p0, int p1,
+                                              double g[3], double &ds)
+{
+    constexpr int NCI = (LI + 1) * (LI + 2) / 2, NCJ = (LJ + 1) * (LJ + 2) / 2;
+    constexpr int NA = LI + 4, NB = LJ + 1;
+    const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    for (int p = p0; p < p1; ++p) {
+        const double *e = pp + TBX_PP_N * p;
+        const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
+        const double h = e[TBX_PP_H], ai = e[TBX_PP_AI], pre = e[TBX_PP_PRE];
+        const double cc = exp(-(mu * r2)) * pre;
+        const double ta = 2.0 * ai;
+        double O[3][NA + NB - 1][NB];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) os_table<NA, NB>(-vec[k] * ajo, vec[k] * aio, h, O[k]);
+#pragma unroll
+        for (int ki = (ROW < 0 ? 0 : ROW); ki < (ROW < 0 ? NCI : ROW + 1); ++ki)
+#pragma unroll
+            for (int kj = 0; kj < NCJ; ++kj) {
+                // one Cartesian pair at a time: keeps the scheduler from
+                // hoisting every pair's record loads (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+                const double *r = rec + ((ROW < 0 ? ki : 0) * NCJ + kj);
+                double V[3][3], G[3][3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int a = cart_exp(LI, ki, k), b = cart_exp(LJ, kj, k);
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        V[k][m] = O[k][a + m][b];
+                        G[k][m] = (a + m > 0) ? ta * O[k][a + m + 1][b] -
+                                                    (double)(a + m) * O[k][a + m - 1][b]
+                                              : ta * O[k][a + m + 1][b];
+                    }
+                }
+                double w[10];
+#pragma unroll
+                for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
+                const double S = V[0][0] * V[1][0] * V[2][0];
+                const double D0 = V[0][1] * V[1][0] * V[2][0];
+                const double D1 = V[0][0] * V[1][1] * V[2][0];
+                const double D2 = V[0][0] * V[1][0] * V[2][1];
+                const double *wq = r + R_WQ * ld;
+                const double wq0 = wq[0], wq1 = wq[ld], wq2 = wq[2 * ld], wq3 = wq[3 * ld],
+                             wq4 = wq[4 * ld], wq5 = wq[5 * ld];
+                const double WD0 = wq0 * D0 + 0.5 * (wq1 * D1 + wq3 * D2);
+                const double WD1 = wq2 * D1 + 0.5 * (wq1 * D0 + wq4 * D2);
+                const double WD2 = wq5 * D2 + 0.5 * (wq3 * D0 + wq4 * D1);
+                g[0] += cc * (contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4) +
+                              S * r[R_EX * ld] + 2.0 * WD0);
+                g[1] += cc * (contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4) +
+                              S * r[(R_EX + 1) * ld] + 2.0 * WD1);
+                g[2] += cc * (contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4) +
+                              S * r[(R_EX + 2) * ld] + 2.0 * WD2);
+                ds += cc * S * r[R_HSW * ld];
+            }
+    }
+}
+
 // One workgroup per work unit (up to TBX_UNIT_PAIRS atom pairs of one
 // molecule, at most TBX_UNIT_TASKS Cartesian-pair tasks).  (1) The unit's
 // pairs are staged in LDS and their shell pairs laid out by type (l_i, l_j);
@@ -1042,7 +1131,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
         tpp[tid] = ppoff;
         tkept[tid] = kept;
-        nch = (kept + chunk - 1) / chunk;
+        // s/p shell pairs run whole on their first Cartesian pair (p-p: by row,
+        // on the first Cartesian pair of each row)
+        const bool whole = (d.x == 0 || d.x == 1 || d.x == 3);
+        const bool rows = (d.x == 4);
+        nch = ((whole && d.w != 0) || (rows && d.z != 0)) ? 0 : (kept + chunk - 1) / chunk;
     }
     // (3) chunk offsets: block-wide exclusive scan
     const int nchunk = block_scan(nch, coff, U.wsum);
@@ -1059,30 +1152,43 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * chunk);
         const double *r = rec + tk;
         const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
-        double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
         const int L = TBX_UNIT_TASKS;
-        switch (t) {
-        case 0: chunk_run<0, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 1: chunk_run<0, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 2: chunk_run<0, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 3: chunk_run<1, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 4: chunk_run<1, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 5: chunk_run<1, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 6: chunk_run<2, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        case 7: chunk_run<2, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-        default: chunk_run<2, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+        double g[3] = {0, 0, 0}, ds = 0.0;
+        if (t == 0 || t == 1 || t == 3 || t == 4) {
+            switch (t) {
+            case 0: shellpair_run<0, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
+            case 1: shellpair_run<0, 1, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
+            case 3: shellpair_run<1, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
+            default:  // p-p by rows
+                if (ki == 0) shellpair_run<1, 1, 0>(sp.pp, r, L, vec, p0, p1, g, ds);
+                else if (ki == 1) shellpair_run<1, 1, 1>(sp.pp, r, L, vec, p0, p1, g, ds);
+                else shellpair_run<1, 1, 2>(sp.pp, r, L, vec, p0, p1, g, ds);
+                break;
+            }
+        } else {
+            double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
+            switch (t) {
+            case 2: chunk_run<0, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 5: chunk_run<1, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 6: chunk_run<2, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 7: chunk_run<2, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            default: chunk_run<2, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            }
+            // explicit v dependence: S ex + 2 Wj D_i
+            const double *wq = r + R_WQ * L;
+            const double wq0 = wq[0], wq1 = wq[L], wq2 = wq[2 * L], wq3 = wq[3 * L],
+                         wq4 = wq[4 * L], wq5 = wq[5 * L];
+            const double WD0 = wq0 * dacc[0] + 0.5 * (wq1 * dacc[1] + wq3 * dacc[2]);
+            const double WD1 = wq2 * dacc[1] + 0.5 * (wq1 * dacc[0] + wq4 * dacc[2]);
+            const double WD2 = wq5 * dacc[2] + 0.5 * (wq3 * dacc[0] + wq4 * dacc[1]);
+            g[0] = a3[0] + s * r[R_EX * L] + 2.0 * WD0;
+            g[1] = a3[1] + s * r[(R_EX + 1) * L] + 2.0 * WD1;
+            g[2] = a3[2] + s * r[(R_EX + 2) * L] + 2.0 * WD2;
+            ds = s * r[R_HSW * L];
         }
-        // explicit v dependence: S ex + 2 Wj D_i
-        const double *wq = r + R_WQ * L;
-        const double wq0 = wq[0], wq1 = wq[L], wq2 = wq[2 * L], wq3 = wq[3 * L],
-                     wq4 = wq[4 * L], wq5 = wq[5 * L];
-        const double WD0 = wq0 * dacc[0] + 0.5 * (wq1 * dacc[1] + wq3 * dacc[2]);
-        const double WD1 = wq2 * dacc[1] + 0.5 * (wq1 * dacc[0] + wq4 * dacc[2]);
-        const double WD2 = wq5 * dacc[2] + 0.5 * (wq3 * dacc[0] + wq4 * dacc[1]);
-        atomicAdd(&accP[p][0], a3[0] + s * r[R_EX * L] + 2.0 * WD0);
-        atomicAdd(&accP[p][1], a3[1] + s * r[(R_EX + 1) * L] + 2.0 * WD1);
-        atomicAdd(&accP[p][2], a3[2] + s * r[(R_EX + 2) * L] + 2.0 * WD2);
-        const double ds = s * r[R_HSW * L];
+        atomicAdd(&accP[p][0], g[0]);
+        atomicAdd(&accP[p][1], g[1]);
+        atomicAdd(&accP[p][2], g[2]);
         atomicAdd(&accP[p][3 + ish], ds);
         atomicAdd(&accP[p][3 + MAXSH + jsh], ds);
     }
@@ -1241,7 +1347,7 @@ extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mo
                                          int *flag, int chunk, hipStream_t stream)
 {
     const int accumulate = mol->ntrans > 1;
-    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate, chunk > 0 ? chunk : CHUNK_DEFAULT};
+    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate, chunk > 0 ? chunk : FWD_CHUNK_DEFAULT};
     (void)hipGetLastError();  // launch errors are read below; drop stale ones
     if (accumulate) {  // lattice images add into the same elements
         double *outs[4] = {S, H, D, Q};
@@ -1270,7 +1376,8 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                                           hipStream_t stream)
 {
     GradArgs a{*sp, *mol, xyz, se, P, W, vao, vdp, vqp, nspin, pstride, vstride,
-               Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag, chunk > 0 ? chunk : CHUNK_DEFAULT};
+               Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag,
+               chunk > 0 ? chunk : GRAD_CHUNK_DEFAULT};
     (void)hipGetLastError();
     if (mol->nunit > 0) {
         if (mode == GRAD_PLAIN)
