@@ -16,7 +16,7 @@ from pathlib import Path
 import torch
 
 UNIT_TASKS = (192, 256)
-CHUNKS = (1, 2, 3, 4, 6, 8)
+CHUNKS = (1, 2, 3, 4, 6)
 
 
 def batch_key(b) -> str:
