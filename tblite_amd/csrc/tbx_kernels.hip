@@ -514,6 +514,42 @@ __device__ __forceinline__ int4 decode_task(const UnitStage &U, int task, int &s
     return make_int4(t, cp / ncj, cp % ncj, cp);  // (type, ki, kj, cp)
 }
 
+// Shell-pair phase of a unit (one lane per shell pair): screening count and
+// first entry of the sorted primitive-pair list, and the H0 factors
+// hs = hscale * shpoly(R), dsr = (d shpoly/dr) / (r shpoly), sesum =
+// (se_i + se_j)/2.  onsite: the j == i, T = 0 block counts (forward only).
+__device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const UnitStage &U,
+                                                const double *se, int k, bool onsite_ok,
+                                                int &kept, int &pi0, double &hs, double &sesum,
+                                                double &dsr)
+{
+    const int4 e = unpack_spl(U.spl[k]);
+    const PairInfo pr = U.pinf[e.x];
+    const int ish = e.z, jsh = e.y;
+    const double vx = U.pv[e.x][0], vy = U.pv[e.x][1], vz = U.pv[e.x][2];
+    const double r2 = vx * vx + vy * vy + vz * vz;
+    const int sloti = ish + sp.msh * pr.spi, slotj = jsh + sp.msh * pr.spj;
+    pi0 = sp.pp_off[sloti * sp.nslot + slotj];
+    kept = 0;
+    hs = sesum = dsr = 0.0;
+    const bool onsite = onsite_ok && (pr.i == pr.j) && r2 < kEps;
+    if (!onsite && (r2 < kEps || r2 > sp.cutoff2)) return;
+    const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
+    kept = kept_pairs(sp.pp_mu + pi0, npp, onsite ? 0.0 : r2, sp.intcut);
+    sesum = 0.5 * (se[pr.shi + ish] + se[pr.shj + jsh]);
+    if (onsite) {
+        hs = 1.0;  // on-site block: H = S (se_i + se_j)/2
+        return;
+    }
+    const double hsc = sp.hscale[jsh + sp.msh * (ish + sp.msh * (pr.spj + sp.nsp * pr.spi))];
+    const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
+    const double rr = sqrt(sqrt(r2) / (sp.rad[pr.spi] + sp.rad[pr.spj]));
+    const double shi = 1.0 + kpi * rr, shj = 1.0 + kpj * rr;
+    const double shp = shi * shj;
+    hs = hsc * shp;
+    dsr = (shi * kpj + shj * kpi) * 0.5 * rr / (r2 * shp);
+}
+
 // block-wide exclusive scan of one int per lane; returns the total
 __device__ __forceinline__ int block_scan(int v, int *out, int (*wsum)[9])
 {
@@ -624,6 +660,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ UnitStage U;
     __shared__ double acc[10][TBX_UNIT_TASKS];
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
+    __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
     __shared__ unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | pair << 12
     __shared__ int tpp[TBX_UNIT_TASKS];
@@ -645,40 +682,27 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
     stage_unit(sp, mt, A.xyz, mt.fseg, s0, nseg, U, spidx);
 
-    // (2) tasks
+    // (2) shell pairs: screening counts and H0 factors (one lane per shell pair)
+    const int nspair = U.tsp[9];
+    for (int k = tid; k < nspair; k += NTHREADS) {
+        int kept, pi0;
+        double hs, sesum, dsr;
+        shellpair_setup(sp, U, A.se, k, true, kept, pi0, hs, sesum, dsr);
+        spk[k] = kept;
+        spp[k] = pi0;
+        hfac[k] = sesum * hs;
+    }
+    __syncthreads();
+    // tasks: one Cartesian pair per lane
     const int ntask = U.toff[9];
     int nch = 0;
     if (tid < ntask) {
         int spx;
         const int4 d = decode_task(U, tid, spx);
         const int4 e = unpack_spl(U.spl[spx]);
-        const PairInfo pr = U.pinf[e.x];
-        const double vx = U.pv[e.x][0], vy = U.pv[e.x][1], vz = U.pv[e.x][2];
-        const double r2 = vx * vx + vy * vy + vz * vz;
-        const int ish = e.z, jsh = e.y;
-        const int sloti = ish + sp.msh * pr.spi, slotj = jsh + sp.msh * pr.spj;
-        const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
-        const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
-        const bool onsite = (pr.i == pr.j) && r2 < kEps;
-        const bool inrange = onsite || (r2 >= kEps && r2 <= sp.cutoff2);
-        const int kept = inrange ? kept_pairs(sp.pp_mu + pi0, npp,
-                                              onsite ? 0.0 : r2, sp.intcut)
-                                 : 0;
-        if (d.w == 0) {
-            const double sesum = 0.5 * (A.se[pr.shi + ish] + A.se[pr.shj + jsh]);
-            double hf = 0.0;
-            if (onsite) {
-                hf = sesum;
-            } else if (inrange) {
-                const double hsc =
-                    sp.hscale[jsh + sp.msh * (ish + sp.msh * (pr.spj + sp.nsp * pr.spi))];
-                const double rr = sqrt(sqrt(r2) / (sp.rad[pr.spi] + sp.rad[pr.spj]));
-                hf = sesum * hsc * (1.0 + sp.shpoly[sloti] * rr) * (1.0 + sp.shpoly[slotj] * rr);
-            }
-            hfac[spx] = hf;
-        }
+        const int kept = spk[spx];
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12);
-        tpp[tid] = pi0;
+        tpp[tid] = spp[spx];
         tkept[tid] = kept;
         nch = (kept + chunk - 1) / chunk;
     }
@@ -831,23 +855,14 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 // shifts the j-centred multipole weights to centre i and stores the record.
 // Returns the number of primitive pairs kept by the screening.
 template <int MODE>
-__device__ int task_setup(const GradArgs &A, const PairInfo &pr, const double v[3], int ish,
-                          int jsh, int ki, int kj, int nao, long long dense0, int ao_m,
-                          double *rec, int ld, int &ppoff)
+__device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v[3], int ish,
+                           int jsh, int li, int lj, int ki, int kj, int nao, long long dense0,
+                           int ao_m, double hs, double sesum, double dsr, double *rec, int ld)
 {
     const TbxSpecies &sp = A.sp;
     const int i = pr.i, j = pr.j, spi = pr.spi, spj = pr.spj;
     const int sloti = ish + sp.msh * spi, slotj = jsh + sp.msh * spj;
-    const int li = sp.ang[sloti], lj = sp.ang[slotj];
     const double vx = v[0], vy = v[1], vz = v[2];
-    const double r2 = vx * vx + vy * vy + vz * vz;
-    if (r2 < kEps || r2 > sp.cutoff2) return 0;
-    const int pi0 = sp.pp_off[sloti * sp.nslot + slotj];
-    const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
-    const int kept = kept_pairs(sp.pp_mu + pi0, npp, r2, sp.intcut);
-    ppoff = pi0;
-    if (kept == 0) return 0;
-
     const int muj0 = pr.aoj + sp.sh_ao[slotj];
     const int nui0 = pr.aoi + sp.sh_ao[sloti];
     const int na = cart_n(lj, kj), nb = cart_n(li, ki);
@@ -924,15 +939,8 @@ __device__ int task_setup(const GradArgs &A, const PairInfo &pr, const double v[
     const double Wv[3] = {wQj[0] * vx + 0.5 * (wQj[1] * vy + wQj[3] * vz),
                           wQj[2] * vy + 0.5 * (wQj[1] * vx + wQj[4] * vz),
                           wQj[5] * vz + 0.5 * (wQj[3] * vx + wQj[4] * vy)};
-    // H0 = S * (se_i + se_j)/2 * hscale * shpoly(R); shpoly's R dependence is explicit
-    const double hsc = sp.hscale[jsh + sp.msh * (ish + sp.msh * (spj + sp.nsp * spi))];
-    const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
-    const double rr = sqrt(sqrt(r2) / (sp.rad[spi] + sp.rad[spj]));
-    const double shi = 1.0 + kpi * rr, shj = 1.0 + kpj * rr;
-    const double shp = shi * shj;
-    const double dshp = (shi * kpj + shj * kpi) * 0.5 * rr / r2;  // (d shp / d r) / r
-    const double hs = hsc * shp;
-    const double sesum = 0.5 * (A.se[pr.shi + ish] + A.se[pr.shj + jsh]);
+    // H0 = S * (se_i + se_j)/2 * hscale * shpoly(R): hs = hscale * shpoly and
+    // dsr = (d shpoly / d r) / (r shpoly) come from the shell-pair phase
     const double hw = sesum * hs * wH;
     rec[(R_W + 0) * ld] = wS + hw + wDj[0] * vx + wDj[1] * vy + wDj[2] * vz + Wv[0] * vx +
                           Wv[1] * vy + Wv[2] * vz;
@@ -941,15 +949,17 @@ __device__ int task_setup(const GradArgs &A, const PairInfo &pr, const double v[
     rec[(R_W + 3) * ld] = wDi[2] + wDj[2] + 2.0 * Wv[2];
 #pragma unroll
     for (int k = 0; k < 6; ++k) rec[(R_W + 4 + k) * ld] = wQi[k] + wQj[k];
-    const double hd = hw * dshp / shp;
+    const double hd = hw * dsr;
     rec[(R_EX + 0) * ld] = wDj[0] + 2.0 * Wv[0] + hd * vx;
     rec[(R_EX + 1) * ld] = wDj[1] + 2.0 * Wv[1] + hd * vy;
     rec[(R_EX + 2) * ld] = wDj[2] + 2.0 * Wv[2] + hd * vz;
 #pragma unroll
     for (int k = 0; k < 6; ++k) rec[(R_WQ + k) * ld] = wQj[k];
     rec[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
-    return kept;
+    (void)sloti;
+    (void)slotj;
 }
+
 
 // Primitive pairs [p0, p1) of one task of type (LI, LJ): accumulate the
 // contracted derivative a3 = sum cc w.dM/dR_i, the overlap s and the i-centred
@@ -1100,6 +1110,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     __shared__ int tpp[TBX_UNIT_TASKS];    // first primitive pair of the task
     __shared__ int tkept[TBX_UNIT_TASKS];
     __shared__ int coff[TBX_UNIT_TASKS + 1];  // first chunk of each task
+    __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
+    __shared__ double sph[3][TBX_UNIT_TASKS];  // hs, sesum, dsr per shell pair
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -1117,19 +1129,35 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     stage_unit(sp, mt, A.xyz, mt.seg, s0, nseg, U, nullptr);
     const int npair = U.sfirst[nseg];
 
-    // (2) task records: one task per lane (ntask <= TBX_UNIT_TASKS = NTHREADS)
+    // (2) shell pairs: screening counts and H0 factors (one lane per shell pair)
+    const int nspair = U.tsp[9];
+    for (int k = tid; k < nspair; k += NTHREADS) {
+        int kept, pi0;
+        double hs, sesum, dsr;
+        shellpair_setup(sp, U, A.se, k, false, kept, pi0, hs, sesum, dsr);
+        spk[k] = kept;
+        spp[k] = pi0;
+        sph[0][k] = hs;
+        sph[1][k] = sesum;
+        sph[2][k] = dsr;
+    }
+    __syncthreads();
+    // task records: one Cartesian pair per lane (ntask <= TBX_UNIT_TASKS = NTHREADS)
     const int ntask = U.toff[9];
     int nch = 0;
     if (tid < ntask) {
         int spx;
         const int4 d = decode_task(U, tid, spx);
         const int4 e = unpack_spl(U.spl[spx]);
-        const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
-        int ppoff = 0;
-        const int kept = task_setup<MODE>(A, U.pinf[e.x], v, e.z, e.y, d.y, d.z, nao, dense0, ao_m,
-                                          rec + tid, TBX_UNIT_TASKS, ppoff);
+        const int kept = spk[spx];
+        if (kept > 0) {
+            const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
+            task_setup<MODE>(A, U.pinf[e.x], v, e.z, e.y, d.x / 3, d.x % 3, d.y, d.z, nao,
+                             dense0, ao_m, sph[0][spx], sph[1][spx], sph[2][spx], rec + tid,
+                             TBX_UNIT_TASKS);
+        }
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
-        tpp[tid] = ppoff;
+        tpp[tid] = spp[spx];
         tkept[tid] = kept;
         // s/p shell pairs run whole on their first Cartesian pair (p-p: by row,
         // on the first Cartesian pair of each row)
