@@ -102,7 +102,7 @@ hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s)
 struct HostSpecies {
     int nsp = 0, msh = 0, maxnc = 0;
     std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao,
-        prim_off, pp_off, ao_info;
+        prim_off, pp_off, ao_info, ao_desc;
     std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp, pp_mu;
     double intcut = 25.0;
 };
@@ -231,6 +231,12 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
         const int isp = (int)(k / TBX_MAXAO);
         const int sh = hs.ao_sh[k];
         hs.ao_info[k] = sh | (hs.ao_m[k] << 4) | (hs.ang[sh + msh * isp] << 8);
+    }
+    hs.ao_desc.assign(nsp, 0);
+    for (int isp = 0; isp < nsp; ++isp) {
+        int d = hs.nsh[isp];
+        for (int ish = 0; ish < hs.nsh[isp]; ++ish) d |= hs.ang[ish + msh * isp] << (3 + 2 * ish);
+        hs.ao_desc[isp] = d;
     }
     return TBX_OK;
 }
@@ -374,7 +380,7 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
 // Device copies of the tables
 struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
-        ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off, pp_mu, ao_info;
+        ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off, pp_mu, ao_info, ao_desc;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
         cart_atom, ao_atom, mol_ao, unit_seg, useg, funit_seg, fseg;
     TbxSpecies sp{};
@@ -384,7 +390,7 @@ struct DevTables {
     {
         DevBuf *all[] = {&nsh, &ang, &nprim, &alpha, &coeff, &shpoly, &rad, &hscale, &ncart, &nao,
                          &cart_sh, &cart_exp, &ao_sh, &ao_m, &sh_cart, &sh_ao, &prim, &prim_off,
-                         &pp, &pp_off, &pp_mu, &ao_info,
+                         &pp, &pp_off, &pp_mu, &ao_info, &ao_desc,
                          &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
                          &at_cart, &cart_atom, &ao_atom, &mol_ao, &unit_seg, &useg,
@@ -399,7 +405,7 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
 #define UP(name) if ((e = upload(d.name, hs.name, s)) != hipSuccess) return e
     UP(nsh); UP(ang); UP(nprim); UP(alpha); UP(coeff); UP(shpoly); UP(rad); UP(hscale); UP(ncart);
     UP(nao); UP(cart_sh); UP(cart_exp); UP(ao_sh); UP(ao_m); UP(sh_cart); UP(sh_ao); UP(prim);
-    UP(prim_off); UP(pp); UP(pp_off); UP(pp_mu); UP(ao_info);
+    UP(prim_off); UP(pp); UP(pp_off); UP(pp_mu); UP(ao_info); UP(ao_desc);
 #undef UP
     TbxSpecies &sp = d.sp;
     sp.nsp = hs.nsp;
@@ -428,6 +434,7 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     sp.pp = d.pp.as<double>();
     sp.pp_mu = d.pp_mu.as<double>();
     sp.ao_info = d.ao_info.as<int>();
+    sp.ao_desc = d.ao_desc.as<int>();
     sp.intcut = hs.intcut;
     sp.cutoff2 = cutoff * cutoff;
     return hipSuccess;

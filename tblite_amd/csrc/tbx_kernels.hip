@@ -377,7 +377,29 @@ struct PairInfo {
     int spi, spj;
     int aoi, aoj;  // first AO of the atoms, molecule-local
     int shi, shj;  // first shell of the atoms (global)
+    int di, dj;    // shell layout of the species (TbxSpecies::ao_desc)
 };
+
+// shell, m and l of spherical function ao of a species with shell layout desc
+__device__ __forceinline__ void ao_decode(int desc, int ao, int &sh, int &m, int &l)
+{
+    const int nsh = desc & 7;
+    sh = 0;
+    m = ao;
+    bool done = false;
+#pragma unroll
+    for (int s = 0; s < MAXSH - 1; ++s) {
+        const int n = 2 * ((desc >> (3 + 2 * s)) & 3) + 1;
+        const bool adv = !done && (s < nsh - 1) && (m >= n);
+        if (adv) {
+            m -= n;
+            sh = s + 1;
+        } else {
+            done = true;
+        }
+    }
+    l = (desc >> (3 + 2 * sh)) & 3;
+}
 
 // inclusive prefix sum over the 64 lanes of a wavefront
 __device__ __forceinline__ int wave_scan(int v)
@@ -407,7 +429,6 @@ __device__ __forceinline__ int kept_pairs(const double *mu, int n, double r2, do
 struct UnitStage {
     PairInfo pinf[TBX_UNIT_PAIRS];
     double pv[TBX_UNIT_PAIRS][3];  // v = R_i - R_j - T
-    int pseg[TBX_UNIT_PAIRS];      // segment of each pair
     int sfirst[TBX_UNIT_SEGS + 1]; // first pair of each segment
     int spl[TBX_UNIT_TASKS];       // shell pairs: pair | jsh << 8 | ish << 12 | type << 16
                                    // (type-major)
@@ -450,8 +471,9 @@ __device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double
         pr.aoj = mt.at_ao[pr.j];
         pr.shi = mt.at_sh[pr.i];
         pr.shj = mt.at_sh[pr.j];
+        pr.di = sp.ao_desc[pr.spi];
+        pr.dj = sp.ao_desc[pr.spj];
         U.pinf[tid] = pr;
-        U.pseg[tid] = k;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             U.pv[tid][c] = xyz[3 * pr.i + c] - xyz[3 * pr.j + c] - mt.trans[3 * g.itr + c];
@@ -662,9 +684,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
     __shared__ unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
-    __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | pair << 12
-    __shared__ int tpp[TBX_UNIT_TASKS];
-    __shared__ int tkept[TBX_UNIT_TASKS];
+    __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12
     __shared__ int coff[TBX_UNIT_TASKS + 1];
     __shared__ int eoff[2 * TBX_UNIT_SEGS + 1];  // element offsets: A blocks, then mirrors
     __shared__ int erow[TBX_UNIT_SEGS][3];       // first row, rows of block A, AOs of atom i
@@ -699,12 +719,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     if (tid < ntask) {
         int spx;
         const int4 d = decode_task(U, tid, spx);
-        const int4 e = unpack_spl(U.spl[spx]);
-        const int kept = spk[spx];
-        tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12);
-        tpp[tid] = spp[spx];
-        tkept[tid] = kept;
-        nch = (kept + chunk - 1) / chunk;
+        tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (spx << 12);
+        nch = (spk[spx] + chunk - 1) / chunk;
     }
     // (3) chunk offsets
     const int nchunk = block_scan(nch, coff, U.wsum);
@@ -714,10 +730,11 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         const int tk = find_last_le(coff, NTHREADS, c);
         const int info = tinfo[tk];
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
-        const int p = (info >> 12) & 4095;
+        const int spx = (info >> 12) & 4095;
+        const int p = U.spl[spx] & 255;
         const int q = c - coff[tk];
-        const int p0 = tpp[tk] + q * chunk;
-        const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * chunk);
+        const int p0 = spp[spx] + q * chunk;
+        const int p1 = spp[spx] + min(spk[spx], (q + 1) * chunk);
         const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
         double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         switch (t) {
@@ -784,13 +801,19 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
                 mu = erow[k][0] + local / naoi;
                 nul = local - (local / naoi) * naoi;
             }
-            const int jj = mt.ao_atom[ao_m + mu];
-            const int p = U.sfirst[k] + (jj - mt.fseg[s0 + k].j0);
+            // pair of row mu: the pairs of a segment hold consecutive atoms j
+            int p = U.sfirst[k];
+            {
+                int hi = U.sfirst[k + 1];
+                while (hi - p > 1) {
+                    const int mid = (p + hi) >> 1;
+                    if (U.pinf[mid].aoj <= mu) p = mid; else hi = mid;
+                }
+            }
             const PairInfo pr = U.pinf[p];
-            const int infi = sp.ao_info[pr.spi * TBX_MAXAO + nul];
-            const int infj = sp.ao_info[pr.spj * TBX_MAXAO + mu - pr.aoj];
-            const int ish = infi & 15, mi = (infi >> 4) & 15, li = infi >> 8;
-            const int jsh = infj & 15, mj = (infj >> 4) & 15, lj = infj >> 8;
+            int ish, mi, li, jsh, mj, lj;
+            ao_decode(pr.di, nul, ish, mi, li);
+            ao_decode(pr.dj, mu - pr.aoj, jsh, mj, lj);
             const int spx = spidx[p * MAXSH * MAXSH + ish * MAXSH + jsh];
             const int t = li * 3 + lj;
             const int ncj = ncart_of(lj);
@@ -1225,7 +1248,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     // (5) pair sums -> atom j (HBM), segment sums (atom i) and sigma (LDS)
     if (tid < npair) {
         const PairInfo pr = U.pinf[tid];
-        const int k = U.pseg[tid];
+        const int k = find_last_le(U.sfirst, nseg, tid);
         const bool self = (pr.i == pr.j);
         const double G0 = accP[tid][0], G1 = accP[tid][1], G2 = accP[tid][2];
         const double vx = U.pv[tid][0], vy = U.pv[tid][1], vz = U.pv[tid][2];

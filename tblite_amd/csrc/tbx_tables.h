@@ -47,6 +47,7 @@ struct TbxSpecies {
     const double *pp;     // [TBX_PP_N * entries]
     const double *pp_mu;  // [entries] reduced exponents alone (screening counts)
     const int *ao_info;   // [nsp*TBX_MAXAO] shell | m << 4 | l << 8 of each spherical function
+    const int *ao_desc;   // [nsp] shells (bits 0-2) | l of shell s << (3 + 2 s)
     double intcut;
     double cutoff2;
 };
