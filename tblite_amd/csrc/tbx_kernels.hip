@@ -401,6 +401,16 @@ __device__ __forceinline__ void ao_decode(int desc, int ao, int &sh, int &m, int
     l = (desc >> (3 + 2 * sh)) & 3;
 }
 
+// first spherical AO of shell sh of a species with shell layout desc
+__device__ __forceinline__ int ao_first(int desc, int sh)
+{
+    int n = 0;
+#pragma unroll
+    for (int s = 0; s < MAXSH - 1; ++s)
+        if (s < sh) n += 2 * ((desc >> (3 + 2 * s)) & 3) + 1;
+    return n;
+}
+
 // inclusive prefix sum over the 64 lanes of a wavefront
 __device__ __forceinline__ int wave_scan(int v)
 {
@@ -417,10 +427,38 @@ __device__ __forceinline__ int wave_scan(int v)
 // list is sorted by mu, so the kept pairs are its leading entries.
 __device__ __forceinline__ int kept_pairs(const double *mu, int n, double r2, double intcut)
 {
-    int lo = 0, hi = n;  // invariant: entries < lo kept, entries >= hi dropped
-    while (lo < hi) {
+    if (n > 64) {
+        int lo = 0, hi = n;  // invariant: entries < lo kept, entries >= hi dropped
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (mu[mid] * r2 <= intcut) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    }
+    // two rounds of eight independent loads: the blocks of eight entries
+    // fully kept, then the kept entries of the first block that is not
+    int b = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int q = 8 * k + 7;
+        b += (q < n && mu[q] * r2 <= intcut) ? 1 : 0;
+    }
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int q = 8 * b + k;
+        c += (q < n && mu[q] * r2 <= intcut) ? 1 : 0;
+    }
+    return 8 * b + c;
+}
+
+// last index k in [0, n) with off[k] <= c (off non-decreasing, off[0] <= c)
+__device__ __forceinline__ int find_last_le(const int *off, int n, int c)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (mu[mid] * r2 <= intcut) lo = mid + 1; else hi = mid;
+        if (off[mid] <= c) lo = mid; else hi = mid;
     }
     return lo;
 }
@@ -458,9 +496,9 @@ __device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double
     __syncthreads();
     const int npair = U.sfirst[nseg];
     int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int di = 0, dj = 0;  // shell layouts of the pair's species (ao_desc)
     if (tid < npair) {
-        int k = 0;
-        while (k + 1 < nseg && U.sfirst[k + 1] <= tid) ++k;
+        const int k = find_last_le(U.sfirst, nseg, tid);
         const TbxSeg g = seg[s0 + k];
         PairInfo pr;
         pr.i = g.i;
@@ -477,9 +515,17 @@ __device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             U.pv[tid][c] = xyz[3 * pr.i + c] - xyz[3 * pr.j + c] - mt.trans[3 * g.itr + c];
+        di = pr.di;
+        dj = pr.dj;
+        // shells per angular momentum, from the packed layouts (no table loads)
         int nli[3] = {0, 0, 0}, nlj[3] = {0, 0, 0};
-        for (int q = 0; q < sp.nsh[pr.spi]; ++q) ++nli[sp.ang[q + sp.msh * pr.spi]];
-        for (int q = 0; q < sp.nsh[pr.spj]; ++q) ++nlj[sp.ang[q + sp.msh * pr.spj]];
+#pragma unroll
+        for (int q = 0; q < MAXSH; ++q)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                nli[l] += (q < (di & 7) && ((di >> (3 + 2 * q)) & 3) == l) ? 1 : 0;
+                nlj[l] += (q < (dj & 7) && ((dj >> (3 + 2 * q)) & 3) == l) ? 1 : 0;
+            }
 #pragma unroll
         for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
     }
@@ -505,15 +551,17 @@ __device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double
     }
     __syncthreads();
     if (tid < npair) {
-        const int spi = U.pinf[tid].spi, spj = U.pinf[tid].spj;
+#pragma unroll
         for (int t = 0; t < 9; ++t) {
             if (cnt[t] == 0) continue;
             int off = U.tsp[t] + incl[t] - cnt[t];
             for (int w = 0; w < wv; ++w) off += U.wsum[w][t];
-            for (int qi = 0; qi < sp.nsh[spi]; ++qi) {
-                if (sp.ang[qi + sp.msh * spi] != t / 3) continue;
-                for (int qj = 0; qj < sp.nsh[spj]; ++qj) {
-                    if (sp.ang[qj + sp.msh * spj] != t % 3) continue;
+#pragma unroll
+            for (int qi = 0; qi < MAXSH; ++qi) {
+                if (qi >= (di & 7) || ((di >> (3 + 2 * qi)) & 3) != t / 3) continue;
+#pragma unroll
+                for (int qj = 0; qj < MAXSH; ++qj) {
+                    if (qj >= (dj & 7) || ((dj >> (3 + 2 * qj)) & 3) != t % 3) continue;
                     if (spidx) spidx[tid * MAXSH * MAXSH + qi * MAXSH + qj] = (unsigned char)off;
                     U.spl[off++] = tid | (qj << 8) | (qi << 12) | (t << 16);
                 }
@@ -588,16 +636,6 @@ __device__ __forceinline__ int block_scan(int v, int *out, int (*wsum)[9])
     return out[NTHREADS];
 }
 
-// last index k in [0, n) with off[k] <= c (off non-decreasing, off[0] <= c)
-__device__ __forceinline__ int find_last_le(const int *off, int n, int c)
-{
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= c) lo = mid; else hi = mid;
-    }
-    return lo;
-}
 
 // ---------------------------------------------------------------------------
 // Forward: dense S, D, Q, H0 over work units of the half list (j <= i).
@@ -882,12 +920,10 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
                            int jsh, int li, int lj, int ki, int kj, int nao, long long dense0,
                            int ao_m, double hs, double sesum, double dsr, double *rec, int ld)
 {
-    const TbxSpecies &sp = A.sp;
-    const int i = pr.i, j = pr.j, spi = pr.spi, spj = pr.spj;
-    const int sloti = ish + sp.msh * spi, slotj = jsh + sp.msh * spj;
+    const int i = pr.i, j = pr.j;
     const double vx = v[0], vy = v[1], vz = v[2];
-    const int muj0 = pr.aoj + sp.sh_ao[slotj];
-    const int nui0 = pr.aoi + sp.sh_ao[sloti];
+    const int muj0 = pr.aoj + ao_first(pr.dj, jsh);
+    const int nui0 = pr.aoi + ao_first(pr.di, ish);
     const int na = cart_n(lj, kj), nb = cart_n(li, ki);
     double wS = 0.0, wH = 0.0;
     double wDi[3] = {0, 0, 0}, wDj[3] = {0, 0, 0};
@@ -979,8 +1015,6 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
 #pragma unroll
     for (int k = 0; k < 6; ++k) rec[(R_WQ + k) * ld] = wQj[k];
     rec[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
-    (void)sloti;
-    (void)slotj;
 }
 
 
@@ -1269,20 +1303,20 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
             atomicAdd(&sig[4], fs * (vy * G2 + G1 * vz));
             atomicAdd(&sig[5], fs * 2.0 * vz * G2);
         }
-        for (int q = 0; q < sp.nsh[pr.spj]; ++q)
+        for (int q = 0; q < (pr.dj & 7); ++q)
             if (accP[tid][3 + MAXSH + q] != 0.0)
                 unsafeAtomicAdd(&A.dse[pr.shj + q], accP[tid][3 + MAXSH + q]);
-        for (int q = 0; q < sp.nsh[pr.spi]; ++q)
+        for (int q = 0; q < (pr.di & 7); ++q)
             if (accP[tid][3 + q] != 0.0) atomicAdd(&accS[k][3 + q], accP[tid][3 + q]);
     }
     __syncthreads();
     if (tid < nseg) {
-        const int i = mt.seg[s0 + tid].i;
+        const PairInfo &first = U.pinf[U.sfirst[tid]];
+        const int i = first.i;
         for (int q = 0; q < 3; ++q)
             if (accS[tid][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * i + q], accS[tid][q]);
-        const int nshi = sp.nsh[mt.at_sp[i]];
-        for (int q = 0; q < nshi; ++q)
-            if (accS[tid][3 + q] != 0.0) unsafeAtomicAdd(&A.dse[mt.at_sh[i] + q], accS[tid][3 + q]);
+        for (int q = 0; q < (first.di & 7); ++q)
+            if (accS[tid][3 + q] != 0.0) unsafeAtomicAdd(&A.dse[first.shi + q], accS[tid][3 + q]);
     } else if (tid >= 64 && tid < 64 + 9) {
         const int k9 = tid - 64;
         const double v = sig[kSigPack[k9]];
