@@ -463,6 +463,26 @@ __device__ __forceinline__ int find_last_le(const int *off, int n, int c)
     return lo;
 }
 
+// Work item -> task: last index k in [0, NTHREADS) with off[k] <= c (off
+// non-decreasing, off[0] = 0 <= c), in two rounds of independent LDS reads
+// (the block of 16 entries, then the entry within it) instead of eight
+// dependent ones.  off must be 16-byte aligned.
+__device__ __forceinline__ int find_task(const int *off, int c)
+{
+    static_assert(NTHREADS == 256, "two-level search over 16 x 16 entries");
+    int b = 0;
+#pragma unroll
+    for (int k = 1; k < 16; ++k) b += (off[16 * k] <= c) ? 1 : 0;
+    const int4 *blk = reinterpret_cast<const int4 *>(off + 16 * b);
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int4 v = blk[k];
+        t += (v.x <= c ? 1 : 0) + (v.y <= c ? 1 : 0) + (v.z <= c ? 1 : 0) + (v.w <= c ? 1 : 0);
+    }
+    return 16 * b + t - 1;
+}
+
 // Unit staging shared by the forward and gradient kernels (LDS)
 struct UnitStage {
     PairInfo pinf[TBX_UNIT_PAIRS];
@@ -722,8 +742,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
     __shared__ unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
-    __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12
-    __shared__ int coff[TBX_UNIT_TASKS + 1];
+    __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12 | pair << 20
+    __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];
     __shared__ int eoff[2 * TBX_UNIT_SEGS + 1];  // element offsets: A blocks, then mirrors
     __shared__ int erow[TBX_UNIT_SEGS][3];       // first row, rows of block A, AOs of atom i
     const TbxSpecies &sp = A.sp;
@@ -757,7 +777,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     if (tid < ntask) {
         int spx;
         const int4 d = decode_task(U, tid, spx);
-        tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (spx << 12);
+        tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (spx << 12) | ((U.spl[spx] & 255) << 20);
         nch = (spk[spx] + chunk - 1) / chunk;
     }
     // (3) chunk offsets
@@ -768,8 +788,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         const int tk = find_last_le(coff, NTHREADS, c);
         const int info = tinfo[tk];
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
-        const int spx = (info >> 12) & 4095;
-        const int p = U.spl[spx] & 255;
+        const int spx = (info >> 12) & 255;
+        const int p = info >> 20;
         const int q = c - coff[tk];
         const int p0 = spp[spx] + q * chunk;
         const int p1 = spp[spx] + min(spk[spx], (q + 1) * chunk);
@@ -1146,6 +1166,77 @@ __device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, con
     }
 }
 
+// Primitive pairs [p0, p1) of one row ki = row of a p-p shell pair: as
+// shellpair_run, with the row a run-time value so that the rows of a
+// wavefront share one code path (the exponent of direction k is k == row).
+__device__ __forceinline__ void pprow_run(const double *__restrict__ pp, const double *rec,
+                                          int ld, const double vec[3], int row, int p0, int p1,
+                                          double g[3], double &ds)
+{
+    constexpr int NA = 5, NB = 2;
+    const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    for (int p = p0; p < p1; ++p) {
+        const double *e = pp + TBX_PP_N * p;
+        const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
+        const double h = e[TBX_PP_H], ai = e[TBX_PP_AI], pre = e[TBX_PP_PRE];
+        const double cc = exp(-(mu * r2)) * pre;
+        const double ta = 2.0 * ai;
+        // per direction: moments m = 0..2 and derivatives for exponent a = 0
+        // or 1 of the bra and b = 0 or 1 of the ket, the bra's chosen by row
+        double V0[3][2][3], G0[3][2][3];  // [direction][b][m], bra exponent 0 / 1 merged
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double O[NA + NB - 1][NB];
+            os_table<NA, NB>(-vec[k] * ajo, vec[k] * aio, h, O);
+            const bool up = (k == row);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const double v0 = O[m][b], v1 = O[m + 1][b];
+                    const double g0 = (m > 0) ? ta * O[m + 1][b] - (double)m * O[m - 1][b]
+                                              : ta * O[1][b];
+                    const double g1 = ta * O[m + 2][b] - (double)(m + 1) * O[m][b];
+                    V0[k][b][m] = up ? v1 : v0;
+                    G0[k][b][m] = up ? g1 : g0;
+                }
+        }
+#pragma unroll
+        for (int kj = 0; kj < 3; ++kj) {
+            __builtin_amdgcn_sched_barrier(0);
+            const double *r = rec + kj;
+            double V[3][3], G[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    V[k][m] = V0[k][k == kj ? 1 : 0][m];
+                    G[k][m] = G0[k][k == kj ? 1 : 0][m];
+                }
+            double w[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
+            const double S = V[0][0] * V[1][0] * V[2][0];
+            const double D0 = V[0][1] * V[1][0] * V[2][0];
+            const double D1 = V[0][0] * V[1][1] * V[2][0];
+            const double D2 = V[0][0] * V[1][0] * V[2][1];
+            const double *wq = r + R_WQ * ld;
+            const double wq0 = wq[0], wq1 = wq[ld], wq2 = wq[2 * ld], wq3 = wq[3 * ld],
+                         wq4 = wq[4 * ld], wq5 = wq[5 * ld];
+            const double WD0 = wq0 * D0 + 0.5 * (wq1 * D1 + wq3 * D2);
+            const double WD1 = wq2 * D1 + 0.5 * (wq1 * D0 + wq4 * D2);
+            const double WD2 = wq5 * D2 + 0.5 * (wq3 * D0 + wq4 * D1);
+            g[0] += cc * (contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4) +
+                          S * r[R_EX * ld] + 2.0 * WD0);
+            g[1] += cc * (contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4) +
+                          S * r[(R_EX + 1) * ld] + 2.0 * WD1);
+            g[2] += cc * (contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4) +
+                          S * r[(R_EX + 2) * ld] + 2.0 * WD2);
+            ds += cc * S * r[R_HSW * ld];
+        }
+    }
+}
+
 // One workgroup per work unit (up to TBX_UNIT_PAIRS atom pairs of one
 // molecule, at most TBX_UNIT_TASKS Cartesian-pair tasks).  (1) The unit's
 // pairs are staged in LDS and their shell pairs laid out by type (l_i, l_j);
@@ -1166,7 +1257,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     __shared__ int tinfo[TBX_UNIT_TASKS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
     __shared__ int tpp[TBX_UNIT_TASKS];    // first primitive pair of the task
     __shared__ int tkept[TBX_UNIT_TASKS];
-    __shared__ int coff[TBX_UNIT_TASKS + 1];  // first chunk of each task
+    __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];  // first chunk of each task
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
     __shared__ double sph[3][TBX_UNIT_TASKS];  // hs, sesum, dsr per shell pair
     const TbxSpecies &sp = A.sp;
@@ -1227,7 +1318,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
 
     // (4) work items
     for (int c = tid; c < nchunk; c += NTHREADS) {
-        const int tk = find_last_le(coff, NTHREADS, c);
+        const int tk = find_task(coff, c);
         const int info = tinfo[tk];
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
         const int p = (info >> 12) & 4095;
@@ -1244,11 +1335,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
             case 0: shellpair_run<0, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
             case 1: shellpair_run<0, 1, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
             case 3: shellpair_run<1, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
-            default:  // p-p by rows
-                if (ki == 0) shellpair_run<1, 1, 0>(sp.pp, r, L, vec, p0, p1, g, ds);
-                else if (ki == 1) shellpair_run<1, 1, 1>(sp.pp, r, L, vec, p0, p1, g, ds);
-                else shellpair_run<1, 1, 2>(sp.pp, r, L, vec, p0, p1, g, ds);
-                break;
+            default: pprow_run(sp.pp, r, L, vec, ki, p0, p1, g, ds); break;  // p-p by rows
             }
         } else {
             double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
