@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -245,9 +246,10 @@ struct HostMols {
     int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, sh_off, unit_seg, funit_seg;
+        cart_atom, ao_atom, sh_off, unit_seg, funit_seg, uplan, fplan;
     std::vector<long long> mol_dense;
     std::vector<TbxSeg> useg, fseg;
+    std::vector<TbxUnitHead> uhead, fhead;
 };
 
 // Work units (see TbxSeg): the half list of every molecule, cut into units of
@@ -292,12 +294,124 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
     }
 }
 
-static void build_units(const HostSpecies &hs, HostMols &hm, int ntrans, const tbx_tiles &t)
+// Plans of a list of units (layout: TbxUnitHead).  The units respect the
+// caps of TBX_UNIT_TASKS, TBX_UNIT_PAIRS and TBX_UNIT_SEGS by construction.
+// Returns false if a unit breaks the caps or its plan does not have the
+// length the kernels read (load_unit).
+static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::vector<TbxSeg> &seg,
+                        const std::vector<int> &unit_seg, bool with_spidx,
+                        std::vector<TbxUnitHead> &head, std::vector<int> &plan)
+{
+    const int nunit = (int)unit_seg.size() - 1;
+    head.assign(nunit, TbxUnitHead{});
+    plan.clear();
+    std::vector<TbxPair> pairs;
+    std::vector<int> pext, sinfo, spl;
+    std::vector<unsigned char> spidx;
+    auto pad4 = [&plan]() {
+        while (plan.size() % 4) plan.push_back(0);
+    };
+    for (int u = 0; u < nunit; ++u) {
+        const int s0 = unit_seg[u], s1 = unit_seg[u + 1];
+        pairs.clear();
+        pext.clear();
+        sinfo.clear();
+        for (int k = s0; k < s1; ++k) {
+            const TbxSeg &g = seg[k];
+            const int first = (int)pairs.size();
+            int rows = 0, mrows = 0;
+            for (int j = g.j0; j < g.j1; ++j) {
+                TbxPair p;
+                p.i = g.i;
+                p.j = j;
+                p.spi = hm.at_sp[g.i];
+                p.spj = hm.at_sp[j];
+                p.aoi = hm.at_ao[g.i];
+                p.aoj = hm.at_ao[j];
+                p.shi = hm.at_sh[g.i];
+                p.shj = hm.at_sh[j];
+                p.di = hs.ao_desc[p.spi];
+                p.dj = hs.ao_desc[p.spj];
+                pairs.push_back(p);
+                pext.push_back((g.itr << 5) | (k - s0));
+                rows += hs.nao[p.spj];
+                if (j < g.i) mrows += hs.nao[p.spj];  // mirrored blocks: j < i
+            }
+            const int spi = hm.at_sp[g.i];
+            sinfo.insert(sinfo.end(), {first, hm.at_ao[g.j0], rows | (mrows << 16), hs.nao[spi]});
+        }
+        const int npair = (int)pairs.size(), nseg = s1 - s0;
+        // shell pairs, type-major; within a type by pair, then ish, then jsh
+        spl.clear();
+        spidx.assign((size_t)16 * npair, 0);
+        int toff[10], tsp[10], ntask = 0;
+        for (int t = 0; t < 9; ++t) {
+            toff[t] = ntask;
+            tsp[t] = (int)spl.size();
+            const int li = t / 3, lj = t % 3;
+            for (int p = 0; p < npair; ++p) {
+                const TbxPair &pr = pairs[p];
+                for (int qi = 0; qi < hs.nsh[pr.spi]; ++qi) {
+                    if (hs.ang[qi + hs.msh * pr.spi] != li) continue;
+                    for (int qj = 0; qj < hs.nsh[pr.spj]; ++qj) {
+                        if (hs.ang[qj + hs.msh * pr.spj] != lj) continue;
+                        spidx[(size_t)16 * p + 4 * qi + qj] = (unsigned char)spl.size();
+                        spl.push_back(p | (qj << 8) | (qi << 12) | (t << 16));
+                        ntask += (li + 1) * (li + 2) / 2 * ((lj + 1) * (lj + 2) / 2);
+                    }
+                }
+            }
+        }
+        toff[9] = ntask;
+        tsp[9] = (int)spl.size();
+        const int m = hm.at_mol[seg[s0].i];
+        TbxUnitHead &h = head[u];
+        h.dense0 = hm.mol_dense[m];
+        h.plan = (int)(plan.size() / 4);
+        h.counts = npair | (nseg << 8) | ((int)spl.size() << 16);
+        h.nao = hm.mol_nao[m];
+        h.ao_m = hm.mol_ao[m];
+        h.mol = m;
+        for (const TbxPair &pr : pairs)
+            plan.insert(plan.end(), {pr.i, pr.j, pr.spi, pr.spj, pr.aoi, pr.aoj, pr.shi, pr.shj,
+                                     pr.di, pr.dj});
+        pad4();
+        plan.insert(plan.end(), pext.begin(), pext.end());
+        pad4();
+        plan.insert(plan.end(), sinfo.begin(), sinfo.end());
+        plan.insert(plan.end(), spl.begin(), spl.end());
+        pad4();
+        plan.insert(plan.end(), toff, toff + 10);
+        plan.insert(plan.end(), tsp, tsp + 10);
+        if (with_spidx)
+            for (int p = 0; p < npair; ++p)
+                for (int w = 0; w < 4; ++w) {
+                    const unsigned char *b = &spidx[(size_t)16 * p + 4 * w];
+                    plan.push_back((int)((unsigned)b[0] | ((unsigned)b[1] << 8) |
+                                         ((unsigned)b[2] << 16) | ((unsigned)b[3] << 24)));
+                }
+        const int nspair = (int)spl.size();
+        const long long words = (10 * npair + 3) / 4 + (npair + 3) / 4 + nseg + (nspair + 3) / 4 +
+                                5 + (with_spidx ? npair : 0);
+        if (npair < 1 || npair > TBX_UNIT_PAIRS || nseg > TBX_UNIT_SEGS ||
+            nspair > TBX_UNIT_TASKS || ntask > TBX_UNIT_TASKS ||
+            (long long)plan.size() != 4 * ((long long)h.plan + words) ||
+            plan.size() / 4 > (size_t)INT_MAX)
+            return false;
+    }
+    return true;
+}
+
+static int build_units(const HostSpecies &hs, HostMols &hm, int ntrans, const tbx_tiles &t)
 {
     build_units_list(hs, hm, ntrans, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg,
                      hm.unit_seg);
     build_units_list(hs, hm, ntrans, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg,
                      hm.funit_seg);
+    if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, hm.uhead, hm.uplan) ||
+        !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, hm.fhead, hm.fplan))
+        return fail(TBX_ERR_ARG, "work-unit plan outside the kernel limits");
+    return TBX_OK;
 }
 
 // defaults: the autotuner's choice on the 2048-molecule batch
@@ -382,7 +496,7 @@ struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
         ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off, pp_mu, ao_info, ao_desc;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, mol_ao, unit_seg, useg, funit_seg, fseg;
+        cart_atom, ao_atom, mol_ao, uhead, uplan, fhead, fplan;
     TbxSpecies sp{};
     TbxMols mt{};
 
@@ -393,8 +507,8 @@ struct DevTables {
                          &pp, &pp_off, &pp_mu, &ao_info, &ao_desc,
                          &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
-                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &unit_seg, &useg,
-                         &funit_seg, &fseg};
+                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &uhead, &uplan,
+                         &fhead, &fplan};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -446,8 +560,8 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     hipError_t e;
 #define UP(name) if ((e = upload(d.name, hm.name, s)) != hipSuccess) return e
     UP(mol_atom); UP(mol_nao); UP(mol_cart); UP(mol_dense); UP(at_sp); UP(at_mol); UP(at_sh);
-    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(unit_seg);
-    UP(useg); UP(funit_seg); UP(fseg);
+    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(uhead); UP(uplan);
+    UP(fhead); UP(fplan);
 #undef UP
     if ((e = upload(d.trans, trans, s)) != hipSuccess) return e;
     TbxMols &mt = d.mt;
@@ -467,12 +581,12 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     mt.cart_atom = d.cart_atom.as<int>();
     mt.ao_atom = d.ao_atom.as<int>();
     mt.mol_ao = d.mol_ao.as<int>();
-    mt.nunit = (int)hm.unit_seg.size() - 1;
-    mt.unit_seg = d.unit_seg.as<int>();
-    mt.seg = d.useg.as<TbxSeg>();
-    mt.nfunit = (int)hm.funit_seg.size() - 1;
-    mt.funit_seg = d.funit_seg.as<int>();
-    mt.fseg = d.fseg.as<TbxSeg>();
+    mt.nunit = (int)hm.uhead.size();
+    mt.unit = d.uhead.as<TbxUnitHead>();
+    mt.plan = d.uplan.as<int>();
+    mt.nfunit = (int)hm.fhead.size();
+    mt.funit = d.fhead.as<TbxUnitHead>();
+    mt.fplan = d.fplan.as<int>();
     return hipSuccess;
 }
 
@@ -591,7 +705,7 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
     hipStream_t s = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
-    build_units(hs, hm, geom->ntrans, kDefaultTiles);
+    if ((st = build_units(hs, hm, geom->ntrans, kDefaultTiles))) return st;
     HIP_TRY(upload_mols(ctx->tab, hm, trans, s));
     HIP_TRY(ctx->xyz.reserve(sizeof(double) * 3 * geom->nat));
     HIP_TRY(hipMemcpyAsync(ctx->xyz.p, geom->xyz, sizeof(double) * 3 * geom->nat,
@@ -728,7 +842,10 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     hipError_t e = hipSetDevice(ctx->device);
     std::vector<double> trans(3, 0.0);
     if (e == hipSuccess) e = upload_species(b->tab, b->hs, desc->cutoff, ctx->stream);
-    build_units(b->hs, b->hm, 1, b->tiles);
+    if ((st = build_units(b->hs, b->hm, 1, b->tiles))) {
+        tbx_batch_delete(b);
+        return st;
+    }
     if (e == hipSuccess) e = upload_mols(b->tab, b->hm, trans, ctx->stream);
     if (e == hipSuccess) e = b->flag.reserve(sizeof(int));
     // scratch of the gradient / adjoint launches: dE/dse (plain) or sigma (adjoint)
@@ -749,8 +866,11 @@ extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
 {
     if (!b || !tiles) return fail(TBX_ERR_ARG, "null argument");
     if (int st = check_tiles(b->hs, tiles)) return st;
+    if (int st = build_units(b->hs, b->hm, 1, *tiles)) {
+        (void)build_units(b->hs, b->hm, 1, b->tiles);  // keep the host copy of the current units
+        return st;
+    }
     b->tiles = *tiles;
-    build_units(b->hs, b->hm, 1, b->tiles);
     std::vector<double> trans(3, 0.0);
     HIP_TRY(hipSetDevice(b->ctx->device));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));  // no launch may still read the old units

@@ -24,6 +24,8 @@
 // All arithmetic is FP64.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "tbx_tables.h"
 
 #define MAXG 12
@@ -372,13 +374,7 @@ enum {
 };
 
 // Atom pair of a unit, staged in LDS
-struct PairInfo {
-    int i, j;
-    int spi, spj;
-    int aoi, aoj;  // first AO of the atoms, molecule-local
-    int shi, shj;  // first shell of the atoms (global)
-    int di, dj;    // shell layout of the species (TbxSpecies::ao_desc)
-};
+using PairInfo = TbxPair;
 
 // shell, m and l of spherical function ao of a species with shell layout desc
 __device__ __forceinline__ void ao_decode(int desc, int ao, int &sh, int &m, int &l)
@@ -483,16 +479,24 @@ __device__ __forceinline__ int find_task(const int *off, int c)
     return 16 * b + t - 1;
 }
 
-// Unit staging shared by the forward and gradient kernels (LDS)
-struct UnitStage {
+// Unit staging shared by the forward and gradient kernels (LDS): the unit's
+// plan (TbxUnitHead) and the pair vectors.
+struct __align__(16) UnitStage {
     PairInfo pinf[TBX_UNIT_PAIRS];
-    double pv[TBX_UNIT_PAIRS][3];  // v = R_i - R_j - T
-    int sfirst[TBX_UNIT_SEGS + 1]; // first pair of each segment
-    int spl[TBX_UNIT_TASKS];       // shell pairs: pair | jsh << 8 | ish << 12 | type << 16
-                                   // (type-major)
+    int pext[TBX_UNIT_PAIRS];       // lattice vector << 5 | segment
+    int4 sinfo[TBX_UNIT_SEGS];      // first pair, first row AO, rows | mirrored rows << 16,
+                                    // AOs of atom i
+    int spl[TBX_UNIT_TASKS];        // shell pairs: pair | jsh << 8 | ish << 12 | type << 16
+                                    // (type-major)
+    int toff[10], tsp[10];          // task / shell-pair offsets per type
+    double pv[TBX_UNIT_PAIRS][3];   // v = R_i - R_j - T
     int wsum[NTHREADS / 64][9];
-    int toff[10], tsp[10];         // task / shell-pair offsets per type
 };
+static_assert(sizeof(PairInfo) == 40, "pair record layout");
+static_assert(offsetof(UnitStage, pext) % 16 == 0 && offsetof(UnitStage, sinfo) % 16 == 0 &&
+                  offsetof(UnitStage, spl) % 16 == 0 && offsetof(UnitStage, toff) % 16 == 0 &&
+                  offsetof(UnitStage, tsp) == offsetof(UnitStage, toff) + 40,
+              "plan sections are copied as 16-byte words");
 
 // (pair, jsh, ish, type) of a packed shell-pair entry
 __device__ __forceinline__ int4 unpack_spl(int v)
@@ -500,95 +504,40 @@ __device__ __forceinline__ int4 unpack_spl(int v)
     return make_int4(v & 255, (v >> 8) & 15, (v >> 12) & 15, v >> 16);
 }
 
-// Stage the pairs of unit segments [s0, s0 + nseg) and lay out their shell
-// pairs by type.  spidx (optional): shell-pair index of (pair, ish, jsh).
-__device__ void stage_unit(const TbxSpecies &sp, const TbxMols &mt, const double *xyz,
-                           const TbxSeg *seg, int s0, int nseg, UnitStage &U,
-                           unsigned char *spidx)
+// Copy a unit's plan (layout: TbxUnitHead) into LDS, one 16-byte word per
+// lane and step.  spidx (forward units): shell-pair index of (pair, ish, jsh).
+__device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan, UnitStage &U,
+                                          unsigned char *spidx)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (wv == 0) {  // nseg <= TBX_UNIT_SEGS <= 64: one wavefront scans the segment lengths
-        const int len = (lane < nseg) ? seg[s0 + lane].j1 - seg[s0 + lane].j0 : 0;
-        const int inc = wave_scan(len);
-        if (lane < nseg) U.sfirst[lane] = inc - len;
-        if (lane == 63) U.sfirst[nseg] = inc;
+    const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
+    const int4 *src = reinterpret_cast<const int4 *>(plan) + H.plan;
+    const int e1 = (10 * npair + 3) >> 2;
+    const int e2 = e1 + ((npair + 3) >> 2);
+    const int e3 = e2 + nseg;
+    const int e4 = e3 + ((nspair + 3) >> 2);
+    const int e5 = e4 + 5;
+    const int total = e5 + (spidx ? npair : 0);
+    int4 *base = reinterpret_cast<int4 *>(&U);
+    for (int k = threadIdx.x; k < total; k += NTHREADS) {
+        const int4 v = src[k];
+        int w;  // destination word within U
+        if (k < e1) w = k;
+        else if (k < e2) w = (int)(offsetof(UnitStage, pext) / 16) + (k - e1);
+        else if (k < e3) w = (int)(offsetof(UnitStage, sinfo) / 16) + (k - e2);
+        else if (k < e4) w = (int)(offsetof(UnitStage, spl) / 16) + (k - e3);
+        else w = (int)(offsetof(UnitStage, toff) / 16) + (k - e4);
+        if (k < e5) base[w] = v;
+        else reinterpret_cast<int4 *>(spidx)[k - e5] = v;
     }
-    __syncthreads();
-    const int npair = U.sfirst[nseg];
-    int cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int di = 0, dj = 0;  // shell layouts of the pair's species (ao_desc)
-    if (tid < npair) {
-        const int k = find_last_le(U.sfirst, nseg, tid);
-        const TbxSeg g = seg[s0 + k];
-        PairInfo pr;
-        pr.i = g.i;
-        pr.j = g.j0 + tid - U.sfirst[k];
-        pr.spi = mt.at_sp[pr.i];
-        pr.spj = mt.at_sp[pr.j];
-        pr.aoi = mt.at_ao[pr.i];
-        pr.aoj = mt.at_ao[pr.j];
-        pr.shi = mt.at_sh[pr.i];
-        pr.shj = mt.at_sh[pr.j];
-        pr.di = sp.ao_desc[pr.spi];
-        pr.dj = sp.ao_desc[pr.spj];
-        U.pinf[tid] = pr;
+}
+
+// v = R_i - R_j - T of a pair
+__device__ __forceinline__ void pair_vec(const TbxMols &mt, const double *xyz, const PairInfo &pr,
+                                         int pext, double v[3])
+{
+    const int itr = pext >> 5;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            U.pv[tid][c] = xyz[3 * pr.i + c] - xyz[3 * pr.j + c] - mt.trans[3 * g.itr + c];
-        di = pr.di;
-        dj = pr.dj;
-        // shells per angular momentum, from the packed layouts (no table loads)
-        int nli[3] = {0, 0, 0}, nlj[3] = {0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < MAXSH; ++q)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-                nli[l] += (q < (di & 7) && ((di >> (3 + 2 * q)) & 3) == l) ? 1 : 0;
-                nlj[l] += (q < (dj & 7) && ((dj >> (3 + 2 * q)) & 3) == l) ? 1 : 0;
-            }
-#pragma unroll
-        for (int t = 0; t < 9; ++t) cnt[t] = nli[t / 3] * nlj[t % 3];
-    }
-    int incl[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        incl[t] = wave_scan(cnt[t]);
-        if (lane == 63) U.wsum[wv][t] = incl[t];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int so = 0, to = 0;
-        for (int t = 0; t < 9; ++t) {
-            int tot = 0;
-            for (int w = 0; w < NTHREADS / 64; ++w) tot += U.wsum[w][t];
-            U.tsp[t] = so;
-            U.toff[t] = to;
-            so += tot;
-            to += tot * ncart_of(t / 3) * ncart_of(t % 3);
-        }
-        U.tsp[9] = so;
-        U.toff[9] = to;
-    }
-    __syncthreads();
-    if (tid < npair) {
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            if (cnt[t] == 0) continue;
-            int off = U.tsp[t] + incl[t] - cnt[t];
-            for (int w = 0; w < wv; ++w) off += U.wsum[w][t];
-#pragma unroll
-            for (int qi = 0; qi < MAXSH; ++qi) {
-                if (qi >= (di & 7) || ((di >> (3 + 2 * qi)) & 3) != t / 3) continue;
-#pragma unroll
-                for (int qj = 0; qj < MAXSH; ++qj) {
-                    if (qj >= (dj & 7) || ((dj >> (3 + 2 * qj)) & 3) != t % 3) continue;
-                    if (spidx) spidx[tid * MAXSH * MAXSH + qi * MAXSH + qj] = (unsigned char)off;
-                    U.spl[off++] = tid | (qj << 8) | (qi << 12) | (t << 16);
-                }
-            }
-        }
-    }
-    __syncthreads();
+    for (int c = 0; c < 3; ++c) v[c] = xyz[3 * pr.i + c] - xyz[3 * pr.j + c] - mt.trans[3 * itr + c];
 }
 
 // Decode task tid of a staged unit: shell pair index, type and Cartesian pair.
@@ -608,7 +557,8 @@ __device__ __forceinline__ int4 decode_task(const UnitStage &U, int task, int &s
 // first entry of the sorted primitive-pair list, and the H0 factors
 // hs = hscale * shpoly(R), dsr = (d shpoly/dr) / (r shpoly), sesum =
 // (se_i + se_j)/2.  onsite: the j == i, T = 0 block counts (forward only).
-__device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const UnitStage &U,
+__device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const TbxMols &mt,
+                                                const double *xyz, const UnitStage &U,
                                                 const double *se, int k, bool onsite_ok,
                                                 int &kept, int &pi0, double &hs, double &sesum,
                                                 double &dsr)
@@ -616,8 +566,9 @@ __device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const Unit
     const int4 e = unpack_spl(U.spl[k]);
     const PairInfo pr = U.pinf[e.x];
     const int ish = e.z, jsh = e.y;
-    const double vx = U.pv[e.x][0], vy = U.pv[e.x][1], vz = U.pv[e.x][2];
-    const double r2 = vx * vx + vy * vy + vz * vz;
+    double v[3];
+    pair_vec(mt, xyz, pr, U.pext[e.x], v);  // the same values as U.pv[e.x]
+    const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     const int sloti = ish + sp.msh * pr.spi, slotj = jsh + sp.msh * pr.spj;
     pi0 = sp.pp_off[sloti * sp.nslot + slotj];
     kept = 0;
@@ -741,31 +692,36 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ double acc[10][TBX_UNIT_TASKS];
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
-    __shared__ unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
+    __shared__ __align__(16) unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12 | pair << 20
     __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];
     __shared__ int eoff[2 * TBX_UNIT_SEGS + 1];  // element offsets: A blocks, then mirrors
-    __shared__ int erow[TBX_UNIT_SEGS][3];       // first row, rows of block A, AOs of atom i
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
-    const int u = blockIdx.x;
-    const int s0 = mt.funit_seg[u], nseg = mt.funit_seg[u + 1] - s0;
-    const int m = mt.at_mol[mt.fseg[s0].i];
-    const int nao = mt.mol_nao[m];
-    const long long dense0 = mt.mol_dense[m];
-    const int ao_m = mt.mol_ao[m];
+    const TbxUnitHead H = mt.funit[blockIdx.x];
+    const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
+    const int nao = H.nao;
+    const long long dense0 = H.dense0;
     const bool accum = A.accumulate != 0;
     const int chunk = A.chunk;
     for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
-    stage_unit(sp, mt, A.xyz, mt.fseg, s0, nseg, U, spidx);
+    // (1) the unit's plan
+    load_unit(H, mt.fplan, U, spidx);
+    __syncthreads();
 
-    // (2) shell pairs: screening counts and H0 factors (one lane per shell pair)
-    const int nspair = U.tsp[9];
+    // (2) pair vectors; shell pairs: screening counts and H0 factors (one
+    // lane per shell pair)
+    if (tid < npair) {
+        double v[3];
+        pair_vec(mt, A.xyz, U.pinf[tid], U.pext[tid], v);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U.pv[tid][c] = v[c];
+    }
     for (int k = tid; k < nspair; k += NTHREADS) {
         int kept, pi0;
         double hs, sesum, dsr;
-        shellpair_setup(sp, U, A.se, k, true, kept, pi0, hs, sesum, dsr);
+        shellpair_setup(sp, mt, A.xyz, U, A.se, k, true, kept, pi0, hs, sesum, dsr);
         spk[k] = kept;
         spp[k] = pi0;
         hfac[k] = sesum * hs;
@@ -815,23 +771,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     if (tid < 64) {  // one wavefront; nseg <= TBX_UNIT_SEGS <= 64
         int na = 0, nb = 0;
         if (tid < nseg) {
-            const TbxSeg g = mt.fseg[s0 + tid];
-            const PairInfo &first = U.pinf[U.sfirst[tid]];
-            const PairInfo &last = U.pinf[U.sfirst[tid + 1] - 1];
-            const int r0 = first.aoj;
-            const int r1 = last.aoj + sp.nao[last.spj];
-            const int jb = min(g.j1, g.i);  // mirrored atoms: j < i
-            int rb1 = r0;
-            if (jb > g.j0) {
-                const PairInfo &lb = U.pinf[U.sfirst[tid] + (jb - 1 - g.j0)];
-                rb1 = lb.aoj + sp.nao[lb.spj];
-            }
-            const int naoi = sp.nao[first.spi];
-            erow[tid][0] = r0;
-            erow[tid][1] = r1 - r0;
-            erow[tid][2] = naoi;
-            na = naoi * (r1 - r0);
-            nb = naoi * (rb1 - r0);
+            const int4 g = U.sinfo[tid];
+            na = g.w * (g.z & 0xffff);
+            nb = g.w * (g.z >> 16);
         }
         const int ia = wave_scan(na), ib = wave_scan(nb);
         const int ta = __shfl(ia, 63, 64);
@@ -850,19 +792,20 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
             const bool mirror = kk >= nseg;
             const int k = mirror ? kk - nseg : kk;
             const int local = e - eoff[kk];
-            const int naoi = erow[k][2];
+            const int4 g = U.sinfo[k];
+            const int naoi = g.w, rows = g.z & 0xffff;
             int mu, nul;
             if (!mirror) {
-                nul = local / erow[k][1];
-                mu = erow[k][0] + local - nul * erow[k][1];
+                nul = local / rows;
+                mu = g.y + local - nul * rows;
             } else {
-                mu = erow[k][0] + local / naoi;
+                mu = g.y + local / naoi;
                 nul = local - (local / naoi) * naoi;
             }
             // pair of row mu: the pairs of a segment hold consecutive atoms j
-            int p = U.sfirst[k];
+            int p = g.x;
             {
-                int hi = U.sfirst[k + 1];
+                int hi = (k + 1 < nseg) ? U.sinfo[k + 1].x : npair;
                 while (hi - p > 1) {
                     const int mid = (p + hi) >> 1;
                     if (U.pinf[mid].aoj <= mu) p = mid; else hi = mid;
@@ -1263,26 +1206,32 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
-    const int u = blockIdx.x;
-    const int s0 = mt.unit_seg[u], nseg = mt.unit_seg[u + 1] - s0;
-    const int m = mt.at_mol[mt.seg[s0].i];
-    const int nao = mt.mol_nao[m];
-    const long long dense0 = mt.mol_dense[m];
-    const int ao_m = mt.mol_ao[m];
+    const TbxUnitHead H = mt.unit[blockIdx.x];
+    const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
+    const int m = H.mol;
+    const int nao = H.nao;
+    const long long dense0 = H.dense0;
+    const int ao_m = H.ao_m;
     const int chunk = A.chunk;
     for (int k = tid; k < TBX_UNIT_PAIRS * (3 + 2 * MAXSH); k += NTHREADS) (&accP[0][0])[k] = 0.0;
     for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += NTHREADS) (&accS[0][0])[k] = 0.0;
     if (tid < 6) sig[tid] = 0.0;
-    // (1) stage the pairs; shell pairs per type, type-major
-    stage_unit(sp, mt, A.xyz, mt.seg, s0, nseg, U, nullptr);
-    const int npair = U.sfirst[nseg];
+    // (1) the unit's plan
+    load_unit(H, mt.plan, U, nullptr);
+    __syncthreads();
 
-    // (2) shell pairs: screening counts and H0 factors (one lane per shell pair)
-    const int nspair = U.tsp[9];
+    // (2) pair vectors; shell pairs: screening counts and H0 factors (one
+    // lane per shell pair)
+    if (tid < npair) {
+        double v[3];
+        pair_vec(mt, A.xyz, U.pinf[tid], U.pext[tid], v);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U.pv[tid][c] = v[c];
+    }
     for (int k = tid; k < nspair; k += NTHREADS) {
         int kept, pi0;
         double hs, sesum, dsr;
-        shellpair_setup(sp, U, A.se, k, false, kept, pi0, hs, sesum, dsr);
+        shellpair_setup(sp, mt, A.xyz, U, A.se, k, false, kept, pi0, hs, sesum, dsr);
         spk[k] = kept;
         spp[k] = pi0;
         sph[0][k] = hs;
@@ -1369,7 +1318,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     // (5) pair sums -> atom j (HBM), segment sums (atom i) and sigma (LDS)
     if (tid < npair) {
         const PairInfo pr = U.pinf[tid];
-        const int k = find_last_le(U.sfirst, nseg, tid);
+        const int k = U.pext[tid] & 31;
         const bool self = (pr.i == pr.j);
         const double G0 = accP[tid][0], G1 = accP[tid][1], G2 = accP[tid][2];
         const double vx = U.pv[tid][0], vy = U.pv[tid][1], vz = U.pv[tid][2];
@@ -1398,7 +1347,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     }
     __syncthreads();
     if (tid < nseg) {
-        const PairInfo &first = U.pinf[U.sfirst[tid]];
+        const PairInfo &first = U.pinf[U.sinfo[tid].x];
         const int i = first.i;
         for (int q = 0; q < 3; ++q)
             if (accS[tid][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * i + q], accS[tid][q]);

@@ -52,15 +52,45 @@ struct TbxSpecies {
     double cutoff2;
 };
 
-// Gradient work units: consecutive (i, j, image) atom pairs of one molecule,
-// j in [j0, j1) for row atom i at lattice vector itr, packed so that a unit
-// holds at most TBX_UNIT_TASKS Cartesian-pair tasks, TBX_UNIT_PAIRS atom pairs
-// and TBX_UNIT_SEGS segments.  One workgroup runs one unit.
+// Work units: consecutive (i, j, image) atom pairs of one molecule, in
+// segments j in [j0, j1) of one row atom i at lattice vector itr, packed so
+// that a unit holds at most TBX_UNIT_TASKS Cartesian-pair tasks,
+// TBX_UNIT_PAIRS atom pairs and TBX_UNIT_SEGS segments.  One workgroup runs
+// one unit.
 #define TBX_UNIT_TASKS 256
 #define TBX_UNIT_PAIRS 128
 #define TBX_UNIT_SEGS 32
 struct TbxSeg {
     int i, j0, j1, itr;
+};
+
+// One atom pair of a unit
+struct TbxPair {
+    int i, j;      // atoms (global)
+    int spi, spj;  // species
+    int aoi, aoj;  // first AO of the atoms, molecule-local
+    int shi, shj;  // first shell of the atoms (global)
+    int di, dj;    // shell layouts of the species (TbxSpecies::ao_desc)
+};
+
+// Unit header (32 bytes).  The unit's plan -- everything about it that does not depend
+// on the geometry -- is a run of 16-byte words at plan[16 * plan ...] in this
+// order, each section padded to whole words:
+//   pairs      npair TbxPair (40 bytes each)
+//   pext       npair ints: lattice vector << 5 | segment
+//   segments   nseg int4: first pair, first row AO, rows | mirrored rows << 16,
+//              AOs of the row atom i
+//   shell pairs nspair ints: pair | jsh << 8 | ish << 12 | type << 16, type-major
+//              (type = 3 l_i + l_j)
+//   offsets    toff[10] then tsp[10]: first task / shell pair of each type
+//   spidx      (forward units only) npair x 16 bytes: shell-pair index of
+//              (pair, ish, jsh) at byte 16 pair + 4 ish + jsh
+struct TbxUnitHead {
+    long long dense0;  // first dense element of the unit's molecule
+    int plan;          // first 16-byte word of the plan
+    int counts;        // npair | nseg << 8 | nspair << 16
+    int nao, ao_m;     // AOs of the molecule, its first AO (global)
+    int mol, pad;
 };
 
 // Molecule / atom tables of a batch (device pointers).  "local" indices
@@ -82,9 +112,9 @@ struct TbxMols {
     const int *ao_atom;            // [total AO] atom (global)
     const int *mol_ao;             // [nmol+1] first AO (global)
     int nunit;
-    const int *unit_seg;           // [nunit+1] first segment of each gradient unit
-    const TbxSeg *seg;             // gradient unit segments (j < i; j <= i with images)
+    const TbxUnitHead *unit;       // [nunit] gradient units (j < i; j <= i with images)
+    const int *plan;               // their plans (16-byte words)
     int nfunit;
-    const int *funit_seg;          // [nfunit+1] first segment of each forward unit
-    const TbxSeg *fseg;            // forward unit segments (j <= i, on-site pairs included)
+    const TbxUnitHead *funit;      // [nfunit] forward units (j <= i, on-site pairs included)
+    const int *fplan;
 };
