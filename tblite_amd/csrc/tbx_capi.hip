@@ -299,7 +299,7 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
 // Returns false if a unit breaks the caps or its plan does not have the
 // length the kernels read (load_unit).
 static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::vector<TbxSeg> &seg,
-                        const std::vector<int> &unit_seg, bool with_spidx,
+                        const std::vector<int> &unit_seg, bool with_elems,
                         std::vector<TbxUnitHead> &head, std::vector<int> &plan)
 {
     const int nunit = (int)unit_seg.size() - 1;
@@ -383,21 +383,47 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
         pad4();
         plan.insert(plan.end(), toff, toff + 10);
         plan.insert(plan.end(), tsp, tsp + 10);
-        if (with_spidx)
-            for (int p = 0; p < npair; ++p)
-                for (int w = 0; w < 4; ++w) {
-                    const unsigned char *b = &spidx[(size_t)16 * p + 4 * w];
-                    plan.push_back((int)((unsigned)b[0] | ((unsigned)b[1] << 8) |
-                                         ((unsigned)b[2] << 16) | ((unsigned)b[3] << 24)));
-                }
         const int nspair = (int)spl.size();
-        const long long words = (10 * npair + 3) / 4 + (npair + 3) / 4 + nseg + (nspair + 3) / 4 +
-                                5 + (with_spidx ? npair : 0);
+        const long long words = (10 * npair + 3) / 4 + (npair + 3) / 4 + nseg + (nspair + 3) / 4 + 5;
         if (npair < 1 || npair > TBX_UNIT_PAIRS || nseg > TBX_UNIT_SEGS ||
             nspair > TBX_UNIT_TASKS || ntask > TBX_UNIT_TASKS ||
-            (long long)plan.size() != 4 * ((long long)h.plan + words) ||
-            plan.size() / 4 > (size_t)INT_MAX)
+            (long long)plan.size() != 4 * ((long long)h.plan + words))
             return false;
+        h.nel = 0;
+        if (with_elems) {
+            // element descriptors: block A of every segment, then the mirrors
+            auto elem = [&](int p, int nul, int muj, bool mirror) {
+                const TbxPair &pr = pairs[p];
+                const int ish = hs.ao_sh[nul + TBX_MAXAO * pr.spi];
+                const int jsh = hs.ao_sh[muj + TBX_MAXAO * pr.spj];
+                const int li = hs.ang[ish + hs.msh * pr.spi], lj = hs.ang[jsh + hs.msh * pr.spj];
+                const int t = 3 * li + lj;
+                const int spx = spidx[(size_t)16 * p + 4 * ish + jsh];
+                const int task0 = toff[t] + (spx - tsp[t]) * ((li + 1) * (li + 2) / 2) *
+                                                ((lj + 1) * (lj + 2) / 2);
+                plan.push_back(p | (nul << 7) | (muj << 12) | (task0 << 17) |
+                               ((mirror ? 1 : 0) << 25));
+            };
+            for (int mirror = 0; mirror < 2; ++mirror)
+                for (int k = 0; k < nseg; ++k) {
+                    const int p0 = sinfo[4 * k], p1 = k + 1 < nseg ? sinfo[4 * (k + 1)] : npair;
+                    const int naoi = sinfo[4 * k + 3];
+                    if (!mirror) {
+                        for (int nul = 0; nul < naoi; ++nul)
+                            for (int p = p0; p < p1; ++p)
+                                for (int muj = 0; muj < hs.nao[pairs[p].spj]; ++muj)
+                                    elem(p, nul, muj, false);
+                    } else {
+                        for (int p = p0; p < p1 && pairs[p].j < pairs[p].i; ++p)
+                            for (int muj = 0; muj < hs.nao[pairs[p].spj]; ++muj)
+                                for (int nul = 0; nul < naoi; ++nul) elem(p, nul, muj, true);
+                    }
+                }
+            h.nel = (int)(plan.size() - 4 * ((size_t)h.plan + words));
+            if (h.nel > 2 * TBX_UNIT_TASKS) return false;  // two elements per lane
+            pad4();
+        }
+        if (plan.size() / 4 > (size_t)INT_MAX) return false;
     }
     return true;
 }

@@ -504,10 +504,16 @@ __device__ __forceinline__ int4 unpack_spl(int v)
     return make_int4(v & 255, (v >> 8) & 15, (v >> 12) & 15, v >> 16);
 }
 
+// Number of 16-byte words of a unit's plan staged in LDS
+__device__ __forceinline__ int plan_words(const TbxUnitHead &H)
+{
+    const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
+    return ((10 * npair + 3) >> 2) + ((npair + 3) >> 2) + nseg + ((nspair + 3) >> 2) + 5;
+}
+
 // Copy a unit's plan (layout: TbxUnitHead) into LDS, one 16-byte word per
-// lane and step.  spidx (forward units): shell-pair index of (pair, ish, jsh).
-__device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan, UnitStage &U,
-                                          unsigned char *spidx)
+// lane and step.
+__device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan, UnitStage &U)
 {
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
     const int4 *src = reinterpret_cast<const int4 *>(plan) + H.plan;
@@ -515,8 +521,7 @@ __device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan,
     const int e2 = e1 + ((npair + 3) >> 2);
     const int e3 = e2 + nseg;
     const int e4 = e3 + ((nspair + 3) >> 2);
-    const int e5 = e4 + 5;
-    const int total = e5 + (spidx ? npair : 0);
+    const int total = e4 + 5;
     int4 *base = reinterpret_cast<int4 *>(&U);
     for (int k = threadIdx.x; k < total; k += NTHREADS) {
         const int4 v = src[k];
@@ -526,8 +531,7 @@ __device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan,
         else if (k < e3) w = (int)(offsetof(UnitStage, sinfo) / 16) + (k - e2);
         else if (k < e4) w = (int)(offsetof(UnitStage, spl) / 16) + (k - e3);
         else w = (int)(offsetof(UnitStage, toff) / 16) + (k - e4);
-        if (k < e5) base[w] = v;
-        else reinterpret_cast<int4 *>(spidx)[k - e5] = v;
+        base[w] = v;
     }
 }
 
@@ -692,10 +696,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ double acc[10][TBX_UNIT_TASKS];
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
-    __shared__ __align__(16) unsigned char spidx[TBX_UNIT_PAIRS * MAXSH * MAXSH];
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12 | pair << 20
     __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];
-    __shared__ int eoff[2 * TBX_UNIT_SEGS + 1];  // element offsets: A blocks, then mirrors
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -706,8 +708,13 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const bool accum = A.accumulate != 0;
     const int chunk = A.chunk;
     for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
-    // (1) the unit's plan
-    load_unit(H, mt.fplan, U, spidx);
+    // (1) the unit's plan; this lane's first element descriptor (used last)
+    load_unit(H, mt.fplan, U);
+    const int *edesc = mt.fplan + 4 * ((long long)H.plan + plan_words(H));
+    const int nel = H.nel;
+    // nel <= 2 * TBX_UNIT_TASKS: at most two elements per lane
+    const int d0 = tid < nel ? edesc[tid] : 0;
+    const int d1 = tid + NTHREADS < nel ? edesc[tid + NTHREADS] : 0;
     __syncthreads();
 
     // (2) pair vectors; shell pairs: screening counts and H0 factors (one
@@ -766,59 +773,25 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         for (int k = 0; k < 10; ++k) atomicAdd(&acc[k][tk], o[k]);
     }
 
-    // (5) element layout per segment: block A (rows of atoms j0..j1-1, the
-    // columns of atom i) and the mirror (rows of atom i, columns of the j != i)
-    if (tid < 64) {  // one wavefront; nseg <= TBX_UNIT_SEGS <= 64
-        int na = 0, nb = 0;
-        if (tid < nseg) {
-            const int4 g = U.sinfo[tid];
-            na = g.w * (g.z & 0xffff);
-            nb = g.w * (g.z >> 16);
-        }
-        const int ia = wave_scan(na), ib = wave_scan(nb);
-        const int ta = __shfl(ia, 63, 64);
-        if (tid < nseg) {
-            eoff[tid] = ia - na;
-            eoff[nseg + tid] = ta + ib - nb;
-        }
-        if (tid == 63) eoff[2 * nseg] = ta + ib;
-    }
+    // (5) one lane per dense element (descriptors from the plan): block A
+    // (rows of atoms j, the columns of atom i) and the mirror (rows of atom
+    // i, columns of the j < i); every element of the molecule's matrices is
+    // written by exactly one lane of one unit
     __syncthreads();
-    const int nel = eoff[2 * nseg];
     for (int base = 0; base < nel; base += NTHREADS) {
         const int e = base + tid;
         if (e < nel) {
-            const int kk = find_last_le(eoff, 2 * nseg, e);
-            const bool mirror = kk >= nseg;
-            const int k = mirror ? kk - nseg : kk;
-            const int local = e - eoff[kk];
-            const int4 g = U.sinfo[k];
-            const int naoi = g.w, rows = g.z & 0xffff;
-            int mu, nul;
-            if (!mirror) {
-                nul = local / rows;
-                mu = g.y + local - nul * rows;
-            } else {
-                mu = g.y + local / naoi;
-                nul = local - (local / naoi) * naoi;
-            }
-            // pair of row mu: the pairs of a segment hold consecutive atoms j
-            int p = g.x;
-            {
-                int hi = (k + 1 < nseg) ? U.sinfo[k + 1].x : npair;
-                while (hi - p > 1) {
-                    const int mid = (p + hi) >> 1;
-                    if (U.pinf[mid].aoj <= mu) p = mid; else hi = mid;
-                }
-            }
+            const int dsc = base == 0 ? d0 : d1;
+            const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31;
+            const int task0 = (dsc >> 17) & 255;
+            const bool mirror = (dsc >> 25) & 1;
             const PairInfo pr = U.pinf[p];
+            const int spx = (tinfo[task0] >> 12) & 255;
             int ish, mi, li, jsh, mj, lj;
             ao_decode(pr.di, nul, ish, mi, li);
-            ao_decode(pr.dj, mu - pr.aoj, jsh, mj, lj);
-            const int spx = spidx[p * MAXSH * MAXSH + ish * MAXSH + jsh];
-            const int t = li * 3 + lj;
+            ao_decode(pr.dj, muj, jsh, mj, lj);
             const int ncj = ncart_of(lj);
-            const int task0 = U.toff[t] + (spx - U.tsp[t]) * ncart_of(li) * ncj;
+            const int mu = pr.aoj + muj;
             double v[10];
 #pragma unroll
             for (int q = 0; q < 10; ++q) v[q] = 0.0;
@@ -1217,7 +1190,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += NTHREADS) (&accS[0][0])[k] = 0.0;
     if (tid < 6) sig[tid] = 0.0;
     // (1) the unit's plan
-    load_unit(H, mt.plan, U, nullptr);
+    load_unit(H, mt.plan, U);
     __syncthreads();
 
     // (2) pair vectors; shell pairs: screening counts and H0 factors (one
@@ -1265,20 +1238,26 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     // (3) chunk offsets: block-wide exclusive scan
     const int nchunk = block_scan(nch, coff, U.wsum);
 
-    // (4) work items
-    for (int c = tid; c < nchunk; c += NTHREADS) {
+    // (4) work items, one per lane and step; the results of a shell pair's
+    // items (consecutive lanes) are summed across the wavefront before the
+    // LDS atomics
+    const int lane = tid & 63;
+    for (int base = 0; base < nchunk; base += NTHREADS) {
+        const int c = base + tid;
+        int key = -1 - lane;  // shell pair of the item (p | ish << 12 | jsh << 15); unique if idle
+        double g[3] = {0, 0, 0}, ds = 0.0;
+        if (c < nchunk) {
         const int tk = find_task(coff, c);
         const int info = tinfo[tk];
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
         const int p = (info >> 12) & 4095;
-        const int ish = (info >> 24) & 7, jsh = (info >> 27) & 7;
+        key = info >> 12;
         const int q = c - coff[tk];
         const int p0 = tpp[tk] + q * chunk;
         const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * chunk);
         const double *r = rec + tk;
         const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
         const int L = TBX_UNIT_TASKS;
-        double g[3] = {0, 0, 0}, ds = 0.0;
         if (t == 0 || t == 1 || t == 3 || t == 4) {
             switch (t) {
             case 0: shellpair_run<0, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
@@ -1307,11 +1286,30 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
             g[2] = a3[2] + s * r[(R_EX + 2) * L] + 2.0 * WD2;
             ds = s * r[R_HSW * L];
         }
-        atomicAdd(&accP[p][0], g[0]);
-        atomicAdd(&accP[p][1], g[1]);
-        atomicAdd(&accP[p][2], g[2]);
-        atomicAdd(&accP[p][3 + ish], ds);
-        atomicAdd(&accP[p][3 + MAXSH + jsh], ds);
+        }
+        // segmented sum over runs of equal key (a shell pair's items are
+        // consecutive): the first lane of each run ends with the run's total
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int k2 = __shfl_down(key, off, 64);
+            const double y0 = __shfl_down(g[0], off, 64), y1 = __shfl_down(g[1], off, 64),
+                         y2 = __shfl_down(g[2], off, 64), y3 = __shfl_down(ds, off, 64);
+            if (lane + off < 64 && k2 == key) {
+                g[0] += y0;
+                g[1] += y1;
+                g[2] += y2;
+                ds += y3;
+            }
+        }
+        const int kprev = __shfl_up(key, 1, 64);
+        if (key >= 0 && (lane == 0 || kprev != key)) {
+            const int p = key & 4095, ish = (key >> 12) & 7, jsh = (key >> 15) & 7;
+            atomicAdd(&accP[p][0], g[0]);
+            atomicAdd(&accP[p][1], g[1]);
+            atomicAdd(&accP[p][2], g[2]);
+            atomicAdd(&accP[p][3 + ish], ds);
+            atomicAdd(&accP[p][3 + MAXSH + jsh], ds);
+        }
     }
     __syncthreads();
 

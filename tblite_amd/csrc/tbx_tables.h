@@ -83,14 +83,19 @@ struct TbxPair {
 //   shell pairs nspair ints: pair | jsh << 8 | ish << 12 | type << 16, type-major
 //              (type = 3 l_i + l_j)
 //   offsets    toff[10] then tsp[10]: first task / shell pair of each type
-//   spidx      (forward units only) npair x 16 bytes: shell-pair index of
-//              (pair, ish, jsh) at byte 16 pair + 4 ish + jsh
+// Forward units add, after these sections (not staged in LDS), one int per
+// dense element the unit writes: pair | nul << 7 | muj << 12 | task0 << 17 |
+// mirror << 25, with nul / muj the AO of the element within atom i / j,
+// task0 the first Cartesian task of its shell pair and mirror set for the
+// transposed block (rows of atom i).  Block A elements come first (segment
+// by segment, column-major), then the mirrored ones (row-major).
 struct TbxUnitHead {
     long long dense0;  // first dense element of the unit's molecule
     int plan;          // first 16-byte word of the plan
     int counts;        // npair | nseg << 8 | nspair << 16
     int nao, ao_m;     // AOs of the molecule, its first AO (global)
-    int mol, pad;
+    int mol;
+    int nel;           // dense elements written (forward units)
 };
 
 // Molecule / atom tables of a batch (device pointers).  "local" indices
