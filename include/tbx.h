@@ -103,6 +103,23 @@ int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const tbx_list *
                         const int *iao_sh, int nao, const double *selfenergy, double *overlap,
                         double *dpint, double *qpint, double *hamiltonian);
 
+/* The full double-loop integral builders get_multipole_integrals_lat,
+ * get_dipole_integrals_lat and get_overlap_lat
+ * (integral/native/integrals.f90:1394-1465, 994-1059, 411-470): dense
+ * S(nao,nao), D(3,nao,nao), Q(6,nao,nao) (traceless, operator on the column
+ * function), intent(out), over every atom pair and translation within
+ * geom->cutoff.  The values are get_hamiltonian's S, D, Q.  Coincident
+ * distinct atoms (r^2 below machine epsilon) contribute nothing, as in the
+ * half neighbour list; the reference's double loop would add their blocks. */
+int tbx_get_multipole_integrals(tbx_context *ctx, const tbx_geom *geom, const tbx_basis *bas,
+                                const int *ish_at, const int *iao_sh, int nao, double *overlap,
+                                double *dpint, double *qpint);
+int tbx_get_dipole_integrals(tbx_context *ctx, const tbx_geom *geom, const tbx_basis *bas,
+                             const int *ish_at, const int *iao_sh, int nao, double *overlap,
+                             double *dpint);
+int tbx_get_overlap(tbx_context *ctx, const tbx_geom *geom, const tbx_basis *bas,
+                    const int *ish_at, const int *iao_sh, int nao, double *overlap);
+
 /* get_hamiltonian_gradient (xtb/h0.f90:432-666).  pmat, xmat: (nao,nao,nspin)
  * (magnet representation, wavefunction/spin.f90:104-153); vao: (nao,nspin);
  * vdp: (3,nat,nspin); vqp: (6,nat,nspin).  dEdcn(nat), gradient(3,nat) and

@@ -188,6 +188,27 @@ def get_hamiltonian(setup, selfenergy):
     return S, D, Q, H
 
 
+def get_multipole_integrals_lat(setup, cutoff=None, which="multipole"):
+    """Full double-loop builders (integral/native/integrals.f90:411-470,
+    994-1059, 1394-1465): which = "overlap" -> S; "dipole" -> S, D;
+    "multipole" -> S, D, Q.  Dense, Fortran order."""
+    mol, bas, nao = setup.mol, setup.bas, setup.bas.nao
+    cut = bas.get_cutoff() if cutoff is None else cutoff
+    S = np.zeros((nao, nao), order="F")
+    D = np.zeros((3, nao, nao), order="F") if which in ("dipole", "multipole") else None
+    Q = np.zeros((6, nao, nao), order="F") if which == "multipole" else None
+    keep = [_i32(mol.id), _f64(mol.xyz), _f64(setup.trans), _i32(setup.spec.nsh_id),
+            _i32(bas.ish_at), _i32(bas.iao_sh), _cgto_array(setup)]
+    st = lib().orc_get_multipole_integrals_lat(
+        ct.c_int(mol.nat), ct.c_int(setup.msh), ct.c_int(nao), _p(keep[0]), _p(keep[1]),
+        ct.c_int(len(setup.trans)), _p(keep[2]), ct.c_double(cut), _p(keep[3]), _p(keep[4]),
+        _p(keep[5]), _p(keep[6]), ct.c_double(bas.intcut), _p(S),
+        _p(D) if D is not None else None, _p(Q) if Q is not None else None)
+    if st != 0:
+        raise ValueError("oracle: unsupported basis")
+    return tuple(x for x in (S, D, Q) if x is not None)
+
+
 def get_hamiltonian_gradient(setup, selfenergy, dsedcn, vao, vdp, vqp, pmat, xmat,
                              dEdcn=None, gradient=None, sigma=None):
     """Accumulates into dEdcn (nat), gradient (3,nat) and sigma (3,3) (Fortran order)."""

@@ -24,6 +24,9 @@
  *   orc_new_adjacency_list    adjlist.f90:96-143
  *   orc_get_hamiltonian       xtb/h0.f90:220-429 (do_diat_scale = .false. for GFN2)
  *   orc_get_hamiltonian_gradient  xtb/h0.f90:432-666
+ *   orc_get_multipole_integrals_lat  integral/native/integrals.f90:1394-1465
+ *                             (its S and S, D are get_overlap_lat :411-470 and
+ *                             get_dipole_integrals_lat :994-1059)
  *
  * Conventions kept from the reference: all arrays are Fortran column-major,
  * indices are 0-based on this side; Cartesian components in CCA order;
@@ -692,6 +695,64 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
 #undef H_
 #undef D_
 #undef Q_
+    return 0;
+}
+
+/* get_multipole_integrals_lat: integral/native/integrals.f90:1394-1465.
+ * Full double loop over (iat, jat, itr) within the cutoff, every block
+ * computed directly with the operator on its column atom (no half list, no
+ * shift_operator).  overlap(nao,nao); dpint(3,nao,nao) and qpint(6,nao,nao)
+ * may be NULL (get_overlap_lat, get_dipole_integrals_lat). */
+int orc_get_multipole_integrals_lat(int nat, int msh, int nao, const int *id, const double *xyz,
+                                    int ntr, const double *trans, double cutoff,
+                                    const int *nsh_id, const int *ish_at, const int *iao_sh,
+                                    const orc_cgto *cgto, double intcut, double *overlap,
+                                    double *dpint, double *qpint)
+{
+    size_t n2 = (size_t)nao * nao;
+    memset(overlap, 0, sizeof(double) * n2);
+    if (dpint) memset(dpint, 0, sizeof(double) * 3 * n2);
+    if (qpint) memset(qpint, 0, sizeof(double) * 6 * n2);
+    for (int iat = 0; iat < nat; ++iat)
+        for (int ish = 0; ish < nsh_id[id[iat]]; ++ish)
+            if (cgto[ish + msh * id[iat]].ang > 2) return -1;
+    const double cutoff2 = cutoff * cutoff;
+#pragma omp parallel for schedule(dynamic)
+    for (int iat = 0; iat < nat; ++iat) {
+        double stmp[25], dtmp[75], qtmp[150];
+        int izp = id[iat], is = ish_at[iat];
+        for (int jat = 0; jat < nat; ++jat) {
+            int jzp = id[jat], js = ish_at[jat];
+            for (int itr = 0; itr < ntr; ++itr) {
+                double vec[3];
+                for (int k = 0; k < 3; ++k)
+                    vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k] - trans[3 * itr + k];
+                double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+                if (r2 > cutoff2) continue;
+                for (int ish = 0; ish < nsh_id[izp]; ++ish) {
+                    int ii = iao_sh[is + ish];
+                    const orc_cgto *ci = &cgto[ish + msh * izp];
+                    for (int jsh = 0; jsh < nsh_id[jzp]; ++jsh) {
+                        int jj = iao_sh[js + jsh];
+                        const orc_cgto *cj = &cgto[jsh + msh * jzp];
+                        orc_multipole_cgto(cj, ci, r2, vec, intcut, stmp, dtmp, qtmp);
+                        int nj = nsph(cj->ang), ni = nsph(ci->ang);
+                        /* column ii + iao belongs to iat: this thread owns it */
+                        for (int iao = 0; iao < ni; ++iao)
+                            for (int jao = 0; jao < nj; ++jao) {
+                                int ij = jao + nj * iao;
+                                size_t e = (size_t)(jj + jao) + (size_t)nao * (ii + iao);
+                                overlap[e] += stmp[ij];
+                                if (dpint)
+                                    for (int k = 0; k < 3; ++k) dpint[k + 3 * e] += dtmp[3 * ij + k];
+                                if (qpint)
+                                    for (int k = 0; k < 6; ++k) qpint[k + 6 * e] += qtmp[6 * ij + k];
+                            }
+                    }
+                }
+            }
+        }
+    }
     return 0;
 }
 

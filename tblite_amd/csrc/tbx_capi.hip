@@ -788,6 +788,62 @@ extern "C" int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const
     return check_flag(ctx);
 }
 
+// get_multipole_integrals / get_dipole_integrals / get_overlap: the forward
+// kernel over the pairs within geom->cutoff with neutral H0 parameters (the
+// H0 it writes to device scratch is dropped).  dpint / qpint may be NULL.
+extern "C" int tbx_get_multipole_integrals(tbx_context *ctx, const tbx_geom *geom,
+                                           const tbx_basis *bas, const int *ish_at,
+                                           const int *iao_sh, int nao, double *overlap,
+                                           double *dpint, double *qpint)
+{
+    if (!bas || bas->nid <= 0 || bas->msh <= 0 || bas->msh > TBX_MAXSH)
+        return fail(TBX_ERR_BASIS, "invalid basis");
+    if (!overlap) return fail(TBX_ERR_ARG, "null output array");
+    const int nid = bas->nid, msh = bas->msh;
+    std::vector<double> hscale((size_t)msh * msh * nid * nid, 0.0), shpoly((size_t)msh * nid, 0.0),
+        rad(nid, 1.0);
+    const tbx_h0par h0 = {hscale.data(), shpoly.data(), rad.data()};
+    HostSpecies hs;
+    HostMols hm;
+    int st = prepare_single(ctx, geom, nullptr, bas, &h0, ish_at, iao_sh, nao, hs, hm);
+    if (st) return st;
+    hipStream_t s = ctx->stream;
+    const size_t n2 = (size_t)nao * nao;
+    const std::vector<double> se(hm.nsh, 0.0);
+    HIP_TRY(ctx->se.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(hipMemcpyAsync(ctx->se.p, se.data(), sizeof(double) * hm.nsh, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->S.reserve(sizeof(double) * n2));
+    HIP_TRY(ctx->D.reserve(sizeof(double) * 3 * n2));
+    HIP_TRY(ctx->Q.reserve(sizeof(double) * 6 * n2));
+    HIP_TRY(ctx->H.reserve(sizeof(double) * n2));
+    HIP_TRY(tbx_launch_forward(&ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
+                               ctx->se.as<double>(), ctx->S.as<double>(), ctx->D.as<double>(),
+                               ctx->Q.as<double>(), ctx->H.as<double>(), hm.ndense, nullptr,
+                               ctx->flag.as<int>(), 0, s));
+    HIP_TRY(hipMemcpyAsync(overlap, ctx->S.p, sizeof(double) * n2, hipMemcpyDeviceToHost, s));
+    if (dpint)
+        HIP_TRY(hipMemcpyAsync(dpint, ctx->D.p, sizeof(double) * 3 * n2, hipMemcpyDeviceToHost, s));
+    if (qpint)
+        HIP_TRY(hipMemcpyAsync(qpint, ctx->Q.p, sizeof(double) * 6 * n2, hipMemcpyDeviceToHost, s));
+    return check_flag(ctx);
+}
+
+extern "C" int tbx_get_dipole_integrals(tbx_context *ctx, const tbx_geom *geom,
+                                        const tbx_basis *bas, const int *ish_at, const int *iao_sh,
+                                        int nao, double *overlap, double *dpint)
+{
+    if (!dpint) return fail(TBX_ERR_ARG, "null output array");
+    return tbx_get_multipole_integrals(ctx, geom, bas, ish_at, iao_sh, nao, overlap, dpint,
+                                       nullptr);
+}
+
+extern "C" int tbx_get_overlap(tbx_context *ctx, const tbx_geom *geom, const tbx_basis *bas,
+                               const int *ish_at, const int *iao_sh, int nao, double *overlap)
+{
+    return tbx_get_multipole_integrals(ctx, geom, bas, ish_at, iao_sh, nao, overlap, nullptr,
+                                       nullptr);
+}
+
 extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *geom,
                                             const tbx_list *list, const tbx_basis *bas,
                                             const tbx_h0par *h0, const int *ish_at,

@@ -5,6 +5,9 @@
 !>   get_selfenergy            src/tblite/xtb/h0.f90:113-217
 !>   get_hamiltonian           src/tblite/xtb/h0.f90:220-429
 !>   get_hamiltonian_gradient  src/tblite/xtb/h0.f90:432-666
+!>   get_multipole_integrals, get_dipole_integrals, get_overlap (the full
+!>   double-loop builders)    src/tblite/integral/native/integrals.f90:1394-1465,
+!>                            994-1059, 411-470
 !> with the derived types (structure_type, basis_type, tb_hamiltonian,
 !> adjacency_list, potential_type) flattened to arrays.  Atom, species and
 !> translation indices passed in are 1-based as in tblite and are converted to
@@ -20,6 +23,7 @@ module tbx_h0
 
    public :: tbx_h0_tables, tbx_h0_context, tbx_h0_release, tbx_h0_last_error
    public :: tbx_h0_get_selfenergy, tbx_h0_get_hamiltonian, tbx_h0_get_hamiltonian_gradient
+   public :: tbx_h0_get_multipole_integrals
 
    integer, parameter, public :: tbx_maxg = 12, tbx_maxsh = 4
 
@@ -116,6 +120,18 @@ module tbx_h0
          type(c_ptr), value :: vao, vdp, vqp, pmat, xmat, dEdcn, gradient, sigma
          integer(c_int) :: stat
       end function c_get_hamiltonian_gradient
+
+      function c_get_multipole_integrals(ctx, geom, bas, ish_at, iao_sh, nao, overlap, dpint, &
+            & qpint) result(stat) bind(C, name="tbx_get_multipole_integrals")
+         import :: c_int, c_ptr, geom_c, basis_c
+         type(c_ptr), value :: ctx
+         type(geom_c), intent(in) :: geom
+         type(basis_c), intent(in) :: bas
+         type(c_ptr), value :: ish_at, iao_sh
+         integer(c_int), value :: nao
+         type(c_ptr), value :: overlap, dpint, qpint
+         integer(c_int) :: stat
+      end function c_get_multipole_integrals
 
       function c_strlen(s) result(n) bind(C, name="strlen")
          import :: c_ptr, c_int
@@ -223,6 +239,39 @@ contains
          p = h0par_c(c_loc(t%hscale), c_loc(t%shpoly), c_loc(t%rad))
       end subroutine pack_tables
    end subroutine tbx_h0_get_hamiltonian
+
+   !> get_multipole_integrals / get_dipole_integrals / get_overlap: dense
+   !> overlap and, when present, dpint(3,nao,nao) and qpint(6,nao,nao)
+   !> (intent(out)) over every atom pair and translation within cutoff.
+   subroutine tbx_h0_get_multipole_integrals(ctx, tab, id, xyz, trans, cutoff, ish_at, iao_sh, &
+         & nao, overlap, stat, dpint, qpint)
+      type(c_ptr), intent(in) :: ctx
+      type(tbx_h0_tables), intent(in), target :: tab
+      integer, intent(in) :: id(:)
+      real(c_double), intent(in), target, contiguous :: xyz(:, :), trans(:, :)
+      real(c_double), intent(in) :: cutoff
+      integer(c_int), intent(in), target, contiguous :: ish_at(:), iao_sh(:)
+      integer, intent(in) :: nao
+      real(c_double), intent(out), target, contiguous :: overlap(:, :)
+      integer, intent(out) :: stat
+      real(c_double), intent(out), target, contiguous, optional :: dpint(:, :, :), qpint(:, :, :)
+      type(geom_c) :: geom
+      type(basis_c) :: bas
+      integer(c_int), allocatable, target :: id0(:)
+      type(c_ptr) :: pd, pq
+
+      id0 = int(id - 1, c_int)
+      bas = basis_c(int(tab%nid, c_int), int(tab%msh, c_int), c_loc(tab%nsh_id), c_loc(tab%ang), &
+         & c_loc(tab%nprim), c_loc(tab%alpha), c_loc(tab%coeff), tab%intcut)
+      geom = geom_c(int(size(id), c_int), c_loc(id0), c_loc(xyz), int(size(trans, 2), c_int), &
+         & c_loc(trans), cutoff)
+      pd = c_null_ptr
+      pq = c_null_ptr
+      if (present(dpint)) pd = c_loc(dpint)
+      if (present(qpint)) pq = c_loc(qpint)
+      stat = c_get_multipole_integrals(ctx, geom, bas, c_loc(ish_at), c_loc(iao_sh), &
+         & int(nao, c_int), c_loc(overlap), pd, pq)
+   end subroutine tbx_h0_get_multipole_integrals
 
    !> get_hamiltonian_gradient: contracts the integral derivatives with the
    !> density pmat and energy-weighted density xmat (nao, nao, nspin, magnet

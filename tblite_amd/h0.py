@@ -90,6 +90,47 @@ def get_hamiltonian(setup: Setup, selfenergy, device: int = 0):
     return S, D, Q, H
 
 
+def get_multipole_integrals(setup: Setup, device: int = 0):
+    """``get_multipole_integrals`` (``integral/native/integrals.f90:1394-1465``,
+    the full double loop over the pairs within the cutoff).
+
+    Returns Fortran-ordered ``overlap(nao,nao)``, ``dpint(3,nao,nao)`` and
+    ``qpint(6,nao,nao)``."""
+    a = _Args(setup)
+    nao = a.nao
+    S = np.zeros((nao, nao), order="F")
+    D = np.zeros((3, nao, nao), order="F")
+    Q = np.zeros((6, nao, nao), order="F")
+    L.check(L.lib().tbx_get_multipole_integrals(L.context(device), ct.byref(a.geom),
+                                                ct.byref(a.basis), _p(a.ish_at), _p(a.iao_sh),
+                                                nao, _p(S), _p(D), _p(Q)))
+    return S, D, Q
+
+
+def get_dipole_integrals(setup: Setup, device: int = 0):
+    """``get_dipole_integrals`` (``integral/native/integrals.f90:994-1059``):
+    ``overlap(nao,nao)`` and ``dpint(3,nao,nao)``."""
+    a = _Args(setup)
+    nao = a.nao
+    S = np.zeros((nao, nao), order="F")
+    D = np.zeros((3, nao, nao), order="F")
+    L.check(L.lib().tbx_get_dipole_integrals(L.context(device), ct.byref(a.geom),
+                                             ct.byref(a.basis), _p(a.ish_at), _p(a.iao_sh), nao,
+                                             _p(S), _p(D)))
+    return S, D
+
+
+def get_overlap(setup: Setup, device: int = 0):
+    """``get_overlap`` (``integral/native/integrals.f90:411-470``):
+    ``overlap(nao,nao)``."""
+    a = _Args(setup)
+    nao = a.nao
+    S = np.zeros((nao, nao), order="F")
+    L.check(L.lib().tbx_get_overlap(L.context(device), ct.byref(a.geom), ct.byref(a.basis),
+                                    _p(a.ish_at), _p(a.iao_sh), nao, _p(S)))
+    return S
+
+
 def get_hamiltonian_gradient(setup: Setup, selfenergy, dsedcn, vao, vdp, vqp, pmat, xmat,
                              dEdcn, gradient, sigma, device: int = 0):
     """``get_hamiltonian_gradient`` (``xtb/h0.f90:432-666``).

@@ -73,6 +73,12 @@ _SIGS = {
     "tbx_get_selfenergy": (ct.c_int, [ct.c_int, _P, _P, _P, ct.c_int, _P, _P, _P, _P, _P]),
     "tbx_get_hamiltonian": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(List), ct.POINTER(Basis),
                                        ct.POINTER(H0Par), _P, _P, ct.c_int, _P, _P, _P, _P, _P]),
+    "tbx_get_multipole_integrals": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(Basis), _P, _P,
+                                               ct.c_int, _P, _P, _P]),
+    "tbx_get_dipole_integrals": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(Basis), _P, _P,
+                                            ct.c_int, _P, _P]),
+    "tbx_get_overlap": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(Basis), _P, _P, ct.c_int,
+                                   _P]),
     "tbx_get_hamiltonian_gradient": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(List),
                                                 ct.POINTER(Basis), ct.POINTER(H0Par), _P, _P,
                                                 ct.c_int, _P, _P, ct.c_int, _P, _P, _P, _P, _P,

@@ -6,7 +6,11 @@ wrong sign or index, a transposed operand) fails one of them:
 
 - 93 self-overlap KATs (``test/unit/test_slater_expansion.f90``);
 - literal STO-6G overlap matrices (``test/unit/test_integral_overlap.f90:273-823``)
-  reproduced after fitting the mstore geometry the reference used;
+  reproduced after fitting the mstore geometry the reference used, through
+  both the half-list builder and the full double-loop one;
+- the full double-loop builders (``integrals.f90:411-470,994-1059,1394-1465``)
+  against the half list with ``shift_operator`` for molecules and lattice
+  images: two routes to the same S, D, Q;
 - literal GFN2 H0 matrices (``test/unit/test_hamiltonian.f90:510-745``)
   reproduced after fitting geometry and coordination numbers;
 - the dipole translation rule (``test/unit/test_integral_multipole.f90:108-211``);
@@ -124,6 +128,13 @@ def test_literal_overlap(name):
         if np.abs(r.fun).max() < 1e5 * EPS:
             break
     assert np.abs(best.fun).max() < 1e5 * EPS
+    # the literals are get_overlap's (the full double-loop builder): check that route too
+    xyz = np.vstack([np.zeros(3), best.x.reshape(nat - 1, 3)])
+    mnums, _ = params.species_of(nums)
+    cg, nsh = refdata.test_basis_cgto("basis_overlap", mnums)
+    st = refdata.custom_setup(nums, xyz, cg, nsh)
+    (S,) = O.get_multipole_integrals_lat(st, which="overlap")
+    assert np.abs(S - ref).max() < 1e5 * EPS
 
 
 # starting points near the mstore geometries (bohr), rounded
@@ -440,3 +451,43 @@ def test_hamiltonian_gradient_fd(nspin):
             return (_energy(x0, mol, st, se2, P, W, vao, vdp, vqp),)
         ncn[iat] = _richardson(f, np.array([cn[iat]]), 0, h=1e-2)[0]
     assert np.abs(dEdcn - ncn).max() < 1e-7 * max(1.0, np.abs(ncn).max())
+
+
+# -- full double-loop builders against the half list ---------------------------
+
+def _lat_case(periodic):
+    from tblite_amd import structure, synthetic
+    if not periodic:
+        z, x = synthetic.octane()
+        return structure.setup(structure.Structure(z, x))
+    z = np.array([8, 1, 1, 6, 1, 16])
+    x = np.array([[0.0, 0.0, 0.0], [1.8, 0.0, 0.0], [-0.5, 1.7, 0.0], [4.0, 3.0, 3.0],
+                  [5.9, 3.2, 3.4], [1.0, 5.0, 6.5]])
+    trans = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)],
+                     dtype=float) * 14.0
+    return structure.setup(structure.Structure(z, x), trans=trans)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_lat_builders_match_half_list(periodic):
+    """get_multipole_integrals_lat computes every block directly with the
+    operator on its column atom; get_hamiltonian computes the half list and
+    moves the operator of each transposed block with shift_operator
+    (h0.f90:704-747).  The two must give the same S, D and Q; the overlap and
+    dipole builders are the leading parts of the multipole one."""
+    st = _lat_case(periodic)
+    S0, D0, Q0, _ = O.get_hamiltonian(st, np.zeros(st.bas.nsh))
+    S, D, Q = O.get_multipole_integrals_lat(st)
+    assert np.abs(S - S0).max() < 1e-12
+    assert np.abs(D - D0).max() < 1e-12
+    assert np.abs(Q - Q0).max() < 1e-12
+    (S1,) = O.get_multipole_integrals_lat(st, which="overlap")
+    S2, D2 = O.get_multipole_integrals_lat(st, which="dipole")
+    assert np.array_equal(S1, S) and np.array_equal(S2, S) and np.array_equal(D2, D)
+    # the operator sits on the column atom: D(:, mu, nu) - D(:, nu, mu) is
+    # (R_row - R_col) S(mu, nu) for a molecule (the dipole translation rule)
+    if not periodic:
+        R = st.mol.xyz.reshape(-1, 3)[st.bas.ao2at]
+        lhs = D - np.transpose(D, (0, 2, 1))
+        rhs = (R[:, None, :] - R[None, :, :]).transpose(2, 0, 1) * S[None]
+        assert np.abs(lhs - rhs).max() < 1e-10
