@@ -188,6 +188,35 @@ int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
                       const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
                       const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream);
 
+/* --- SCF-loop consumers of the resident integrals (batch layout) ---------
+ * Dense spin arrays (d_P, d_h1) hold nspin (1 or 2) blocks of ndense
+ * doubles; d_vao is [nao*nspin], d_vmp [vdim*nat*nspin].  The multipole
+ * integrals are the forward's D (mpdim 3) or Q (mpdim 6) output, operator
+ * on the column atom.  Asynchronous on `stream`, no allocation. */
+
+/* add_vao_to_h1 (scf/potential.f90:192-219): d_h1 -= S (vao_r + vao_c)/2. */
+int tbx_batch_add_vao_to_h1(tbx_batch *b, int nspin, const double *d_S, const double *d_vao,
+                            double *d_h1, void *stream);
+/* add_vmp_to_h1 (scf/potential.f90:221-255): the first min(mpdim, vdim)
+ * components of d_vmp against d_mpint [mpdim*ndense]. */
+int tbx_batch_add_vmp_to_h1(tbx_batch *b, int nspin, int mpdim, const double *d_mpint, int vdim,
+                            const double *d_vmp, double *d_h1, void *stream);
+/* get_mulliken_shell_charges (wavefunction/mulliken.f90:37-62): d_P in the
+ * up/down representation; d_qsh [nsh*nspin] OVERWRITTEN in the magnet
+ * representation (updown_to_magnet, wavefunction/spin.f90), + d_n0sh [nsh]
+ * on the first. */
+int tbx_batch_mulliken_shell_charges(tbx_batch *b, int nspin, const double *d_S,
+                                     const double *d_P, const double *d_n0sh, double *d_qsh,
+                                     void *stream);
+/* get_mulliken_atomic_multipoles (wavefunction/mulliken.f90:65-98):
+ * d_mpat [mpdim*nat*nspin] OVERWRITTEN (magnet representation). */
+int tbx_batch_mulliken_atomic_multipoles(tbx_batch *b, int nspin, int mpdim,
+                                         const double *d_mpint, const double *d_P,
+                                         double *d_mpat, void *stream);
+/* Milliseconds of the last add_vao_to_h1, add_vmp_to_h1, shell-charge and
+ * atomic-multipole launches (ms[4]; -1 if not run), no reference counterpart. */
+int tbx_batch_last_scf_timing(const tbx_batch *b, double *ms);
+
 /* Kernel timing of the last batched call on the context's internal events:
  * milliseconds of the forward / pair-list / gradient / adjoint kernels. */
 int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pairs,

@@ -160,6 +160,47 @@ class Batch:
                                           self._stream()))
         return dxyz, dse
 
+    # -- SCF-loop consumers of the resident integrals ------------------------
+    # (scf/potential.f90:192-255, wavefunction/mulliken.f90:37-98).  Spin
+    # arrays are packed spin-outermost: P and h1 [nspin*ndense], vao
+    # [nspin*nao], vmp [nspin*nat*vdim]; mp is the forward's D (mpdim 3) or
+    # Q (mpdim 6).
+
+    def add_vao_to_h1(self, S, vao, h1, nspin=1):
+        """``add_vao_to_h1``: h1 -= S (vao_r + vao_c)/2, in place."""
+        L.check(L.lib().tbx_batch_add_vao_to_h1(self.handle, nspin, _ptr(S), _ptr(vao), _ptr(h1),
+                                                self._stream()))
+        return h1
+
+    def add_vmp_to_h1(self, mp, mpdim, vmp, vdim, h1, nspin=1):
+        """``add_vmp_to_h1`` with the first min(mpdim, vdim) components, in place."""
+        L.check(L.lib().tbx_batch_add_vmp_to_h1(self.handle, nspin, mpdim, _ptr(mp), vdim,
+                                                _ptr(vmp), _ptr(h1), self._stream()))
+        return h1
+
+    def mulliken_shell_charges(self, S, P, n0sh, nspin=1, out=None):
+        """``get_mulliken_shell_charges``: qsh [nspin*nsh] (magnet representation)."""
+        if out is None:
+            out = torch.empty(nspin * self.nsh, dtype=torch.float64, device=self.torch_device)
+        L.check(L.lib().tbx_batch_mulliken_shell_charges(self.handle, nspin, _ptr(S), _ptr(P),
+                                                         _ptr(n0sh), _ptr(out), self._stream()))
+        return out
+
+    def mulliken_atomic_multipoles(self, mp, mpdim, P, nspin=1, out=None):
+        """``get_mulliken_atomic_multipoles``: mpat [nspin*nat*mpdim] (magnet representation)."""
+        if out is None:
+            out = torch.empty(nspin * self.nat * mpdim, dtype=torch.float64,
+                              device=self.torch_device)
+        L.check(L.lib().tbx_batch_mulliken_atomic_multipoles(self.handle, nspin, mpdim, _ptr(mp),
+                                                             _ptr(P), _ptr(out), self._stream()))
+        return out
+
+    def last_scf_timing(self):
+        v = (ct.c_double * 4)()
+        L.check(L.lib().tbx_batch_last_scf_timing(self.handle, v))
+        return dict(zip(("add_vao_ms", "add_vmp_ms", "shell_charges_ms", "atomic_multipoles_ms"),
+                        list(v)))
+
     def tiles(self) -> dict:
         """Current tile shapes (work-unit caps and chunk lengths) of the kernels."""
         t = L.Tiles()

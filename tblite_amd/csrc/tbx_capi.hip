@@ -37,6 +37,11 @@ extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
                                        const double *dse, const double *dsedcn, const double *P,
                                        double *dEdcn, hipStream_t stream);
 extern "C" hipError_t tbx_launch_fill(double *p, long long n, double v, hipStream_t stream);
+extern "C" hipError_t tbx_launch_scf(int what, const TbxMols *mol, int nspin, long long dstride,
+                                     int mpdim, int vdim, const double *S, const double *mp,
+                                     const double *P, const double *vao, const double *vmp,
+                                     const double *n0sh, double *h1, double *out,
+                                     hipStream_t stream);
 extern "C" hipError_t tbx_launch_copy(const double *src, double *dst, long long n,
                                       hipStream_t stream);
 
@@ -246,7 +251,7 @@ struct HostMols {
     int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, sh_off, unit_seg, funit_seg, uplan, fplan;
+        cart_atom, ao_atom, sh_off, unit_seg, funit_seg, uplan, fplan, sh_aog, tiles;
     std::vector<long long> mol_dense;
     std::vector<TbxSeg> useg, fseg;
     std::vector<TbxUnitHead> uhead, fhead;
@@ -496,6 +501,8 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
             hm.at_sh[a] = sh;
             hm.at_ao[a] = ao;
             hm.at_cart[a] = c;
+            for (int q = 0; q < hs.nsh[s]; ++q)
+                hm.sh_aog.push_back(hm.mol_ao[m] + ao + hs.sh_ao[q + hs.msh * s]);
             sh += hs.nsh[s];
             ao += hs.nao[s];
             c += hs.ncart[s];
@@ -510,6 +517,12 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
         cg += c;
     }
     hm.sh_off[nmol] = sh;
+    hm.sh_aog.push_back(aog);
+    for (int m = 0; m < nmol; ++m) {  // 32 x 32 tile pairs (bi <= bj) of every dense block
+        const int t = (hm.mol_nao[m] + 31) / 32;
+        for (int bj = 0; bj < t; ++bj)
+            for (int bi = 0; bi <= bj; ++bi) hm.tiles.insert(hm.tiles.end(), {m, bi, bj, 0});
+    }
     hm.nsh = sh;
     hm.nao = aog;
     hm.ncart = cg;
@@ -522,7 +535,7 @@ struct DevTables {
     DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
         ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off, pp_mu, ao_info, ao_desc;
     DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, mol_ao, uhead, uplan, fhead, fplan;
+        cart_atom, ao_atom, mol_ao, uhead, uplan, fhead, fplan, sh_aog, tiles;
     TbxSpecies sp{};
     TbxMols mt{};
 
@@ -534,7 +547,7 @@ struct DevTables {
                          &trans, &mol_atom,
                          &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
                          &at_cart, &cart_atom, &ao_atom, &mol_ao, &uhead, &uplan,
-                         &fhead, &fplan};
+                         &fhead, &fplan, &sh_aog, &tiles};
         for (DevBuf *b : all) b->release();
     }
 };
@@ -587,7 +600,7 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
 #define UP(name) if ((e = upload(d.name, hm.name, s)) != hipSuccess) return e
     UP(mol_atom); UP(mol_nao); UP(mol_cart); UP(mol_dense); UP(at_sp); UP(at_mol); UP(at_sh);
     UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(uhead); UP(uplan);
-    UP(fhead); UP(fplan);
+    UP(fhead); UP(fplan); UP(sh_aog); UP(tiles);
 #undef UP
     if ((e = upload(d.trans, trans, s)) != hipSuccess) return e;
     TbxMols &mt = d.mt;
@@ -613,6 +626,11 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     mt.nfunit = (int)hm.fhead.size();
     mt.funit = d.fhead.as<TbxUnitHead>();
     mt.fplan = d.fplan.as<int>();
+    mt.nsh_tot = hm.nsh;
+    mt.nao_tot = hm.nao;
+    mt.sh_aog = d.sh_aog.as<int>();
+    mt.ntile = (int)(hm.tiles.size() / 4);
+    mt.tiles = d.tiles.as<int>();
     return hipSuccess;
 }
 
@@ -633,7 +651,7 @@ struct tbx_batch {
     tbx_tiles tiles = kDefaultTiles;
     DevTables tab;
     DevBuf flag, dse_tmp;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[16] = {};  // pairs (start, end): forward, -, gradient, adjoint, 4 SCF kernels
     bool timed = false;
 };
 
@@ -934,7 +952,7 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     if (e == hipSuccess)
         e = b->dse_tmp.reserve(sizeof(double) * std::max({1, b->hm.nsh, 9 * b->hm.nmol}));
     if (e == hipSuccess) e = hipMemsetAsync(b->flag.p, 0, sizeof(int), ctx->stream);
-    for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
+    for (int k = 0; k < 16 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         tbx_batch_delete(b);
@@ -1089,6 +1107,70 @@ extern "C" int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, dou
     if (ms_pairs) *ms_pairs = 0.0;
     if (ms_gradient) *ms_gradient = el(4, 5);
     if (ms_adjoint) *ms_adjoint = el(6, 7);
+    (void)hipGetLastError();  // a query of a never-recorded event is not an error here
+    return TBX_OK;
+}
+
+// SCF-loop consumers of the resident integrals (scf/potential.f90:192-255,
+// wavefunction/mulliken.f90:37-98): one launch each, events 8 + 2 what.
+static int scf_launch(tbx_batch *b, int what, int nspin, int mpdim, int vdim, const double *S,
+                      const double *mp, const double *P, const double *vao, const double *vmp,
+                      const double *n0sh, double *h1, double *out, void *stream)
+{
+    if (nspin < 1 || nspin > 2) return fail(TBX_ERR_ARG, "nspin=%d", nspin);
+    if (mpdim < 1 || mpdim > 6 || vdim < 1) return fail(TBX_ERR_ARG, "mpdim=%d vdim=%d", mpdim, vdim);
+    hipStream_t s = pick_stream(b, stream);
+    HIP_TRY(mark(b, 8 + 2 * what, s));
+    HIP_TRY(tbx_launch_scf(what, &b->tab.mt, nspin, b->hm.ndense, mpdim, vdim, S, mp, P, vao, vmp,
+                           n0sh, h1, out, s));
+    HIP_TRY(mark(b, 9 + 2 * what, s));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_add_vao_to_h1(tbx_batch *b, int nspin, const double *d_S,
+                                       const double *d_vao, double *d_h1, void *stream)
+{
+    if (!b || !d_S || !d_vao || !d_h1) return fail(TBX_ERR_ARG, "null argument");
+    return scf_launch(b, 0, nspin, 1, 1, d_S, nullptr, nullptr, d_vao, nullptr, nullptr, d_h1,
+                      nullptr, stream);
+}
+
+extern "C" int tbx_batch_add_vmp_to_h1(tbx_batch *b, int nspin, int mpdim, const double *d_mpint,
+                                       int vdim, const double *d_vmp, double *d_h1, void *stream)
+{
+    if (!b || !d_mpint || !d_vmp || !d_h1) return fail(TBX_ERR_ARG, "null argument");
+    return scf_launch(b, 1, nspin, mpdim, vdim, nullptr, d_mpint, nullptr, nullptr, d_vmp,
+                      nullptr, d_h1, nullptr, stream);
+}
+
+extern "C" int tbx_batch_mulliken_shell_charges(tbx_batch *b, int nspin, const double *d_S,
+                                                const double *d_P, const double *d_n0sh,
+                                                double *d_qsh, void *stream)
+{
+    if (!b || !d_S || !d_P || !d_n0sh || !d_qsh) return fail(TBX_ERR_ARG, "null argument");
+    return scf_launch(b, 2, nspin, 1, 1, d_S, nullptr, d_P, nullptr, nullptr, d_n0sh, nullptr,
+                      d_qsh, stream);
+}
+
+extern "C" int tbx_batch_mulliken_atomic_multipoles(tbx_batch *b, int nspin, int mpdim,
+                                                    const double *d_mpint, const double *d_P,
+                                                    double *d_mpat, void *stream)
+{
+    if (!b || !d_mpint || !d_P || !d_mpat) return fail(TBX_ERR_ARG, "null argument");
+    return scf_launch(b, 3, nspin, mpdim, 1, nullptr, d_mpint, d_P, nullptr, nullptr, nullptr,
+                      nullptr, d_mpat, stream);
+}
+
+extern "C" int tbx_batch_last_scf_timing(const tbx_batch *b, double *ms)
+{
+    if (!b || !ms) return fail(TBX_ERR_ARG, "null argument");
+    for (int k = 0; k < 4; ++k) {
+        float t = 0.0f;
+        ms[k] = -1.0;
+        if (hipEventQuery(b->ev[9 + 2 * k]) == hipSuccess &&
+            hipEventElapsedTime(&t, b->ev[8 + 2 * k], b->ev[9 + 2 * k]) == hipSuccess)
+            ms[k] = t;
+    }
     (void)hipGetLastError();  // a query of a never-recorded event is not an error here
     return TBX_OK;
 }

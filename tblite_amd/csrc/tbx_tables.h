@@ -122,4 +122,9 @@ struct TbxMols {
     int nfunit;
     const TbxUnitHead *funit;      // [nfunit] forward units (j <= i, on-site pairs included)
     const int *fplan;
+    int nsh_tot, nao_tot;
+    const int *sh_aog;             // [nsh_tot+1] first AO (global) of every shell
+    int ntile;
+    const int *tiles;              // [4*ntile] (mol, bi, bj, 0): 32 x 32 tile pairs bi <= bj
+                                   // of the dense blocks
 };

@@ -83,6 +83,12 @@ _SIGS = {
                                                 ct.POINTER(Basis), ct.POINTER(H0Par), _P, _P,
                                                 ct.c_int, _P, _P, ct.c_int, _P, _P, _P, _P, _P,
                                                 _P, _P, _P]),
+    "tbx_batch_add_vao_to_h1": (ct.c_int, [_P, ct.c_int, _P, _P, _P, _P]),
+    "tbx_batch_add_vmp_to_h1": (ct.c_int, [_P, ct.c_int, ct.c_int, _P, ct.c_int, _P, _P, _P]),
+    "tbx_batch_mulliken_shell_charges": (ct.c_int, [_P, ct.c_int, _P, _P, _P, _P, _P]),
+    "tbx_batch_mulliken_atomic_multipoles": (ct.c_int, [_P, ct.c_int, ct.c_int, _P, _P, _P,
+                                                        _P]),
+    "tbx_batch_last_scf_timing": (ct.c_int, [_P, _P]),
     "tbx_batch_create": (ct.c_int, [_P, ct.POINTER(BatchDesc), ct.POINTER(_P)]),
     "tbx_batch_delete": (None, [_P]),
     "tbx_batch_sizes_get": (ct.c_int, [_P, ct.POINTER(BatchSizes)]),

@@ -1,0 +1,133 @@
+"""Pin the oracle of the SCF-loop consumers (oracle/scf.py) before trusting it.
+
+The checks are identities the physics fixes, computed by a different route
+(traces and plain matrix algebra) from the loops of the restatement:
+
+- a constant potential: add_vao_to_h1 adds -c S;
+- the variational pairs: tr(P dh1) of add_vao_to_h1 / add_vmp_to_h1 equals the
+  potential contracted with get_mulliken_shell_charges / _atomic_multipoles
+  of the same density (the Mulliken charges and multipoles are the
+  derivatives of the energy with respect to those potentials);
+- the charge sum sum(qsh) = sum(n0sh) - tr(P S);
+- the spin representation: equal up and down densities give the closed-shell
+  charges with zero magnetisation (updown_to_magnet, wavefunction/spin.f90).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import scf as OS
+from tblite_amd import structure, synthetic
+
+
+def _setup(name):
+    if name == "octane":
+        z, x = synthetic.octane()
+    else:  # d shells
+        z, x = np.array([16, 16, 17, 1, 8, 1, 1]), np.array(
+            [[0, 0, 0], [0.3, 0.2, 3.7], [3.1, -1.0, 1.2], [-1.9, 1.1, -0.4], [5.0, 2.0, 1.0],
+             [6.7, 2.1, 1.5], [4.6, 3.6, 0.5]])
+    st = structure.setup(structure.Structure(z, x))
+    S, D, Q, _ = O.get_hamiltonian(st, np.zeros(st.bas.nsh))
+    return st, S, D, Q
+
+
+def _density(nao, rng):
+    c = rng.normal(size=(nao, nao // 2))
+    return c @ c.T / nao
+
+
+def _atom_ranges(bas):
+    nat = len(bas.ish_at)
+    first = bas.iao_sh[bas.ish_at]
+    nao_at = np.bincount(bas.ao2at, minlength=nat)
+    return first, nao_at
+
+
+@pytest.mark.parametrize("name", ["octane", "sulfur"])
+def test_constant_potential(name):
+    st, S, _, _ = _setup(name)
+    nao = st.bas.nao
+    h1 = np.zeros((nao, nao, 2))
+    vao = np.array([[0.3, -1.1]] * nao)
+    OS.add_vao_to_h1(S, vao, h1)
+    assert np.abs(h1[:, :, 0] + 0.3 * S).max() < 1e-14
+    assert np.abs(h1[:, :, 1] - 1.1 * S).max() < 1e-14
+
+
+@pytest.mark.parametrize("name", ["octane", "sulfur"])
+def test_vao_against_shell_charges(name):
+    st, S, _, _ = _setup(name)
+    bas, nao = st.bas, st.bas.nao
+    rng = np.random.default_rng(1)
+    P = _density(nao, rng)
+    vsh = rng.normal(size=bas.nsh)
+    vao = np.repeat(vsh, bas.nao_sh)[:, None]
+    h1 = np.zeros((nao, nao, 1))
+    OS.add_vao_to_h1(S, vao, h1)
+    n0 = rng.uniform(0, 2, bas.nsh)
+    qsh = OS.get_mulliken_shell_charges(S, P[:, :, None], n0, bas.iao_sh, bas.nao_sh)
+    assert abs(np.sum(P * h1[:, :, 0]) - vsh @ (qsh[:, 0] - n0)) < 1e-12
+    assert abs(qsh[:, 0].sum() - (n0.sum() - np.sum(P * S))) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["octane", "sulfur"])
+@pytest.mark.parametrize("which", ["dipole", "quadrupole"])
+def test_vmp_against_atomic_multipoles(name, which):
+    st, S, D, Q = _setup(name)
+    bas, nao, nat = st.bas, st.bas.nao, st.mol.nat
+    mp = D if which == "dipole" else Q
+    nmp = mp.shape[0]
+    rng = np.random.default_rng(2)
+    P = _density(nao, rng)
+    vmp = rng.normal(size=(nmp, nat, 1))
+    h1 = np.zeros((nao, nao, 1))
+    OS.add_vmp_to_h1(mp, vmp, bas.ao2at, h1)
+    first, nao_at = _atom_ranges(bas)
+    mpat = OS.get_mulliken_atomic_multipoles(mp, P[:, :, None], first, nao_at)
+    assert abs(np.sum(P * h1[:, :, 0]) - np.sum(vmp[:, :, 0] * mpat[:, :, 0])) < 1e-12
+    # the update is symmetric and only the leading min(nmp, nvmp) components enter
+    assert np.abs(h1[:, :, 0] - h1[:, :, 0].T).max() < 1e-14
+    h2 = np.zeros((nao, nao, 1))
+    OS.add_vmp_to_h1(mp, np.concatenate([vmp, rng.normal(size=(2, nat, 1))]), bas.ao2at, h2)
+    assert np.array_equal(h1, h2)
+
+
+def test_spin_representation():
+    st, S, D, _ = _setup("sulfur")
+    bas, nao = st.bas, st.bas.nao
+    rng = np.random.default_rng(3)
+    P = _density(nao, rng)
+    n0 = rng.uniform(0, 2, bas.nsh)
+    q1 = OS.get_mulliken_shell_charges(S, P[:, :, None], n0, bas.iao_sh, bas.nao_sh)
+    q2 = OS.get_mulliken_shell_charges(S, np.stack([P / 2, P / 2], axis=2), n0, bas.iao_sh,
+                                       bas.nao_sh)
+    assert np.abs(q2[:, 0] - q1[:, 0]).max() < 1e-13 and np.abs(q2[:, 1]).max() < 1e-13
+    Pu, Pd = _density(nao, rng), _density(nao, rng)
+    q3 = OS.get_mulliken_shell_charges(S, np.stack([Pu, Pd], axis=2), n0, bas.iao_sh, bas.nao_sh)
+    qu = OS.get_mulliken_shell_charges(S, Pu[:, :, None], 0 * n0, bas.iao_sh, bas.nao_sh)[:, 0]
+    qd = OS.get_mulliken_shell_charges(S, Pd[:, :, None], 0 * n0, bas.iao_sh, bas.nao_sh)[:, 0]
+    assert np.abs(q3[:, 0] - (qu + qd + n0)).max() < 1e-13
+    assert np.abs(q3[:, 1] - (qu - qd)).max() < 1e-13
+    first, nao_at = _atom_ranges(bas)
+    m2 = OS.get_mulliken_atomic_multipoles(D, np.stack([Pu, Pd], axis=2), first, nao_at)
+    mu = OS.get_mulliken_atomic_multipoles(D, Pu[:, :, None], first, nao_at)[:, :, 0]
+    md = OS.get_mulliken_atomic_multipoles(D, Pd[:, :, None], first, nao_at)[:, :, 0]
+    assert np.abs(m2[:, :, 0] - (mu + md)).max() < 1e-13
+    assert np.abs(m2[:, :, 1] - (mu - md)).max() < 1e-13
+
+
+def test_identity_detects_planted_mistake():
+    """A transposed operand in add_vmp_to_h1 (the atom of the row instead of
+    the column) breaks the variational identity."""
+    st, S, D, _ = _setup("sulfur")
+    bas, nao, nat = st.bas, st.bas.nao, st.mol.nat
+    rng = np.random.default_rng(4)
+    P = _density(nao, rng)
+    vmp = rng.normal(size=(3, nat, 1))
+    h1 = np.zeros((nao, nao, 1))
+    wrong = np.transpose(D, (0, 2, 1)).copy()
+    OS.add_vmp_to_h1(wrong, vmp, bas.ao2at, h1)
+    first, nao_at = _atom_ranges(bas)
+    mpat = OS.get_mulliken_atomic_multipoles(D, P[:, :, None], first, nao_at)
+    assert abs(np.sum(P * h1[:, :, 0]) - np.sum(vmp[:, :, 0] * mpat[:, :, 0])) > 1e-6
