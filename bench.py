@@ -263,6 +263,36 @@ def main():
         t += time.perf_counter() - t1
     plain_rate = nsample / t
 
+    # SCF-loop consumers of the resident S / Q (one HBM pass each, nspin = 1):
+    # kernel time from the batch's events, algorithmic bytes per dense element
+    S_r, _, Q_r, _ = outs
+    h1 = torch.zeros(b.ndense, dtype=torch.float64, device=dev)
+    n0 = torch.zeros(b.nsh, dtype=torch.float64, device=dev)
+    vmp = torch.randn(6 * b.nat, generator=g, dtype=torch.float64, device=dev) * 1e-3
+    scf = {}
+    for name, key, fn, per_elem in (
+            ("add_vao_to_h1", "add_vao_ms", lambda: b.add_vao_to_h1(S_r, vao, h1), 24),
+            ("add_vmp_to_h1 (quadrupole)", "add_vmp_ms",
+             lambda: b.add_vmp_to_h1(Q_r, 6, vmp, 6, h1), 64),
+            ("get_mulliken_shell_charges", "shell_charges_ms",
+             lambda: b.mulliken_shell_charges(S_r, P, n0), 16),
+            ("get_mulliken_atomic_multipoles (quadrupole)", "atomic_multipoles_ms",
+             lambda: b.mulliken_atomic_multipoles(Q_r, 6, P), 56)):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(5):
+            fn()
+            torch.cuda.synchronize()
+            ms.append(b.last_scf_timing()[key])
+        t_ms = float(np.median(ms))
+        gbs = per_elem * b.ndense / (t_ms * 1e-3) / 1e9
+        scf[name] = {"ms": t_ms, "bytes_per_dense_element": per_elem, "GB/s": gbs,
+                     "frac_hbm_peak": gbs / roofline.PEAK_HBM_GBS}
+    log("scf consumers: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GB/s']:.0f} GB/s)"
+                                      for k, v in scf.items()))
+    del h1
+
     ms_step = el_a / args.steps * 1e3
     value = b.nmol / (el_a / args.steps)
     # dominant kernel of the autograd step
@@ -309,6 +339,7 @@ def main():
         "gradient_fp64_TFLOPs": adj_flop / (grad_p * 1e-3) / 1e12,
         "autograd_vs_plain_max_abs_diff": agree,
         "tiles": b.tiles(),
+        "scf_consumers": scf,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -34,6 +34,27 @@ struct ScfArgs {
     double *out;         // qsh [nsh_tot * nspin] or mpat [mpdim * nat * nspin]
 };
 
+// sum_q e[q] v[q], q < NMP; 16-byte loads of e when mpdim is even (every
+// element then starts on a 16-byte boundary)
+template <int NMP>
+__device__ __forceinline__ double mp_dot(const double *e, const double *v, int mpdim)
+{
+    double b = 0.0;
+    if ((mpdim & 1) == 0) {
+        const double2 *e2 = reinterpret_cast<const double2 *>(e);
+#pragma unroll
+        for (int q = 0; q + 1 < NMP; q += 2) {
+            const double2 x = e2[q >> 1];
+            b += x.x * v[q] + x.y * v[q + 1];
+        }
+        if (NMP & 1) b += e[NMP - 1] * v[NMP - 1];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NMP; ++q) b += e[q] * v[q];
+    }
+    return b;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -65,10 +86,14 @@ __global__ void __launch_bounds__(SCF_THREADS) k_add_vao(ScfArgs A)
 // B(r, c) = sum_k mp(k, r, c) vmp(k, atom(c), s): the operator of mp sits on
 // the column atom.  One workgroup per pair of 32 x 32 tiles (bi, bj), bi <= bj,
 // and its transpose, so that both reads are coalesced; B goes through LDS.
+// NMP = min(mpdim, vdim) components.
+template <int NMP>
 __global__ void __launch_bounds__(SCF_THREADS) k_add_vmp(ScfArgs A)
 {
+    constexpr int KS = SCF_TILE / (SCF_THREADS / SCF_TILE);  // columns per thread and tile
     __shared__ double B1[SCF_TILE][SCF_TILE + 1];  // tile (bi, bj): [col][row]
     __shared__ double B2[SCF_TILE][SCF_TILE + 1];  // tile (bj, bi): [col][row]
+    __shared__ double vc[2][SCF_TILE][NMP];        // potential of the column atoms of both tiles
     const TbxMols &mt = A.mt;
     const int4 tile = reinterpret_cast<const int4 *>(mt.tiles)[blockIdx.x];
     const int m = tile.x, bi = tile.y, bj = tile.z;
@@ -76,41 +101,52 @@ __global__ void __launch_bounds__(SCF_THREADS) k_add_vmp(ScfArgs A)
     const long long dense0 = mt.mol_dense[m];
     const int rl = threadIdx.x & (SCF_TILE - 1), cl0 = threadIdx.x / SCF_TILE;
     const bool diag = bi == bj;
+    const int r1 = bi * SCF_TILE + rl, r2 = bj * SCF_TILE + rl;
     for (int s = 0; s < A.nspin; ++s) {
         const double *vm = A.vmp + (long long)s * A.vdim * nat;
+        double *h = A.h1 + s * A.dstride + dense0;
+        if (threadIdx.x < 2 * SCF_TILE) {  // columns bj*32.. (tile 1) and bi*32.. (tile 2)
+            const int t = threadIdx.x / SCF_TILE, cl = threadIdx.x % SCF_TILE;
+            const int c = (t == 0 ? bj : bi) * SCF_TILE + cl;
+            if (c < n) {
+                const double *v = vm + (long long)A.vdim * mt.ao_atom[a0 + c];
 #pragma unroll
-        for (int k = 0; k < SCF_TILE / (SCF_THREADS / SCF_TILE); ++k) {
+                for (int q = 0; q < NMP; ++q) vc[t][cl][q] = v[q];
+            }
+        }
+        __syncthreads();
+        double h1a[KS], h1b[KS];  // current h1 of the two tiles, loaded with the integrals
+#pragma unroll
+        for (int k = 0; k < KS; ++k) {
             const int cl = cl0 + k * (SCF_THREADS / SCF_TILE);
-            // tile (bi, bj): row bi*32 + rl, column bj*32 + cl; and its transpose
-            const int r1 = bi * SCF_TILE + rl, c1 = bj * SCF_TILE + cl;
-            const int r2 = bj * SCF_TILE + rl, c2 = bi * SCF_TILE + cl;
+            const int c1 = bj * SCF_TILE + cl, c2 = bi * SCF_TILE + cl;
             double b1 = 0.0, b2 = 0.0;
+            h1a[k] = h1b[k] = 0.0;
             if (r1 < n && c1 < n) {
-                const double *e = A.mp + (long long)A.mpdim * (dense0 + (long long)c1 * n + r1);
-                const double *v = vm + (long long)A.vdim * mt.ao_atom[a0 + c1];
-                for (int q = 0; q < A.nmp; ++q) b1 += e[q] * v[q];
+                const long long e = (long long)c1 * n + r1;
+                b1 = mp_dot<NMP>(A.mp + (long long)A.mpdim * (dense0 + e), vc[0][cl], A.mpdim);
+                h1a[k] = h[e];
             }
             if (!diag && r2 < n && c2 < n) {
-                const double *e = A.mp + (long long)A.mpdim * (dense0 + (long long)c2 * n + r2);
-                const double *v = vm + (long long)A.vdim * mt.ao_atom[a0 + c2];
-                for (int q = 0; q < A.nmp; ++q) b2 += e[q] * v[q];
+                const long long e = (long long)c2 * n + r2;
+                b2 = mp_dot<NMP>(A.mp + (long long)A.mpdim * (dense0 + e), vc[1][cl], A.mpdim);
+                h1b[k] = h[e];
             }
             B1[cl][rl] = b1;
             B2[cl][rl] = b2;
         }
         __syncthreads();
-        double *h = A.h1 + s * A.dstride + dense0;
 #pragma unroll
-        for (int k = 0; k < SCF_TILE / (SCF_THREADS / SCF_TILE); ++k) {
+        for (int k = 0; k < KS; ++k) {
             const int cl = cl0 + k * (SCF_THREADS / SCF_TILE);
-            const int r1 = bi * SCF_TILE + rl, c1 = bj * SCF_TILE + cl;
+            const int c1 = bj * SCF_TILE + cl, c2 = bi * SCF_TILE + cl;
             if (r1 < n && c1 < n)
-                h[(long long)c1 * n + r1] -= 0.5 * (B1[cl][rl] + (diag ? B1[rl][cl] : B2[rl][cl]));
-            const int r2 = bj * SCF_TILE + rl, c2 = bi * SCF_TILE + cl;
+                h[(long long)c1 * n + r1] =
+                    h1a[k] - 0.5 * (B1[cl][rl] + (diag ? B1[rl][cl] : B2[rl][cl]));
             if (!diag && r2 < n && c2 < n)
-                h[(long long)c2 * n + r2] -= 0.5 * (B2[cl][rl] + B1[rl][cl]);
+                h[(long long)c2 * n + r2] = h1b[k] - 0.5 * (B2[cl][rl] + B1[rl][cl]);
         }
-        __syncthreads();  // B1 / B2 are rewritten for the next spin
+        __syncthreads();  // B1, B2 and vc are rewritten for the next spin
     }
 }
 
@@ -127,14 +163,14 @@ __global__ void __launch_bounds__(SCF_THREADS) k_mulliken_shell(ScfArgs A)
     const int m = mt.at_mol[mt.ao_atom[g0]];
     const int a0 = mt.mol_ao[m], n = mt.mol_nao[m];
     const long long dense0 = mt.mol_dense[m];
+    // the shell's columns are one contiguous run of (g1 - g0) * n elements
+    const long long e0 = dense0 + (long long)(g0 - a0) * n;
+    const int len = (g1 - g0) * n;
     double q[2] = {0.0, 0.0};
     for (int s = 0; s < A.nspin; ++s) {
+        const double *p = A.P + s * A.dstride + e0;
         double acc = 0.0;
-        for (int g = g0; g < g1; ++g) {
-            const long long col = dense0 + (long long)(g - a0) * n;
-            const double *p = A.P + s * A.dstride + col;
-            for (int r = lane; r < n; r += 64) acc += p[r] * A.S[col + r];
-        }
+        for (int k = lane; k < len; k += 64) acc += p[k] * A.S[e0 + k];
         q[s] = -wave_sum(acc);
     }
     if (lane == 0) {
@@ -160,18 +196,32 @@ __global__ void __launch_bounds__(SCF_THREADS) k_mulliken_atom(ScfArgs A)
     const int g0 = a0 + mt.at_ao[at];
     const int g1 = (at + 1 < mt.nat && mt.at_mol[at + 1] == m) ? a0 + mt.at_ao[at + 1] : a0 + n;
     const long long dense0 = mt.mol_dense[m];
+    // the atom's columns are one contiguous run of (g1 - g0) * n elements
+    const long long e0 = dense0 + (long long)(g0 - a0) * n;
+    const int len = (g1 - g0) * n;
     double q[2][6];
     for (int s = 0; s < A.nspin; ++s) {
+        const double *p = A.P + s * A.dstride + e0;
         double acc[6] = {0, 0, 0, 0, 0, 0};
-        for (int g = g0; g < g1; ++g) {
-            const long long col = dense0 + (long long)(g - a0) * n;
-            const double *p = A.P + s * A.dstride + col;
-            for (int r = lane; r < n; r += 64) {
-                const double pv = p[r];
-                const double *e = A.mp + (long long)A.mpdim * (col + r);
+        if (A.mpdim == 6) {
+            const double2 *e2 = reinterpret_cast<const double2 *>(A.mp + 6 * e0);
+            for (int k = lane; k < len; k += 64) {
+                const double pv = p[k];
+                const double2 x = e2[3 * k], y = e2[3 * k + 1], z = e2[3 * k + 2];
+                acc[0] += pv * x.x;
+                acc[1] += pv * x.y;
+                acc[2] += pv * y.x;
+                acc[3] += pv * y.y;
+                acc[4] += pv * z.x;
+                acc[5] += pv * z.y;
+            }
+        } else {
+            for (int k = lane; k < len; k += 64) {
+                const double pv = p[k];
+                const double *e = A.mp + (long long)A.mpdim * (e0 + k);
 #pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    if (k < A.mpdim) acc[k] += pv * e[k];
+                for (int c = 0; c < 6; ++c)
+                    if (c < A.mpdim) acc[c] += pv * e[c];
             }
         }
 #pragma unroll
@@ -226,8 +276,17 @@ extern "C" hipError_t tbx_launch_scf(int what, const TbxMols *mol, int nspin, lo
         break;
     }
     case 1:  // add_vmp_to_h1
-        if (mol->ntile > 0)
-            hipLaunchKernelGGL(k_add_vmp, dim3(mol->ntile), dim3(SCF_THREADS), 0, stream, a);
+        if (mol->ntile > 0) {
+            const dim3 g(mol->ntile), t(SCF_THREADS);
+            switch (a.nmp) {
+            case 1: hipLaunchKernelGGL(k_add_vmp<1>, g, t, 0, stream, a); break;
+            case 2: hipLaunchKernelGGL(k_add_vmp<2>, g, t, 0, stream, a); break;
+            case 3: hipLaunchKernelGGL(k_add_vmp<3>, g, t, 0, stream, a); break;
+            case 4: hipLaunchKernelGGL(k_add_vmp<4>, g, t, 0, stream, a); break;
+            case 5: hipLaunchKernelGGL(k_add_vmp<5>, g, t, 0, stream, a); break;
+            default: hipLaunchKernelGGL(k_add_vmp<6>, g, t, 0, stream, a); break;
+            }
+        }
         break;
     case 2:  // Mulliken shell charges
         if (mol->nsh_tot > 0)
