@@ -24,6 +24,7 @@
 // All arithmetic is FP64.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstddef>
 
 #include "tbx_tables.h"
@@ -1153,6 +1154,33 @@ __device__ __forceinline__ void pprow_run(const double *__restrict__ pp, const d
     }
 }
 
+// One step of a segmented inclusive scan of (g, ds) over lanes with equal
+// key: add the values of the lane that the DPP control CTRL (row mask
+// ROWMASK) reads from, if its key is the same.  Lanes the move does not reach
+// read INT_MIN, which no key equals.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_scan_step(int key, double g[3], double &ds)
+{
+    const int k2 = __builtin_amdgcn_update_dpp(INT_MIN, key, CTRL, ROWMASK, 0xf, false);
+    const double y0 = dpp_move<CTRL, ROWMASK>(g[0]), y1 = dpp_move<CTRL, ROWMASK>(g[1]);
+    const double y2 = dpp_move<CTRL, ROWMASK>(g[2]), y3 = dpp_move<CTRL, ROWMASK>(ds);
+    if (k2 == key) {
+        g[0] += y0;
+        g[1] += y1;
+        g[2] += y2;
+        ds += y3;
+    }
+}
+
 // One workgroup per work unit (up to TBX_UNIT_PAIRS atom pairs of one
 // molecule, at most TBX_UNIT_TASKS Cartesian-pair tasks).  (1) The unit's
 // pairs are staged in LDS and their shell pairs laid out by type (l_i, l_j);
@@ -1287,22 +1315,18 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
             ds = s * r[R_HSW * L];
         }
         }
-        // segmented sum over runs of equal key (a shell pair's items are
-        // consecutive): the first lane of each run ends with the run's total
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int k2 = __shfl_down(key, off, 64);
-            const double y0 = __shfl_down(g[0], off, 64), y1 = __shfl_down(g[1], off, 64),
-                         y2 = __shfl_down(g[2], off, 64), y3 = __shfl_down(ds, off, 64);
-            if (lane + off < 64 && k2 == key) {
-                g[0] += y0;
-                g[1] += y1;
-                g[2] += y2;
-                ds += y3;
-            }
-        }
-        const int kprev = __shfl_up(key, 1, 64);
-        if (key >= 0 && (lane == 0 || kprev != key)) {
+        // segmented inclusive scan over runs of equal key (a shell pair's
+        // items are consecutive lanes) with DPP moves: shifts by 1, 2, 4, 8
+        // within rows of 16 lanes, then the row broadcasts of lanes 15 and 31;
+        // the last lane of each run ends with the run's total
+        seg_scan_step<0x111, 0xf>(key, g, ds);  // row_shr:1
+        seg_scan_step<0x112, 0xf>(key, g, ds);  // row_shr:2
+        seg_scan_step<0x114, 0xf>(key, g, ds);  // row_shr:4
+        seg_scan_step<0x118, 0xf>(key, g, ds);  // row_shr:8
+        seg_scan_step<0x142, 0xa>(key, g, ds);  // row_bcast:15 into rows 1 and 3
+        seg_scan_step<0x143, 0xc>(key, g, ds);  // row_bcast:31 into rows 2 and 3
+        const int knext = __shfl_down(key, 1, 64);
+        if (key >= 0 && (lane == 63 || knext != key)) {
             const int p = key & 4095, ish = (key >> 12) & 7, jsh = (key >> 15) & 7;
             atomicAdd(&accP[p][0], g[0]);
             atomicAdd(&accP[p][1], g[1]);
