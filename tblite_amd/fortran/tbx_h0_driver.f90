@@ -1,20 +1,21 @@
 !> Fortran host driver of the tbx_h0 binding: reads one structure with its
 !> flattened GFN2 tables, density and potential from a stream file, calls
-!> tbx_h0_get_hamiltonian and tbx_h0_get_hamiltonian_gradient on HIP device 0
-!> and writes the dense integrals and the accumulated gradient terms.
+!> tbx_h0_get_hamiltonian, tbx_h0_get_hamiltonian_gradient and
+!> tbx_h0_get_multipole_integrals on HIP device 0 and writes the dense
+!> integrals, the accumulated gradient terms and the full double-loop S, D, Q.
 !> Usage: tbx_h0_driver <input> <output>
 program tbx_h0_driver
    use, intrinsic :: iso_c_binding, only : c_int, c_double, c_ptr, c_associated
    use tbx_h0
    implicit none
    integer(c_int) :: hdr(8)
-   integer :: nat, nid, msh, nao, nsh, ntrans, nspin, nnbr, stat1, stat2, u
+   integer :: nat, nid, msh, nao, nsh, ntrans, nspin, nnbr, stat1, stat2, stat3, u
    type(tbx_h0_tables) :: tab
    integer(c_int), allocatable :: id(:), ish_at(:), iao_sh(:), inl(:), nnl(:), nlat(:), nltr(:)
    real(c_double), allocatable :: xyz(:, :), trans(:, :), se(:), dsedcn(:), vao(:, :), &
       & vdp(:, :, :), vqp(:, :, :), pmat(:, :, :), xmat(:, :, :)
    real(c_double), allocatable :: s(:, :), d(:, :, :), q(:, :, :), h(:, :), dEdcn(:), &
-      & grad(:, :), sigma(:, :)
+      & grad(:, :), sigma(:, :), s2(:, :), d2(:, :, :), q2(:, :, :)
    real(c_double) :: cutoff
    character(len=512) :: fin, fout
    type(c_ptr) :: ctx
@@ -57,11 +58,15 @@ program tbx_h0_driver
       & se, dsedcn, vao, vdp, vqp, pmat, xmat, dEdcn, grad, sigma, stat2, &
       & inl=inl, nnl=nnl, nlat=nlat, nltr=nltr)
    if (stat2 /= 0) write(*, '(a)') "get_hamiltonian_gradient: " // tbx_h0_last_error()
+   allocate(s2(nao, nao), d2(3, nao, nao), q2(6, nao, nao))
+   call tbx_h0_get_multipole_integrals(ctx, tab, id, xyz, trans, cutoff, ish_at, iao_sh, nao, &
+      & s2, stat3, dpint=d2, qpint=q2)
+   if (stat3 /= 0) write(*, '(a)') "get_multipole_integrals: " // tbx_h0_last_error()
    call tbx_h0_release(ctx)
 
    open(newunit=u, file=trim(fout), access="stream", form="unformatted", status="replace")
-   write(u) int([stat1, stat2], c_int)
-   write(u) s, d, q, h, dEdcn, grad, sigma
+   write(u) int([stat1, stat2, stat3], c_int)
+   write(u) s, d, q, h, dEdcn, grad, sigma, s2, d2, q2
    close(u)
-   if (stat1 /= 0 .or. stat2 /= 0) error stop 1
+   if (stat1 /= 0 .or. stat2 /= 0 .or. stat3 /= 0) error stop 1
 end program tbx_h0_driver

@@ -40,10 +40,12 @@ def write_input(path, st, se, dse, vao, vdp, vqp, P, W):
 def read_output(path, st):
     nat, nao = st.mol.nat, st.bas.nao
     raw = open(path, "rb").read()
-    stat = np.frombuffer(raw[:8], dtype="<i4")
-    v = np.frombuffer(raw[8:], dtype="<f8")
-    sizes = [nao * nao, 3 * nao * nao, 6 * nao * nao, nao * nao, nat, 3 * nat, 9]
-    shapes = [(nao, nao), (3, nao, nao), (6, nao, nao), (nao, nao), (nat,), (3, nat), (3, 3)]
+    stat = np.frombuffer(raw[:12], dtype="<i4")
+    v = np.frombuffer(raw[12:], dtype="<f8")
+    sizes = [nao * nao, 3 * nao * nao, 6 * nao * nao, nao * nao, nat, 3 * nat, 9,
+             nao * nao, 3 * nao * nao, 6 * nao * nao]
+    shapes = [(nao, nao), (3, nao, nao), (6, nao, nao), (nao, nao), (nat,), (3, nat), (3, 3),
+              (nao, nao), (3, nao, nao), (6, nao, nao)]
     out, o = [], 0
     for n, sh in zip(sizes, shapes):
         out.append(v[o:o + n].reshape(sh, order="F"))

@@ -44,11 +44,14 @@ def test_fortran_driver_parity(name, tmp_path):
                        timeout=120)
     sys.stdout.write(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    stat, (S, D, Q, H, dEdcn, grad, sigma) = read_output(fout, st)
-    assert list(stat) == [0, 0]
+    stat, (S, D, Q, H, dEdcn, grad, sigma, S2, D2, Q2) = read_output(fout, st)
+    assert list(stat) == [0, 0, 0]
     for ref, got, lab in zip(O.get_hamiltonian(st, se), (S, D, Q, H), "SDQH"):
         assert np.abs(ref - got).max() < 1e-10, lab
     cr, gr, sr = O.get_hamiltonian_gradient(st, se, dse, vao, vdp, vqp, P, W)
     assert np.abs(cr - dEdcn).max() < 1e-8
     assert np.abs(gr - grad).max() < 1e-8
     assert np.abs(sr - sigma).max() < 1e-8
+    # tbx_h0_get_multipole_integrals against the oracle's full double loop
+    for ref, got in zip(O.get_multipole_integrals_lat(st), (S2, D2, Q2)):
+        assert np.abs(ref - got).max() < 1e-10
