@@ -139,14 +139,6 @@ __device__ __forceinline__ double mom(const Mom &M, int t)
     }
 }
 
-// (y + p)^n for n <= 2 as coefficients c[0..2]
-__device__ __forceinline__ void pow2_poly(int n, double p, double c[3])
-{
-    c[0] = (n == 0) ? 1.0 : ((n == 1) ? p : p * p);
-    c[1] = (n == 0) ? 0.0 : ((n == 1) ? 1.0 : 2.0 * p);
-    c[2] = (n == 2) ? 1.0 : 0.0;
-}
-
 // integral of sum_k P[k] y^k (y + c)^m, m = 0, 1, 2; N coefficients
 template <int N>
 __device__ __forceinline__ void mom3(const double *P, double c, const Mom &M, double &i0,
@@ -163,88 +155,6 @@ __device__ __forceinline__ void mom3(const double *P, double c, const Mom &M, do
     i0 = a0;
     i1 = a1;
     i2 = a2;
-}
-
-template <int N>
-__device__ __forceinline__ void mom12(const double *P, double c, const Mom &M, double &i1,
-                                      double &i2)
-{
-    double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        double mk = mom(M, k), mk1 = mom(M, k + 1), mk2 = mom(M, k + 2);
-        a1 += P[k] * (mk1 + c * mk);
-        a2 += P[k] * (mk2 + 2.0 * c * mk1 + c * c * mk);
-    }
-    i1 = a1;
-    i2 = a2;
-}
-
-// Forward 1D factors of one direction: V[m] = int (y+pa)^(a+m) (y+pb)^b, m=0..2
-__device__ __forceinline__ void fwd1d(int a, int b, double pa, double pb, const Mom &M,
-                                      double V[3])
-{
-    double A[3], B[3], P[5];
-    pow2_poly(a, pa, A);
-    pow2_poly(b, pb, B);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) P[k] = 0.0;
-#pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int y = 0; y < 3; ++y) P[x + y] += A[x] * B[y];
-    mom3<5>(P, pa, M, V[0], V[1], V[2]);
-}
-
-// Gradient 1D factors of one direction (derivative w.r.t. centre i, exponent ai):
-//   Vi[m] = int (y+pa)^(a+m) (y+pb)^b          (operator on i)
-//   Vj[m] = int (y+pa)^a (y+pb)^(b+m)          (operator on j), Vj[0] = Vi[0]
-//   Gi[m] = d/dA_x of Vi[m] (function and operator move with centre i)
-//   Gj[m] = d/dA_x of Vj[m]
-__device__ __forceinline__ void grad1d(int a, int b, double pa, double pb, double ai,
-                                       const Mom &M, double Vi[3], double Vj[3], double Gi[3],
-                                       double Gj[3])
-{
-    double A[3], B[3], P[5], G[4], PG[6];
-    pow2_poly(a, pa, A);
-    pow2_poly(b, pb, B);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) P[k] = 0.0;
-#pragma unroll
-    for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int y = 0; y < 3; ++y) P[x + y] += A[x] * B[y];
-    // G = d/dA [(x-A)^a e^{-ai (x-A)^2}] / e = 2 ai (y+pa)^(a+1) - a (y+pa)^(a-1)
-    double ta = 2.0 * ai, fa = (double)a;
-    double Am0 = (a == 2) ? pa : ((a == 1) ? 1.0 : 0.0);
-    double Am1 = (a == 2) ? 1.0 : 0.0;
-    G[0] = ta * (pa * A[0]) - fa * Am0;
-    G[1] = ta * (A[0] + pa * A[1]) - fa * Am1;
-    G[2] = ta * (A[1] + pa * A[2]);
-    G[3] = ta * A[2];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) PG[k] = 0.0;
-#pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 3; ++y) PG[x + y] += G[x] * B[y];
-    mom3<5>(P, pa, M, Vi[0], Vi[1], Vi[2]);
-    Vj[0] = Vi[0];
-    mom12<5>(P, pb, M, Vj[1], Vj[2]);
-    double g1, g2;
-    mom3<6>(PG, pa, M, Gi[0], g1, g2);
-    // the operator (x - A)^m moves with A as well: - m (x-A)^(m-1)
-    Gi[1] = g1 - Vi[0];
-    Gi[2] = g2 - 2.0 * Vi[1];
-    Gj[0] = Gi[0];
-    mom12<6>(PG, pb, M, Gj[1], Gj[2]);
-}
-
-__device__ __forceinline__ void unpack_exp(int e, int x[3])
-{
-    x[0] = e & 15;
-    x[1] = (e >> 4) & 15;
-    x[2] = (e >> 8) & 15;
 }
 
 // quadrupole component q -> moment exponent per direction (xx xy yy xz yz zz)
