@@ -9,6 +9,8 @@ import pytest
 from oracle import oracle as O
 from tblite_amd import h0, lib as L, structure, synthetic
 
+TBX_ERR_ARG, TBX_ERR_BASIS = 1, 2  # include/tbx.h
+
 
 def test_library_exports_header_symbols():
     names = L.header_functions()
@@ -114,3 +116,36 @@ def test_fortran_driver_reads_input_and_reports_missing_device(tmp_path):
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 2, r.stdout + r.stderr
     assert "no context" in r.stdout and len(r.stdout.strip()) > len("tbx_h0_driver: no context:")
+
+
+def test_full_builders_validate_arguments():
+    """get_multipole_integrals / get_dipole_integrals / get_overlap reject a
+    missing basis or output before any device work."""
+    z, x = synthetic.octane()
+    st = structure.setup(structure.Structure(z, x))
+    a = h0._Args(st)
+    nao = st.bas.nao
+    S = np.zeros((nao, nao), order="F")
+    so = L.lib()
+    args = (ct.byref(a.geom), None, h0._p(a.ish_at), h0._p(a.iao_sh), nao)
+    assert so.tbx_get_multipole_integrals(None, *args, h0._p(S), None, None) == TBX_ERR_BASIS
+    assert so.tbx_get_overlap(None, ct.byref(a.geom), ct.byref(a.basis), h0._p(a.ish_at),
+                              h0._p(a.iao_sh), nao, None) == TBX_ERR_ARG
+    assert so.tbx_get_dipole_integrals(None, ct.byref(a.geom), ct.byref(a.basis),
+                                       h0._p(a.ish_at), h0._p(a.iao_sh), nao, h0._p(S),
+                                       None) == TBX_ERR_ARG
+    # a null context is reported, not dereferenced
+    assert so.tbx_get_overlap(None, ct.byref(a.geom), ct.byref(a.basis), h0._p(a.ish_at),
+                              h0._p(a.iao_sh), nao, h0._p(S)) == TBX_ERR_ARG
+    assert b"context" in so.tbx_last_error()
+
+
+def test_scf_consumers_reject_null_batch():
+    so = L.lib()
+    buf = (ct.c_double * 8)()
+    assert so.tbx_batch_add_vao_to_h1(None, 1, buf, buf, buf, None) == TBX_ERR_ARG
+    assert so.tbx_batch_add_vmp_to_h1(None, 1, 3, buf, 3, buf, buf, None) == TBX_ERR_ARG
+    assert so.tbx_batch_mulliken_shell_charges(None, 1, buf, buf, buf, buf, None) == TBX_ERR_ARG
+    assert so.tbx_batch_mulliken_atomic_multipoles(None, 1, 6, buf, buf, buf,
+                                                   None) == TBX_ERR_ARG
+    assert so.tbx_batch_last_scf_timing(None, buf) == TBX_ERR_ARG
