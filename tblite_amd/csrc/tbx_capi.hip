@@ -530,64 +530,86 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
     return TBX_OK;
 }
 
-// Device copies of the tables
+// Host staging of many small tables for one host-to-device copy: every
+// table starts on a 256-byte boundary of the arena.
+struct Arena {
+    std::vector<char> host;
+    template <class T> size_t add(const std::vector<T> &v)
+    {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + sizeof(T) * std::max<size_t>(v.size(), 1), 0);
+        if (!v.empty()) std::memcpy(host.data() + off, v.data(), sizeof(T) * v.size());
+        return off;
+    }
+};
+
+// Device copies of the tables: one buffer for the species tables, one for
+// the molecule and work-unit tables.  The host arenas stay alive with them.
 struct DevTables {
-    DevBuf nsh, ang, nprim, alpha, coeff, shpoly, rad, hscale, ncart, nao, cart_sh, cart_exp, ao_sh,
-        ao_m, sh_cart, sh_ao, prim, prim_off, pp, pp_off, pp_mu, ao_info, ao_desc;
-    DevBuf trans, mol_atom, mol_nao, mol_cart, mol_dense, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, mol_ao, uhead, uplan, fhead, fplan, sh_aog, tiles;
+    DevBuf species, mols;
+    Arena sp_host, mol_host;
     TbxSpecies sp{};
     TbxMols mt{};
 
     void release()
     {
-        DevBuf *all[] = {&nsh, &ang, &nprim, &alpha, &coeff, &shpoly, &rad, &hscale, &ncart, &nao,
-                         &cart_sh, &cart_exp, &ao_sh, &ao_m, &sh_cart, &sh_ao, &prim, &prim_off,
-                         &pp, &pp_off, &pp_mu, &ao_info, &ao_desc,
-                         &trans, &mol_atom,
-                         &mol_nao, &mol_cart, &mol_dense, &at_sp, &at_mol, &at_sh, &at_ao,
-                         &at_cart, &cart_atom, &ao_atom, &mol_ao, &uhead, &uplan,
-                         &fhead, &fplan, &sh_aog, &tiles};
-        for (DevBuf *b : all) b->release();
+        species.release();
+        mols.release();
     }
 };
 
+static hipError_t upload_arena(DevBuf &d, const Arena &a, hipStream_t s)
+{
+    hipError_t e = d.reserve(a.host.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(d.p, a.host.data(), a.host.size(), hipMemcpyHostToDevice, s);
+}
+
 static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cutoff, hipStream_t s)
 {
-    hipError_t e;
-#define UP(name) if ((e = upload(d.name, hs.name, s)) != hipSuccess) return e
-    UP(nsh); UP(ang); UP(nprim); UP(alpha); UP(coeff); UP(shpoly); UP(rad); UP(hscale); UP(ncart);
-    UP(nao); UP(cart_sh); UP(cart_exp); UP(ao_sh); UP(ao_m); UP(sh_cart); UP(sh_ao); UP(prim);
-    UP(prim_off); UP(pp); UP(pp_off); UP(pp_mu); UP(ao_info); UP(ao_desc);
-#undef UP
+    Arena &a = d.sp_host;
+    a.host.clear();
+    const size_t o_nsh = a.add(hs.nsh), o_ang = a.add(hs.ang), o_nprim = a.add(hs.nprim),
+                 o_alpha = a.add(hs.alpha), o_coeff = a.add(hs.coeff), o_shpoly = a.add(hs.shpoly),
+                 o_rad = a.add(hs.rad), o_hscale = a.add(hs.hscale), o_ncart = a.add(hs.ncart),
+                 o_nao = a.add(hs.nao), o_cart_sh = a.add(hs.cart_sh),
+                 o_cart_exp = a.add(hs.cart_exp), o_ao_sh = a.add(hs.ao_sh), o_ao_m = a.add(hs.ao_m),
+                 o_sh_cart = a.add(hs.sh_cart), o_sh_ao = a.add(hs.sh_ao), o_prim = a.add(hs.prim),
+                 o_prim_off = a.add(hs.prim_off), o_pp = a.add(hs.pp), o_pp_off = a.add(hs.pp_off),
+                 o_pp_mu = a.add(hs.pp_mu), o_ao_info = a.add(hs.ao_info),
+                 o_ao_desc = a.add(hs.ao_desc);
+    const hipError_t e = upload_arena(d.species, a, s);
+    if (e != hipSuccess) return e;
+    const char *base = static_cast<const char *>(d.species.p);
+    auto at = [base](size_t off) { return base + off; };
     TbxSpecies &sp = d.sp;
     sp.nsp = hs.nsp;
     sp.msh = hs.msh;
-    sp.nsh = d.nsh.as<int>();
-    sp.ang = d.ang.as<int>();
-    sp.nprim = d.nprim.as<int>();
-    sp.alpha = d.alpha.as<double>();
-    sp.coeff = d.coeff.as<double>();
-    sp.shpoly = d.shpoly.as<double>();
-    sp.rad = d.rad.as<double>();
-    sp.hscale = d.hscale.as<double>();
-    sp.ncart = d.ncart.as<int>();
-    sp.nao = d.nao.as<int>();
-    sp.cart_sh = d.cart_sh.as<int>();
-    sp.cart_exp = d.cart_exp.as<int>();
-    sp.ao_sh = d.ao_sh.as<int>();
-    sp.ao_m = d.ao_m.as<int>();
-    sp.sh_cart = d.sh_cart.as<int>();
-    sp.sh_ao = d.sh_ao.as<int>();
-    sp.prim = d.prim.as<double>();
-    sp.prim_off = d.prim_off.as<int>();
+    sp.nsh = reinterpret_cast<const int *>(at(o_nsh));
+    sp.ang = reinterpret_cast<const int *>(at(o_ang));
+    sp.nprim = reinterpret_cast<const int *>(at(o_nprim));
+    sp.alpha = reinterpret_cast<const double *>(at(o_alpha));
+    sp.coeff = reinterpret_cast<const double *>(at(o_coeff));
+    sp.shpoly = reinterpret_cast<const double *>(at(o_shpoly));
+    sp.rad = reinterpret_cast<const double *>(at(o_rad));
+    sp.hscale = reinterpret_cast<const double *>(at(o_hscale));
+    sp.ncart = reinterpret_cast<const int *>(at(o_ncart));
+    sp.nao = reinterpret_cast<const int *>(at(o_nao));
+    sp.cart_sh = reinterpret_cast<const int *>(at(o_cart_sh));
+    sp.cart_exp = reinterpret_cast<const int *>(at(o_cart_exp));
+    sp.ao_sh = reinterpret_cast<const int *>(at(o_ao_sh));
+    sp.ao_m = reinterpret_cast<const int *>(at(o_ao_m));
+    sp.sh_cart = reinterpret_cast<const int *>(at(o_sh_cart));
+    sp.sh_ao = reinterpret_cast<const int *>(at(o_sh_ao));
+    sp.prim = reinterpret_cast<const double *>(at(o_prim));
+    sp.prim_off = reinterpret_cast<const int *>(at(o_prim_off));
     sp.nprim_tot = (int)(hs.prim.size() / 2);
     sp.nslot = hs.nsp * hs.msh;
-    sp.pp_off = d.pp_off.as<int>();
-    sp.pp = d.pp.as<double>();
-    sp.pp_mu = d.pp_mu.as<double>();
-    sp.ao_info = d.ao_info.as<int>();
-    sp.ao_desc = d.ao_desc.as<int>();
+    sp.pp_off = reinterpret_cast<const int *>(at(o_pp_off));
+    sp.pp = reinterpret_cast<const double *>(at(o_pp));
+    sp.pp_mu = reinterpret_cast<const double *>(at(o_pp_mu));
+    sp.ao_info = reinterpret_cast<const int *>(at(o_ao_info));
+    sp.ao_desc = reinterpret_cast<const int *>(at(o_ao_desc));
     sp.intcut = hs.intcut;
     sp.cutoff2 = cutoff * cutoff;
     return hipSuccess;
@@ -596,41 +618,48 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
 static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vector<double> &trans,
                               hipStream_t s)
 {
-    hipError_t e;
-#define UP(name) if ((e = upload(d.name, hm.name, s)) != hipSuccess) return e
-    UP(mol_atom); UP(mol_nao); UP(mol_cart); UP(mol_dense); UP(at_sp); UP(at_mol); UP(at_sh);
-    UP(at_ao); UP(at_cart); UP(cart_atom); UP(ao_atom); UP(mol_ao); UP(uhead); UP(uplan);
-    UP(fhead); UP(fplan); UP(sh_aog); UP(tiles);
-#undef UP
-    if ((e = upload(d.trans, trans, s)) != hipSuccess) return e;
+    Arena &a = d.mol_host;
+    a.host.clear();
+    const size_t o_trans = a.add(trans), o_mol_atom = a.add(hm.mol_atom),
+                 o_mol_nao = a.add(hm.mol_nao), o_mol_cart = a.add(hm.mol_cart),
+                 o_mol_dense = a.add(hm.mol_dense), o_at_sp = a.add(hm.at_sp),
+                 o_at_mol = a.add(hm.at_mol), o_at_sh = a.add(hm.at_sh), o_at_ao = a.add(hm.at_ao),
+                 o_at_cart = a.add(hm.at_cart), o_cart_atom = a.add(hm.cart_atom),
+                 o_ao_atom = a.add(hm.ao_atom), o_mol_ao = a.add(hm.mol_ao),
+                 o_uhead = a.add(hm.uhead), o_uplan = a.add(hm.uplan), o_fhead = a.add(hm.fhead),
+                 o_fplan = a.add(hm.fplan), o_sh_aog = a.add(hm.sh_aog), o_tiles = a.add(hm.tiles);
+    const hipError_t e = upload_arena(d.mols, a, s);
+    if (e != hipSuccess) return e;
+    const char *base = static_cast<const char *>(d.mols.p);
+    auto at = [base](size_t off) { return base + off; };
     TbxMols &mt = d.mt;
     mt.nmol = hm.nmol;
     mt.nat = hm.nat;
     mt.ntrans = (int)(trans.size() / 3);
-    mt.trans = d.trans.as<double>();
-    mt.mol_atom = d.mol_atom.as<int>();
-    mt.mol_nao = d.mol_nao.as<int>();
-    mt.mol_cart = d.mol_cart.as<int>();
-    mt.mol_dense = d.mol_dense.as<long long>();
-    mt.at_sp = d.at_sp.as<int>();
-    mt.at_mol = d.at_mol.as<int>();
-    mt.at_sh = d.at_sh.as<int>();
-    mt.at_ao = d.at_ao.as<int>();
-    mt.at_cart = d.at_cart.as<int>();
-    mt.cart_atom = d.cart_atom.as<int>();
-    mt.ao_atom = d.ao_atom.as<int>();
-    mt.mol_ao = d.mol_ao.as<int>();
+    mt.trans = reinterpret_cast<const double *>(at(o_trans));
+    mt.mol_atom = reinterpret_cast<const int *>(at(o_mol_atom));
+    mt.mol_nao = reinterpret_cast<const int *>(at(o_mol_nao));
+    mt.mol_cart = reinterpret_cast<const int *>(at(o_mol_cart));
+    mt.mol_dense = reinterpret_cast<const long long *>(at(o_mol_dense));
+    mt.at_sp = reinterpret_cast<const int *>(at(o_at_sp));
+    mt.at_mol = reinterpret_cast<const int *>(at(o_at_mol));
+    mt.at_sh = reinterpret_cast<const int *>(at(o_at_sh));
+    mt.at_ao = reinterpret_cast<const int *>(at(o_at_ao));
+    mt.at_cart = reinterpret_cast<const int *>(at(o_at_cart));
+    mt.cart_atom = reinterpret_cast<const int *>(at(o_cart_atom));
+    mt.ao_atom = reinterpret_cast<const int *>(at(o_ao_atom));
+    mt.mol_ao = reinterpret_cast<const int *>(at(o_mol_ao));
     mt.nunit = (int)hm.uhead.size();
-    mt.unit = d.uhead.as<TbxUnitHead>();
-    mt.plan = d.uplan.as<int>();
+    mt.unit = reinterpret_cast<const TbxUnitHead *>(at(o_uhead));
+    mt.plan = reinterpret_cast<const int *>(at(o_uplan));
     mt.nfunit = (int)hm.fhead.size();
-    mt.funit = d.fhead.as<TbxUnitHead>();
-    mt.fplan = d.fplan.as<int>();
+    mt.funit = reinterpret_cast<const TbxUnitHead *>(at(o_fhead));
+    mt.fplan = reinterpret_cast<const int *>(at(o_fplan));
     mt.nsh_tot = hm.nsh;
     mt.nao_tot = hm.nao;
-    mt.sh_aog = d.sh_aog.as<int>();
+    mt.sh_aog = reinterpret_cast<const int *>(at(o_sh_aog));
     mt.ntile = (int)(hm.tiles.size() / 4);
-    mt.tiles = d.tiles.as<int>();
+    mt.tiles = reinterpret_cast<const int *>(at(o_tiles));
     return hipSuccess;
 }
 
@@ -642,7 +671,42 @@ struct tbx_context {
     DevTables tab;
     DevBuf xyz, se, dsedcn, S, D, Q, H, P, W, vao, vdp, vqp, grad, dse, sigma, dEdcn, nnl, flag;
     std::vector<double> host_tmp;
+    // species tables of the last call (host copy, and the raw inputs they were
+    // built from): calls with the same basis, H0 parameters and cutoff reuse
+    // them and their device copy
+    HostSpecies hs;
+    std::vector<char> hs_raw;
+    bool hs_valid = false;
 };
+
+// The raw bytes of a basis + H0 parameter set + cutoff (empty if a table is
+// missing), to recognise a repeated parametrisation exactly.
+static std::vector<char> species_bytes(const tbx_basis *bas, const tbx_h0par *h0, double cutoff)
+{
+    std::vector<char> r;
+    if (!bas || !h0 || bas->nid <= 0 || bas->msh <= 0 || bas->msh > TBX_MAXSH || !bas->nsh_id ||
+        !bas->ang || !bas->nprim || !bas->alpha || !bas->coeff || !h0->hscale || !h0->shpoly ||
+        !h0->rad)
+        return r;
+    const size_t nsp = bas->nid, msh = bas->msh;
+    auto put = [&r](const void *p, size_t n) {
+        const char *c = static_cast<const char *>(p);
+        r.insert(r.end(), c, c + n);
+    };
+    put(&bas->nid, sizeof(int));
+    put(&bas->msh, sizeof(int));
+    put(&bas->intcut, sizeof(double));
+    put(&cutoff, sizeof(double));
+    put(bas->nsh_id, sizeof(int) * nsp);
+    put(bas->ang, sizeof(int) * nsp * msh);
+    put(bas->nprim, sizeof(int) * nsp * msh);
+    put(bas->alpha, sizeof(double) * TBX_MAXG * nsp * msh);
+    put(bas->coeff, sizeof(double) * TBX_MAXG * nsp * msh);
+    put(h0->hscale, sizeof(double) * msh * msh * nsp * nsp);
+    put(h0->shpoly, sizeof(double) * msh * nsp);
+    put(h0->rad, sizeof(double) * nsp);
+    return r;
+}
 
 struct tbx_batch {
     tbx_context *ctx = nullptr;
@@ -729,8 +793,16 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
         return fail(TBX_ERR_ARG, "invalid cutoff %g", geom->cutoff);
     for (int k = 0; k < 3 * geom->nat; ++k)
         if (!std::isfinite(geom->xyz[k])) return fail(TBX_ERR_ARG, "non-finite coordinate %d", k);
-    int st = build_species(bas, h0, hs);
-    if (st) return st;
+    // species tables: rebuilt and uploaded only when the parametrisation changes
+    std::vector<char> raw = species_bytes(bas, h0, geom->cutoff);
+    const bool reuse = ctx->hs_valid && !raw.empty() && raw == ctx->hs_raw;
+    int st = TBX_OK;
+    if (reuse) {
+        hs = ctx->hs;
+    } else {
+        ctx->hs_valid = false;
+        if ((st = build_species(bas, h0, hs))) return st;
+    }
     st = build_mols(hs, 1, &geom->nat, geom->id, hm);
     if (st) return st;
     if (hm.nao != nao) return fail(TBX_ERR_ARG, "nao=%d but the basis gives %d", nao, hm.nao);
@@ -748,7 +820,12 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
     std::vector<double> trans(geom->trans, geom->trans + 3 * geom->ntrans);
     hipStream_t s = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
+    if (!reuse) {
+        HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
+        ctx->hs = hs;
+        ctx->hs_raw = std::move(raw);
+        ctx->hs_valid = true;
+    }
     if ((st = build_units(hs, hm, geom->ntrans, kDefaultTiles))) return st;
     HIP_TRY(upload_mols(ctx->tab, hm, trans, s));
     HIP_TRY(ctx->xyz.reserve(sizeof(double) * 3 * geom->nat));
@@ -954,6 +1031,8 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     if (e == hipSuccess) e = hipMemsetAsync(b->flag.p, 0, sizeof(int), ctx->stream);
     for (int k = 0; k < 16 && e == hipSuccess; ++k) e = hipEventCreate(&b->ev[k]);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    b->tab.sp_host.host = std::vector<char>();  // copies done: free the host staging
+    b->tab.mol_host.host = std::vector<char>();
     if (e != hipSuccess) {
         tbx_batch_delete(b);
         return fail(TBX_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e));
@@ -976,6 +1055,7 @@ extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));  // no launch may still read the old units
     HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    b->tab.mol_host.host = std::vector<char>();
     return TBX_OK;
 }
 
