@@ -746,12 +746,28 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
                 qv[5] += 1.5 * sq[5] - ts;
                 for (int c = 0; c < 3; ++c) d[c] += vv[c] * sv;
             }
-            put(A.S + idx, sv, accum);
-            put(A.H + idx, hfac[spx] * sv, accum);
+            if (accum) {
+                put(A.S + idx, sv, true);
+                put(A.H + idx, hfac[spx] * sv, true);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) put(A.D + 3 * idx + c, d[c], accum);
+                for (int c = 0; c < 3; ++c) put(A.D + 3 * idx + c, d[c], true);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) put(A.Q + 6 * idx + c, qv[c], accum);
+                for (int c = 0; c < 6; ++c) put(A.Q + 6 * idx + c, qv[c], true);
+            } else {
+                A.S[idx] = sv;
+                A.H[idx] = hfac[spx] * sv;
+                // the 3 dipole components: one 16-byte store (aligned for either
+                // parity of idx) and one 8-byte store
+                const bool odd = (idx & 1) != 0;
+                *reinterpret_cast<double2 *>(A.D + 3 * idx + (odd ? 1 : 0)) =
+                    odd ? make_double2(d[1], d[2]) : make_double2(d[0], d[1]);
+                A.D[3 * idx + (odd ? 0 : 2)] = odd ? d[0] : d[2];
+                // the 6 quadrupole components of an element start on a 16-byte boundary
+                double2 *q2 = reinterpret_cast<double2 *>(A.Q + 6 * idx);
+                q2[0] = make_double2(qv[0], qv[1]);
+                q2[1] = make_double2(qv[2], qv[3]);
+                q2[2] = make_double2(qv[4], qv[5]);
+            }
         }
     }
 }
