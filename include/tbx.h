@@ -167,7 +167,9 @@ int tbx_batch_offsets(const tbx_batch *b, int *atom_off, int *sh_off, int *ao_of
  * device pointers.  Neighbours are the pairs within desc->cutoff, decided on
  * the device from d_xyz (no host neighbour list).  `stream` is the HIP
  * stream to launch on; NULL is the default (null) stream.  All tbx_batch_*
- * launches are asynchronous and allocate nothing on the device. */
+ * launches are asynchronous and allocate nothing on the device.  The dipole
+ * and quadrupole outputs d_D, d_Q (and d_mpint of the SCF-loop consumers)
+ * must be 16-byte aligned (hipMalloc and torch allocations are). */
 int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double *d_se, double *d_S,
                       double *d_D, double *d_Q, double *d_H, void *stream);
 

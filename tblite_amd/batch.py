@@ -241,7 +241,7 @@ class H0Integrals(torch.autograd.Function):
         def dense(t, n):
             if t is None:
                 return torch.zeros(n, dtype=torch.float64, device=dev)
-            return t.contiguous()
+            return t.to(torch.float64).contiguous()
 
         dxyz, dse = b.adjoint(xyz.detach(), se.detach(), dense(Sb, b.ndense),
                               dense(Db, 3 * b.ndense), dense(Qb, 6 * b.ndense),

@@ -10,6 +10,7 @@
 #include <array>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <climits>
 #include <cstring>
@@ -1121,10 +1122,15 @@ static hipError_t mark(tbx_batch *b, int k, hipStream_t s)
     return hipEventRecord(b->ev[k], s);
 }
 
+// The kernels move dipole and quadrupole components with 16-byte accesses.
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double *d_se,
                                  double *d_S, double *d_D, double *d_Q, double *d_H, void *stream)
 {
     if (!b || !d_xyz || !d_se || !d_S || !d_D || !d_Q || !d_H) return fail(TBX_ERR_ARG, "null argument");
+    if (!aligned16(d_D) || !aligned16(d_Q))
+        return fail(TBX_ERR_ARG, "d_D and d_Q must be 16-byte aligned");
     hipStream_t s = pick_stream(b, stream);
     HIP_TRY(mark(b, 0, s));
     HIP_TRY(tbx_launch_forward(&b->tab.sp, &b->tab.mt, d_xyz, d_se, d_S, d_D, d_Q, d_H,
@@ -1199,6 +1205,7 @@ static int scf_launch(tbx_batch *b, int what, int nspin, int mpdim, int vdim, co
 {
     if (nspin < 1 || nspin > 2) return fail(TBX_ERR_ARG, "nspin=%d", nspin);
     if (mpdim < 1 || mpdim > 6 || vdim < 1) return fail(TBX_ERR_ARG, "mpdim=%d vdim=%d", mpdim, vdim);
+    if (mp && !aligned16(mp)) return fail(TBX_ERR_ARG, "d_mpint must be 16-byte aligned");
     hipStream_t s = pick_stream(b, stream);
     HIP_TRY(mark(b, 8 + 2 * what, s));
     HIP_TRY(tbx_launch_scf(what, &b->tab.mt, nspin, b->hm.ndense, mpdim, vdim, S, mp, P, vao, vmp,

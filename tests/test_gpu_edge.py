@@ -114,3 +114,20 @@ def test_edge_dropin(k):
         assert np.abs(r - g).max() < TOL_INT
     for r, g in zip(O.get_multipole_integrals_lat(st), h0.get_multipole_integrals(st)):
         assert np.abs(r - g).max() < TOL_INT
+
+
+def test_misaligned_multipole_output_is_rejected(edge_case):
+    """The forward stores dipoles and quadrupoles with 16-byte accesses: an
+    output that is not 16-byte aligned is an argument error, and nothing is
+    launched."""
+    from tblite_amd import lib as L
+    mols, b, cn, se, dse = edge_case
+    dev = b.torch_device
+    S = torch.empty(b.ndense, dtype=torch.float64, device=dev)
+    H = torch.empty_like(S)
+    Dbuf = torch.empty(3 * b.ndense + 1, dtype=torch.float64, device=dev)
+    Q = torch.empty(6 * b.ndense, dtype=torch.float64, device=dev)
+    D = Dbuf[1:]
+    assert D.data_ptr() % 16 == 8
+    with pytest.raises(L.TbxError):
+        b.forward(b.tensor_xyz(), se, out=(S, D, Q, H))
