@@ -604,7 +604,7 @@ __device__ __forceinline__ void put(double *p, double v, bool acc)
 __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 {
     __shared__ UnitStage U;
-    __shared__ double acc[10][TBX_UNIT_TASKS];
+    __shared__ __align__(16) double acc[TBX_UNIT_TASKS][10];  // S, D (3), Q (6) per task
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12 | pair << 20
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         default: fchunk_run<2, 2>(sp.pp, vec, ki, kj, p0, p1, o); break;
         }
 #pragma unroll
-        for (int k = 0; k < 10; ++k) atomicAdd(&acc[k][tk], o[k]);
+        for (int k = 0; k < 10; ++k) atomicAdd(&acc[tk][k], o[k]);
     }
 
     // (5) one lane per dense element (descriptors from the plan): block A
@@ -713,8 +713,13 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
                 for (int b = 0; b < nsi; ++b) {
                     const double w = wa * sph_w(li, mi, b);
                     const int tk = task0 + sph_c(li, mi, b) * ncj + cj;
+                    const double2 *a2 = reinterpret_cast<const double2 *>(acc[tk]);
 #pragma unroll
-                    for (int q = 0; q < 10; ++q) v[q] += w * acc[q][tk];
+                    for (int q = 0; q < 5; ++q) {
+                        const double2 x = a2[q];
+                        v[2 * q] += w * x.x;
+                        v[2 * q + 1] += w * x.y;
+                    }
                 }
             }
             const double sv = v[0];
