@@ -226,10 +226,11 @@ int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pai
 
 /* Tile shapes of the batched kernels, no counterpart in the reference: the
  * half list of every molecule is cut into work units of at most
- * *_unit_tasks Cartesian-pair tasks (<= 256, >= the largest atom pair) and
- * *_unit_pairs atom pairs (<= 128); kept primitive pairs are processed in
- * chunks of *_chunk (1..64) per work item.  Defaults 256, 128 and chunks of
- * 3 (forward) and 1 (gradient).
+ * *_unit_tasks Cartesian-pair tasks (>= the largest atom pair; <= 256 for
+ * the forward, <= 128 for the gradient) and *_unit_pairs atom pairs (<= 128
+ * and <= 64); kept primitive pairs are processed in chunks of *_chunk
+ * (1..64) per work item.  Defaults: forward 256, 128, chunk 3; gradient
+ * 128, 64, chunk 1.
  * tbx_batch_set_tiles rebuilds the units (host) and waits for the context's
  * stream; tblite_amd/tune.py chooses them by timing the kernels. */
 typedef struct {

@@ -301,12 +301,12 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
 }
 
 // Plans of a list of units (layout: TbxUnitHead).  The units respect the
-// caps of TBX_UNIT_TASKS, TBX_UNIT_PAIRS and TBX_UNIT_SEGS by construction.
-// Returns false if a unit breaks the caps or its plan does not have the
-// length the kernels read (load_unit).
+// caps max_tasks, max_pairs (those of the kernel that runs them) and
+// TBX_UNIT_SEGS by construction.  Returns false if a unit breaks the caps or
+// its plan does not have the length the kernels read (load_unit).
 static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::vector<TbxSeg> &seg,
-                        const std::vector<int> &unit_seg, bool with_elems,
-                        std::vector<TbxUnitHead> &head, std::vector<int> &plan)
+                        const std::vector<int> &unit_seg, bool with_elems, int max_tasks,
+                        int max_pairs, std::vector<TbxUnitHead> &head, std::vector<int> &plan)
 {
     const int nunit = (int)unit_seg.size() - 1;
     head.assign(nunit, TbxUnitHead{});
@@ -391,8 +391,8 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
         plan.insert(plan.end(), tsp, tsp + 10);
         const int nspair = (int)spl.size();
         const long long words = (10 * npair + 3) / 4 + (npair + 3) / 4 + nseg + (nspair + 3) / 4 + 5;
-        if (npair < 1 || npair > TBX_UNIT_PAIRS || nseg > TBX_UNIT_SEGS ||
-            nspair > TBX_UNIT_TASKS || ntask > TBX_UNIT_TASKS ||
+        if (npair < 1 || npair > max_pairs || nseg > TBX_UNIT_SEGS || nspair > max_tasks ||
+            ntask > max_tasks ||
             (long long)plan.size() != 4 * ((long long)h.plan + words))
             return false;
         h.nel = 0;
@@ -426,7 +426,7 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
                     }
                 }
             h.nel = (int)(plan.size() - 4 * ((size_t)h.plan + words));
-            if (h.nel > 2 * TBX_UNIT_TASKS) return false;  // two elements per lane
+            if (h.nel > 2 * max_tasks) return false;  // two elements per lane
             pad4();
         }
         if (plan.size() / 4 > (size_t)INT_MAX) return false;
@@ -440,15 +440,17 @@ static int build_units(const HostSpecies &hs, HostMols &hm, int ntrans, const tb
                      hm.unit_seg);
     build_units_list(hs, hm, ntrans, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg,
                      hm.funit_seg);
-    if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, hm.uhead, hm.uplan) ||
-        !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, hm.fhead, hm.fplan))
+    if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, TBX_GRAD_UNIT_TASKS,
+                     TBX_GRAD_UNIT_PAIRS, hm.uhead, hm.uplan) ||
+        !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, TBX_UNIT_TASKS, TBX_UNIT_PAIRS,
+                     hm.fhead, hm.fplan))
         return fail(TBX_ERR_ARG, "work-unit plan outside the kernel limits");
     return TBX_OK;
 }
 
 // defaults: the autotuner's choice on the 2048-molecule batch
 static const tbx_tiles kDefaultTiles = {TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 3,
-                                        TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 1};
+                                        TBX_GRAD_UNIT_TASKS, TBX_GRAD_UNIT_PAIRS, 1};
 
 // Tile shapes must fit the kernels' LDS arrays and hold the largest atom pair.
 static int check_tiles(const HostSpecies &hs, const tbx_tiles *t)
@@ -457,12 +459,14 @@ static int check_tiles(const HostSpecies &hs, const tbx_tiles *t)
     const int tasks[2] = {t->fwd_unit_tasks, t->grad_unit_tasks};
     const int pairs[2] = {t->fwd_unit_pairs, t->grad_unit_pairs};
     const int chunk[2] = {t->fwd_chunk, t->grad_chunk};
+    const int max_tasks[2] = {TBX_UNIT_TASKS, TBX_GRAD_UNIT_TASKS};
+    const int max_pairs[2] = {TBX_UNIT_PAIRS, TBX_GRAD_UNIT_PAIRS};
     for (int k = 0; k < 2; ++k) {
-        if (tasks[k] < need || tasks[k] > TBX_UNIT_TASKS)
+        if (tasks[k] < need || tasks[k] > max_tasks[k])
             return fail(TBX_ERR_ARG, "unit tasks %d outside %d..%d", tasks[k], need,
-                        TBX_UNIT_TASKS);
-        if (pairs[k] < 1 || pairs[k] > TBX_UNIT_PAIRS)
-            return fail(TBX_ERR_ARG, "unit pairs %d outside 1..%d", pairs[k], TBX_UNIT_PAIRS);
+                        max_tasks[k]);
+        if (pairs[k] < 1 || pairs[k] > max_pairs[k])
+            return fail(TBX_ERR_ARG, "unit pairs %d outside 1..%d", pairs[k], max_pairs[k]);
         if (chunk[k] < 1 || chunk[k] > 64)
             return fail(TBX_ERR_ARG, "chunk %d outside 1..64", chunk[k]);
     }

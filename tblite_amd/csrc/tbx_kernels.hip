@@ -30,7 +30,9 @@
 #include "tbx_tables.h"
 
 #define MAXG 12
-#define NTHREADS 256
+// Workgroup sizes: one lane per Cartesian-pair task of a work unit
+#define NTHREADS TBX_UNIT_TASKS        // forward
+#define GTHREADS TBX_GRAD_UNIT_TASKS   // gradient
 #define TW_FWD 512   // Cartesian-pair tasks per forward window (LDS: 11 doubles each)
 #define TW_GRAD 256  // Cartesian-pair tasks per gradient window
 #define MAXSH 4
@@ -370,16 +372,17 @@ __device__ __forceinline__ int find_last_le(const int *off, int n, int c)
     return lo;
 }
 
-// Work item -> task: last index k in [0, NTHREADS) with off[k] <= c (off
+// Work item -> task: last index k in [0, NT) with off[k] <= c (off
 // non-decreasing, off[0] = 0 <= c), in two rounds of independent LDS reads
 // (the block of 16 entries, then the entry within it) instead of eight
 // dependent ones.  off must be 16-byte aligned.
+template <int NT>
 __device__ __forceinline__ int find_task(const int *off, int c)
 {
-    static_assert(NTHREADS == 256, "two-level search over 16 x 16 entries");
+    static_assert(NT % 16 == 0 && NT <= 256, "two-level search over blocks of 16");
     int b = 0;
 #pragma unroll
-    for (int k = 1; k < 16; ++k) b += (off[16 * k] <= c) ? 1 : 0;
+    for (int k = 1; k < NT / 16; ++k) b += (off[16 * k] <= c) ? 1 : 0;
     const int4 *blk = reinterpret_cast<const int4 *>(off + 16 * b);
     int t = 0;
 #pragma unroll
@@ -391,23 +394,32 @@ __device__ __forceinline__ int find_task(const int *off, int c)
 }
 
 // Unit staging shared by the forward and gradient kernels (LDS): the unit's
-// plan (TbxUnitHead) and the pair vectors.
-struct __align__(16) UnitStage {
-    PairInfo pinf[TBX_UNIT_PAIRS];
-    int pext[TBX_UNIT_PAIRS];       // lattice vector << 5 | segment
+// plan (TbxUnitHead) and the pair vectors, for units of at most NT tasks
+// (= workgroup size) and NP atom pairs.
+template <int NT, int NP>
+struct __align__(16) UnitStageT {
+    static constexpr int kThreads = NT;
+    PairInfo pinf[NP];
+    int pext[NP];                   // lattice vector << 5 | segment
     int4 sinfo[TBX_UNIT_SEGS];      // first pair, first row AO, rows | mirrored rows << 16,
                                     // AOs of atom i
-    int spl[TBX_UNIT_TASKS];        // shell pairs: pair | jsh << 8 | ish << 12 | type << 16
+    int spl[NT];                    // shell pairs: pair | jsh << 8 | ish << 12 | type << 16
                                     // (type-major)
     int toff[10], tsp[10];          // task / shell-pair offsets per type
-    double pv[TBX_UNIT_PAIRS][3];   // v = R_i - R_j - T
-    int wsum[NTHREADS / 64][9];
+    double pv[NP][3];               // v = R_i - R_j - T
+    int wsum[NT / 64][9];
 };
+using FwdStage = UnitStageT<TBX_UNIT_TASKS, TBX_UNIT_PAIRS>;
+using GradStage = UnitStageT<TBX_GRAD_UNIT_TASKS, TBX_GRAD_UNIT_PAIRS>;
 static_assert(sizeof(PairInfo) == 40, "pair record layout");
-static_assert(offsetof(UnitStage, pext) % 16 == 0 && offsetof(UnitStage, sinfo) % 16 == 0 &&
-                  offsetof(UnitStage, spl) % 16 == 0 && offsetof(UnitStage, toff) % 16 == 0 &&
-                  offsetof(UnitStage, tsp) == offsetof(UnitStage, toff) + 40,
-              "plan sections are copied as 16-byte words");
+#define TBX_STAGE_LAYOUT(US)                                                                   \
+    static_assert(offsetof(US, pext) % 16 == 0 && offsetof(US, sinfo) % 16 == 0 &&             \
+                      offsetof(US, spl) % 16 == 0 && offsetof(US, toff) % 16 == 0 &&           \
+                      offsetof(US, tsp) == offsetof(US, toff) + 40,                            \
+                  "plan sections are copied as 16-byte words")
+TBX_STAGE_LAYOUT(FwdStage);
+TBX_STAGE_LAYOUT(GradStage);
+#undef TBX_STAGE_LAYOUT
 
 // (pair, jsh, ish, type) of a packed shell-pair entry
 __device__ __forceinline__ int4 unpack_spl(int v)
@@ -424,7 +436,8 @@ __device__ __forceinline__ int plan_words(const TbxUnitHead &H)
 
 // Copy a unit's plan (layout: TbxUnitHead) into LDS, one 16-byte word per
 // lane and step.
-__device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan, UnitStage &U)
+template <class US>
+__device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan, US &U)
 {
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
     const int4 *src = reinterpret_cast<const int4 *>(plan) + H.plan;
@@ -434,14 +447,14 @@ __device__ __forceinline__ void load_unit(const TbxUnitHead &H, const int *plan,
     const int e4 = e3 + ((nspair + 3) >> 2);
     const int total = e4 + 5;
     int4 *base = reinterpret_cast<int4 *>(&U);
-    for (int k = threadIdx.x; k < total; k += NTHREADS) {
+    for (int k = threadIdx.x; k < total; k += US::kThreads) {
         const int4 v = src[k];
         int w;  // destination word within U
         if (k < e1) w = k;
-        else if (k < e2) w = (int)(offsetof(UnitStage, pext) / 16) + (k - e1);
-        else if (k < e3) w = (int)(offsetof(UnitStage, sinfo) / 16) + (k - e2);
-        else if (k < e4) w = (int)(offsetof(UnitStage, spl) / 16) + (k - e3);
-        else w = (int)(offsetof(UnitStage, toff) / 16) + (k - e4);
+        else if (k < e2) w = (int)(offsetof(US, pext) / 16) + (k - e1);
+        else if (k < e3) w = (int)(offsetof(US, sinfo) / 16) + (k - e2);
+        else if (k < e4) w = (int)(offsetof(US, spl) / 16) + (k - e3);
+        else w = (int)(offsetof(US, toff) / 16) + (k - e4);
         base[w] = v;
     }
 }
@@ -456,7 +469,8 @@ __device__ __forceinline__ void pair_vec(const TbxMols &mt, const double *xyz, c
 }
 
 // Decode task tid of a staged unit: shell pair index, type and Cartesian pair.
-__device__ __forceinline__ int4 decode_task(const UnitStage &U, int task, int &spx)
+template <class US>
+__device__ __forceinline__ int4 decode_task(const US &U, int task, int &spx)
 {
     int t = 0;
     while (t < 8 && task >= U.toff[t + 1]) ++t;
@@ -472,8 +486,9 @@ __device__ __forceinline__ int4 decode_task(const UnitStage &U, int task, int &s
 // first entry of the sorted primitive-pair list, and the H0 factors
 // hs = hscale * shpoly(R), dsr = (d shpoly/dr) / (r shpoly), sesum =
 // (se_i + se_j)/2.  onsite: the j == i, T = 0 block counts (forward only).
+template <class US>
 __device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const TbxMols &mt,
-                                                const double *xyz, const UnitStage &U,
+                                                const double *xyz, const US &U,
                                                 const double *se, int k, bool onsite_ok,
                                                 int &kept, int &pi0, double &hs, double &sesum,
                                                 double &dsr)
@@ -507,6 +522,7 @@ __device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const TbxM
 }
 
 // block-wide exclusive scan of one int per lane; returns the total
+template <int NT>
 __device__ __forceinline__ int block_scan(int v, int *out, int (*wsum)[9])
 {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -517,9 +533,9 @@ __device__ __forceinline__ int block_scan(int v, int *out, int (*wsum)[9])
     int base = 0;
     for (int w = 0; w < wv; ++w) base += wsum[w][0];
     out[tid] = base + inc - v;
-    if (tid == NTHREADS - 1) out[NTHREADS] = base + inc;
+    if (tid == NT - 1) out[NT] = base + inc;
     __syncthreads();
-    return out[NTHREADS];
+    return out[NT];
 }
 
 
@@ -603,7 +619,7 @@ __device__ __forceinline__ void put(double *p, double v, bool acc)
 // every element of the molecule's matrices is written exactly once.
 __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 {
-    __shared__ UnitStage U;
+    __shared__ FwdStage U;
     __shared__ __align__(16) double acc[TBX_UNIT_TASKS][10];  // S, D (3), Q (6) per task
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
@@ -655,7 +671,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         nch = (spk[spx] + chunk - 1) / chunk;
     }
     // (3) chunk offsets
-    const int nchunk = block_scan(nch, coff, U.wsum);
+    const int nchunk = block_scan<NTHREADS>(nch, coff, U.wsum);
 
     // (4) work items
     for (int c = tid; c < nchunk; c += NTHREADS) {
@@ -1122,19 +1138,19 @@ __device__ __forceinline__ void seg_scan_step(int key, double g[3], double &ds)
 // the results to its pair's LDS sums; (5) the pair sums go to atoms j, the
 // segment sums to atoms i and the unit sum to sigma, with FP64 atomics.
 template <int MODE>
-__global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
+__global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
 {
-    __shared__ UnitStage U;
-    __shared__ double rec[NREC * TBX_UNIT_TASKS];
-    __shared__ double accP[TBX_UNIT_PAIRS][3 + 2 * MAXSH];  // G (3), dse_i, dse_j per shell
+    __shared__ GradStage U;
+    __shared__ double rec[NREC * GTHREADS];
+    __shared__ double accP[TBX_GRAD_UNIT_PAIRS][3 + 2 * MAXSH];  // G (3), dse_i, dse_j per shell
     __shared__ double accS[TBX_UNIT_SEGS][3 + MAXSH];      // atom-i sums per segment
     __shared__ double sig[6];
-    __shared__ int tinfo[TBX_UNIT_TASKS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
-    __shared__ int tpp[TBX_UNIT_TASKS];    // first primitive pair of the task
-    __shared__ int tkept[TBX_UNIT_TASKS];
-    __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];  // first chunk of each task
-    __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
-    __shared__ double sph[3][TBX_UNIT_TASKS];  // hs, sesum, dsr per shell pair
+    __shared__ int tinfo[GTHREADS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
+    __shared__ int tpp[GTHREADS];    // first primitive pair of the task
+    __shared__ int tkept[GTHREADS];
+    __shared__ __align__(16) int coff[GTHREADS + 1];  // first chunk of each task
+    __shared__ int spk[GTHREADS], spp[GTHREADS];  // kept pairs, first entry
+    __shared__ double sph[3][GTHREADS];  // hs, sesum, dsr per shell pair
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -1145,8 +1161,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
     const long long dense0 = H.dense0;
     const int ao_m = H.ao_m;
     const int chunk = A.chunk;
-    for (int k = tid; k < TBX_UNIT_PAIRS * (3 + 2 * MAXSH); k += NTHREADS) (&accP[0][0])[k] = 0.0;
-    for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += NTHREADS) (&accS[0][0])[k] = 0.0;
+    for (int k = tid; k < TBX_GRAD_UNIT_PAIRS * (3 + 2 * MAXSH); k += GTHREADS) (&accP[0][0])[k] = 0.0;
+    for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += GTHREADS) (&accS[0][0])[k] = 0.0;
     if (tid < 6) sig[tid] = 0.0;
     // (1) the unit's plan
     load_unit(H, mt.plan, U);
@@ -1160,7 +1176,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
 #pragma unroll
         for (int c = 0; c < 3; ++c) U.pv[tid][c] = v[c];
     }
-    for (int k = tid; k < nspair; k += NTHREADS) {
+    for (int k = tid; k < nspair; k += GTHREADS) {
         int kept, pi0;
         double hs, sesum, dsr;
         shellpair_setup(sp, mt, A.xyz, U, A.se, k, false, kept, pi0, hs, sesum, dsr);
@@ -1171,7 +1187,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         sph[2][k] = dsr;
     }
     __syncthreads();
-    // task records: one Cartesian pair per lane (ntask <= TBX_UNIT_TASKS = NTHREADS)
+    // task records: one Cartesian pair per lane (ntask <= GTHREADS = GTHREADS)
     const int ntask = U.toff[9];
     int nch = 0;
     if (tid < ntask) {
@@ -1183,7 +1199,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
             const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
             task_setup<MODE>(A, U.pinf[e.x], v, e.z, e.y, d.x / 3, d.x % 3, d.y, d.z, nao,
                              dense0, ao_m, sph[0][spx], sph[1][spx], sph[2][spx], rec + tid,
-                             TBX_UNIT_TASKS);
+                             GTHREADS);
         }
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
         tpp[tid] = spp[spx];
@@ -1195,18 +1211,18 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         nch = ((whole && d.w != 0) || (rows && d.z != 0)) ? 0 : (kept + chunk - 1) / chunk;
     }
     // (3) chunk offsets: block-wide exclusive scan
-    const int nchunk = block_scan(nch, coff, U.wsum);
+    const int nchunk = block_scan<GTHREADS>(nch, coff, U.wsum);
 
     // (4) work items, one per lane and step; the results of a shell pair's
     // items (consecutive lanes) are summed across the wavefront before the
     // LDS atomics
     const int lane = tid & 63;
-    for (int base = 0; base < nchunk; base += NTHREADS) {
+    for (int base = 0; base < nchunk; base += GTHREADS) {
         const int c = base + tid;
         int key = -1 - lane;  // shell pair of the item (p | ish << 12 | jsh << 15); unique if idle
         double g[3] = {0, 0, 0}, ds = 0.0;
         if (c < nchunk) {
-        const int tk = find_task(coff, c);
+        const int tk = find_task<GTHREADS>(coff, c);
         const int info = tinfo[tk];
         const int t = info & 15, ki = (info >> 4) & 15, kj = (info >> 8) & 15;
         const int p = (info >> 12) & 4095;
@@ -1216,7 +1232,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_gradient(GradArgs A)
         const int p1 = tpp[tk] + min(tkept[tk], (q + 1) * chunk);
         const double *r = rec + tk;
         const double vec[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
-        const int L = TBX_UNIT_TASKS;
+        const int L = GTHREADS;
         if (t == 0 || t == 1 || t == 3 || t == 4) {
             switch (t) {
             case 0: shellpair_run<0, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
@@ -1455,10 +1471,10 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
     (void)hipGetLastError();
     if (mol->nunit > 0) {
         if (mode == GRAD_PLAIN)
-            hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nunit), dim3(NTHREADS), 0, stream,
+            hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nunit), dim3(GTHREADS), 0, stream,
                                a);
         else
-            hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nunit), dim3(NTHREADS), 0,
+            hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nunit), dim3(GTHREADS), 0,
                                stream, a);
     }
     const int per = NTHREADS / 16;

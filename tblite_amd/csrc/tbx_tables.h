@@ -54,12 +54,14 @@ struct TbxSpecies {
 
 // Work units: consecutive (i, j, image) atom pairs of one molecule, in
 // segments j in [j0, j1) of one row atom i at lattice vector itr, packed so
-// that a unit holds at most TBX_UNIT_TASKS Cartesian-pair tasks,
-// TBX_UNIT_PAIRS atom pairs and TBX_UNIT_SEGS segments.  One workgroup runs
-// one unit.
-#define TBX_UNIT_TASKS 256
+// that a unit holds at most TBX_UNIT_TASKS Cartesian-pair tasks and
+// TBX_UNIT_PAIRS atom pairs (forward units; TBX_GRAD_UNIT_* for the gradient)
+// and TBX_UNIT_SEGS segments.  One workgroup runs one unit.
+#define TBX_UNIT_TASKS 256       /* forward units (and workgroup size) */
 #define TBX_UNIT_PAIRS 128
 #define TBX_UNIT_SEGS 32
+#define TBX_GRAD_UNIT_TASKS 128  /* gradient units (and workgroup size) */
+#define TBX_GRAD_UNIT_PAIRS 64
 struct TbxSeg {
     int i, j0, j1, itr;
 };

@@ -15,7 +15,8 @@ from pathlib import Path
 
 import torch
 
-UNIT_TASKS = (192, 256)
+FWD_UNIT_TASKS = (192, 256)   # forward workgroups hold up to 256 tasks
+GRAD_UNIT_TASKS = (112, 128)  # gradient workgroups up to 128
 CHUNKS = (1, 2, 3, 4, 6)
 
 
@@ -23,8 +24,9 @@ def batch_key(b) -> str:
     return f"nmol{b.nmol}-nat{b.nat}-nao{b.nao}-nd{b.ndense}-sp{'.'.join(map(str, b.nums))}"
 
 
-def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, unit_tasks=UNIT_TASKS,
-         chunks=CHUNKS, cache: str | os.PathLike | None = None, log=None):
+def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, fwd_unit_tasks=FWD_UNIT_TASKS,
+         grad_unit_tasks=GRAD_UNIT_TASKS, chunks=CHUNKS,
+         cache: str | os.PathLike | None = None, log=None):
     """Choose tile shapes for batch ``b`` on the given inputs.
 
     ``adjoints`` = (Sb, Db, Qb, Hb) times the adjoint kernel; ``plain`` =
@@ -39,12 +41,15 @@ def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, unit_tasks=UNIT_TASKS,
             return known[key]
     min_tasks = int(max(sum((c.ang + 1) * (c.ang + 2) // 2 for c in sh)
                         for sh in b.spec.basis_cgto)) ** 2
+    fwd = [t for t in fwd_unit_tasks if t >= min_tasks]
+    grad = [t for t in grad_unit_tasks if t >= min_tasks]
     table = []
-    for tasks, chunk in itertools.product(unit_tasks, chunks):
-        if tasks < min_tasks:
-            continue
-        b.set_tiles(fwd_unit_tasks=tasks, fwd_chunk=chunk, grad_unit_tasks=tasks,
-                    grad_chunk=chunk)
+    # both kernels are timed in every step; the forward walks its task caps,
+    # the gradient its own (cycled), with the same chunk length
+    steps = max(len(fwd), len(grad))
+    for k, chunk in itertools.product(range(steps), chunks):
+        tf, tg = fwd[k % len(fwd)], grad[k % len(grad)]
+        b.set_tiles(fwd_unit_tasks=tf, fwd_chunk=chunk, grad_unit_tasks=tg, grad_chunk=chunk)
         f = a = g = 0.0
         for _ in range(reps):
             b.forward(xyz, se)
@@ -57,16 +62,16 @@ def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, unit_tasks=UNIT_TASKS,
                 b.gradient(xyz, se, *plain)
                 torch.cuda.synchronize()
                 g += b.last_timing()["gradient_ms"] / reps
-        row = {"unit_tasks": tasks, "chunk": chunk, "forward_ms": f, "adjoint_ms": a,
-               "gradient_ms": g}
+        row = {"fwd_unit_tasks": tf, "grad_unit_tasks": tg, "chunk": chunk, "forward_ms": f,
+               "adjoint_ms": a, "gradient_ms": g}
         table.append(row)
         if log:
-            log(f"tune: tasks={tasks} chunk={chunk}: forward {f:.3f} ms, adjoint {a:.3f} ms"
+            log(f"tune: tasks={tf}/{tg} chunk={chunk}: forward {f:.3f} ms, adjoint {a:.3f} ms"
                 + (f", gradient {g:.3f} ms" if plain is not None else ""))
     bf = min(table, key=lambda r: r["forward_ms"])
     bg = min(table, key=lambda r: r["adjoint_ms"] + r["gradient_ms"])
-    tiles = {"fwd_unit_tasks": bf["unit_tasks"], "fwd_chunk": bf["chunk"],
-             "grad_unit_tasks": bg["unit_tasks"], "grad_chunk": bg["chunk"]}
+    tiles = {"fwd_unit_tasks": bf["fwd_unit_tasks"], "fwd_chunk": bf["chunk"],
+             "grad_unit_tasks": bg["grad_unit_tasks"], "grad_chunk": bg["chunk"]}
     b.set_tiles(**tiles)
     result = {"tiles": b.tiles(), "table": table}
     if cache is not None:

@@ -246,8 +246,9 @@ def test_tile_shapes_keep_results(batch_case):
     ref_a = [t.clone() for t in b.adjoint(xyz, se, *adj)]
     default = b.tiles()
     try:
-        for tasks, chunk in [(128, 4), (192, 12), (256, 1), (256, 16)]:
-            b.set_tiles(fwd_unit_tasks=tasks, fwd_chunk=chunk, grad_unit_tasks=tasks,
+        for tasks, gtasks, chunk in [(128, 128, 4), (192, 112, 12), (256, 128, 1),
+                                     (256, 104, 16)]:
+            b.set_tiles(fwd_unit_tasks=tasks, fwd_chunk=chunk, grad_unit_tasks=gtasks,
                         grad_chunk=chunk)
             for r, g in zip(ref_f, b.forward(xyz, se)):
                 assert torch.abs(r - g).max().item() < 1e-12, (tasks, chunk)
@@ -255,6 +256,8 @@ def test_tile_shapes_keep_results(batch_case):
                 assert torch.abs(r - g).max().item() < 1e-10, (tasks, chunk)
         with pytest.raises(L.TbxError):
             b.set_tiles(fwd_unit_tasks=512)
+        with pytest.raises(L.TbxError):
+            b.set_tiles(grad_unit_tasks=256)
         with pytest.raises(L.TbxError):
             b.set_tiles(grad_chunk=0)
     finally:
