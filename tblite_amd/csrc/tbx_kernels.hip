@@ -393,6 +393,18 @@ __device__ __forceinline__ int find_task(const int *off, int c)
     return 16 * b + t - 1;
 }
 
+// Unit of workgroup b of n: the hardware deals consecutive workgroups to
+// the 8 XCDs in turn; this permutation gives each XCD a contiguous range of
+// units instead, so that neighbouring units of a molecule (the same dense
+// columns and cache lines) share one L2.
+#define TBX_NXCD 8
+__device__ __forceinline__ int xcd_unit(int b, int n)
+{
+    const int x = b % TBX_NXCD, k = b / TBX_NXCD;
+    const int per = n / TBX_NXCD, rem = n % TBX_NXCD;
+    return x * per + min(x, rem) + k;
+}
+
 // Unit staging shared by the forward and gradient kernels (LDS): the unit's
 // plan (TbxUnitHead) and the pair vectors, for units of at most NT tasks
 // (= workgroup size) and NP atom pairs.
@@ -628,7 +640,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
-    const TbxUnitHead H = mt.funit[blockIdx.x];
+    const TbxUnitHead H = mt.funit[xcd_unit(blockIdx.x, gridDim.x)];
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
     const int nao = H.nao;
     const long long dense0 = H.dense0;
