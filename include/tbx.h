@@ -241,6 +241,29 @@ typedef struct {
 int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles);
 int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles);
 
+/* Work-unit plans of a batch description, built on the host only (no
+ * context, no device) with `tiles` (NULL: the defaults), and checked against
+ * what the kernels assume; no counterpart in the reference (an inspection
+ * entry for tests and tooling).  Checked: every atom pair (i, j <= i) of every
+ * molecule is in exactly one forward unit and every (i, j < i) in exactly one
+ * gradient unit; every dense element of every molecule is written by exactly
+ * one forward element descriptor, inside its molecule's block, and reads only
+ * its unit's own Cartesian-pair tasks; no unit exceeds the tiles' caps. */
+typedef struct {
+    long long nunit_fwd, nunit_grad;     /* work units */
+    long long npair_fwd, npair_grad;     /* atom pairs over all units */
+    long long ntask_fwd, ntask_grad;     /* Cartesian-pair tasks over all units */
+    long long nelem;                     /* forward element descriptors */
+    long long ndense;                    /* dense elements of the batch, sum of nao^2 */
+    long long plan_bytes_fwd, plan_bytes_grad;
+    long long bad_pairs;  /* pairs missing, repeated, or not of one molecule with j <= i */
+    long long bad_elems;  /* elements written 0 or > 1 times, or reading outside the unit */
+    long long bad_units;  /* units over a cap */
+} tbx_plan_stats;
+
+int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles *tiles,
+                         tbx_plan_stats *out);
+
 /* Measurement helper, no counterpart in the reference: copies n doubles
  * d_src -> d_dst with a plain streaming kernel (k_copy) on `stream`.  Its
  * hardware-counter readings (FETCH_SIZE, WRITE_SIZE) for a known byte count

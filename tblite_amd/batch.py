@@ -28,35 +28,61 @@ def _ptr(t: torch.Tensor):
     return ct.c_void_p(t.data_ptr())
 
 
+class _Description:
+    """GFN2 species tables and molecule sizes of a batch, as the C-ABI's
+    ``tbx_batch_desc`` (the arrays it points to are kept alive here)."""
+
+    def __init__(self, molecules):
+        numbers = [np.asarray(z, dtype=np.int32) for z, _ in molecules]
+        allz = np.concatenate(numbers)
+        self.nums, self.ids = params.species_of(allz)
+        self.spec = params.gfn2_species(self.nums)
+        self.msh = int(self.spec.nsh_id.max())
+        bas = params.Basis.new(self.ids, self.spec.nsh_id, self.spec.basis_cgto)
+        self.cutoff = bas.get_cutoff()
+        self.intcut = bas.intcut
+        self.mol_nat = np.array([len(z) for z in numbers], dtype=np.int32)
+        ang, nprim, alpha, coeff = structure.flat_cgto(self.spec, self.msh)
+        self.keep = [np.ascontiguousarray(a) for a in (
+            self.spec.nsh_id.astype(np.int32), ang, nprim, alpha, coeff,
+            self.spec.hscale.reshape(-1, order="F"), self.spec.shpoly.reshape(-1, order="F"),
+            self.spec.rad, self.mol_nat, self.ids.astype(np.int32))]
+        k = self.keep
+        p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+        self.basis = L.Basis(len(self.nums), self.msh, p(k[0]), p(k[1]), p(k[2]), p(k[3]),
+                             p(k[4]), self.intcut)
+        self.h0 = L.H0Par(p(k[5]), p(k[6]), p(k[7]))
+        self.desc = L.BatchDesc(len(molecules), p(k[8]), p(k[9]), ct.pointer(self.basis),
+                                ct.pointer(self.h0), self.cutoff)
+
+
+def plan_stats(molecules, tiles: dict | None = None) -> dict:
+    """Work-unit plans of a batch of ``(numbers, xyz_bohr)``, built and checked
+    on the host (no device): unit, pair, task and element counts, plan bytes,
+    and the number of violations of what the kernels assume (``bad_pairs``,
+    ``bad_elems``, ``bad_units``; all zero for a correct planner).  ``tiles``
+    as in ``Batch.set_tiles`` (None: the library defaults)."""
+    d = _Description(molecules)
+    t = None if tiles is None else ct.pointer(L.Tiles(**tiles))
+    out = L.PlanStats()
+    L.check(L.lib().tbx_batch_plan_stats(ct.byref(d.desc), t, ct.byref(out)))
+    return {name: int(getattr(out, name)) for name, _ in L.PlanStats._fields_}
+
+
 class Batch:
     """Batched GFN2 S/D/Q/H0 + gradient for a list of ``(numbers, xyz_bohr)``."""
 
     def __init__(self, molecules, device: int = 0):
         self.device = device
-        numbers = [np.asarray(z, dtype=np.int32) for z, _ in molecules]
-        allz = np.concatenate(numbers)
-        self.nums, ids = params.species_of(allz)
-        self.spec = params.gfn2_species(self.nums)
-        self.msh = int(self.spec.nsh_id.max())
-        bas = params.Basis.new(ids, self.spec.nsh_id, self.spec.basis_cgto)
-        self.cutoff = bas.get_cutoff()
-        self.intcut = bas.intcut
-        self.mol_nat = np.array([len(z) for z in numbers], dtype=np.int32)
-        self.ids = ids
+        d = _Description(molecules)
+        self.nums, ids, self.spec, self.msh = d.nums, d.ids, d.spec, d.msh
+        self.cutoff, self.intcut, self.mol_nat, self.ids = d.cutoff, d.intcut, d.mol_nat, d.ids
         self.xyz0 = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)
                                     for _, x in molecules])
-        ang, nprim, alpha, coeff = structure.flat_cgto(self.spec, self.msh)
-        self._keep = [np.ascontiguousarray(a) for a in (
-            self.spec.nsh_id.astype(np.int32), ang, nprim, alpha, coeff,
-            self.spec.hscale.reshape(-1, order="F"), self.spec.shpoly.reshape(-1, order="F"),
-            self.spec.rad, self.mol_nat, ids.astype(np.int32))]
-        k = self._keep
+        self._descr = d  # keeps the tables the descriptor points to alive
+        self._keep, self._basis, self._h0 = d.keep, d.basis, d.h0
+        desc = d.desc
         p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
-        self._basis = L.Basis(len(self.nums), self.msh, p(k[0]), p(k[1]), p(k[2]), p(k[3]),
-                              p(k[4]), self.intcut)
-        self._h0 = L.H0Par(p(k[5]), p(k[6]), p(k[7]))
-        desc = L.BatchDesc(len(molecules), p(k[8]), p(k[9]), ct.pointer(self._basis),
-                           ct.pointer(self._h0), self.cutoff)
         handle = ct.c_void_p()
         L.check(L.lib().tbx_batch_create(L.context(device), ct.byref(desc), ct.byref(handle)))
         self.handle = handle

@@ -1071,6 +1071,143 @@ extern "C" int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles)
     return TBX_OK;
 }
 
+// Sections of a staged plan (16-byte words from TbxUnitHead::plan), as the
+// kernels' load_unit reads them: pairs, pext, segments, shell pairs, offsets.
+struct PlanView {
+    const int *pairs, *pext, *sinfo, *spl, *toff, *tsp;
+    int npair, nseg, nspair, words;
+};
+
+static PlanView plan_view(const std::vector<int> &plan, const TbxUnitHead &h)
+{
+    PlanView v;
+    v.npair = h.counts & 255;
+    v.nseg = (h.counts >> 8) & 255;
+    v.nspair = h.counts >> 16;
+    const int e1 = (10 * v.npair + 3) / 4, e2 = e1 + (v.npair + 3) / 4, e3 = e2 + v.nseg,
+              e4 = e3 + (v.nspair + 3) / 4;
+    const int *w = plan.data() + 4 * (size_t)h.plan;
+    v.pairs = w;
+    v.pext = w + 4 * e1;
+    v.sinfo = w + 4 * e2;
+    v.spl = w + 4 * e3;
+    v.toff = w + 4 * e4;
+    v.tsp = v.toff + 10;
+    v.words = e4 + 5;
+    return v;
+}
+
+extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles *tiles,
+                                    tbx_plan_stats *out)
+{
+    if (!desc || !out) return fail(TBX_ERR_ARG, "null argument");
+    HostSpecies hs;
+    HostMols hm;
+    int st = build_species(desc->bas, desc->h0, hs);
+    if (!st) st = build_mols(hs, desc->nmol, desc->mol_nat, desc->id, hm);
+    if (st) return st;
+    const tbx_tiles t = tiles ? *tiles : kDefaultTiles;
+    if ((st = check_tiles(hs, &t))) return st;
+    if ((st = build_units(hs, hm, 1, t))) return st;
+    tbx_plan_stats s = {};
+    s.ndense = hm.ndense;
+    s.nunit_fwd = (long long)hm.fhead.size();
+    s.nunit_grad = (long long)hm.uhead.size();
+    s.plan_bytes_fwd = 4 * (long long)hm.fplan.size();
+    s.plan_bytes_grad = 4 * (long long)hm.uplan.size();
+    // atom pairs: one counter per ordered pair (i, j) of each molecule
+    std::vector<long long> poff(hm.nmol + 1, 0);
+    for (int m = 0; m < hm.nmol; ++m) {
+        const long long n = hm.mol_atom[m + 1] - hm.mol_atom[m];
+        poff[m + 1] = poff[m] + n * n;
+    }
+    for (int fwd = 0; fwd < 2; ++fwd) {
+        const std::vector<TbxUnitHead> &head = fwd ? hm.fhead : hm.uhead;
+        const std::vector<int> &plan = fwd ? hm.fplan : hm.uplan;
+        const int cap_tasks = fwd ? t.fwd_unit_tasks : t.grad_unit_tasks;
+        const int cap_pairs = fwd ? t.fwd_unit_pairs : t.grad_unit_pairs;
+        std::vector<unsigned char> seen((size_t)poff[hm.nmol], 0);
+        std::vector<unsigned char> hit(fwd ? (size_t)hm.ndense : 0, 0);
+        for (const TbxUnitHead &h : head) {
+            const PlanView v = plan_view(plan, h);
+            const int ntask = v.toff[9];
+            (fwd ? s.npair_fwd : s.npair_grad) += v.npair;
+            (fwd ? s.ntask_fwd : s.ntask_grad) += ntask;
+            if (ntask > cap_tasks || v.npair > cap_pairs || v.nseg > TBX_UNIT_SEGS ||
+                v.nspair > ntask || (fwd && h.nel > 2 * t.fwd_unit_tasks))
+                ++s.bad_units;
+            for (int p = 0; p < v.npair; ++p) {
+                const int i = v.pairs[10 * p], j = v.pairs[10 * p + 1];
+                const bool ok = i >= 0 && i < hm.nat && j >= 0 && j < hm.nat &&
+                                hm.at_mol[i] == hm.at_mol[j] && (fwd ? j <= i : j < i) &&
+                                h.mol == hm.at_mol[i];
+                if (!ok) {
+                    ++s.bad_pairs;
+                    continue;
+                }
+                const int m = hm.at_mol[i], a0 = hm.mol_atom[m];
+                const long long n = hm.mol_atom[m + 1] - a0;
+                unsigned char &c = seen[poff[m] + (i - a0) * n + (j - a0)];
+                if (c < 2) ++c;
+            }
+            if (!fwd) continue;
+            const int *ed = plan.data() + 4 * ((size_t)h.plan + v.words);
+            const int m = h.mol;
+            const long long nao = hm.mol_nao[m];
+            const bool block_ok = h.dense0 == hm.mol_dense[m] && h.nao == nao;
+            for (int e = 0; e < h.nel; ++e) {
+                ++s.nelem;
+                const int dsc = ed[e];
+                const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31,
+                          task0 = (dsc >> 17) & 255;
+                const bool mirror = (dsc >> 25) & 1;
+                if (!block_ok || p >= v.npair) {
+                    ++s.bad_elems;
+                    continue;
+                }
+                const int *pr = v.pairs + 10 * p;  // i, j, spi, spj, aoi, aoj, ...
+                const int spi = pr[2], spj = pr[3];
+                if (nul >= hs.nao[spi] || muj >= hs.nao[spj] || (mirror && pr[1] >= pr[0])) {
+                    ++s.bad_elems;
+                    continue;
+                }
+                // the element's Cartesian tasks are the block of its shell pair
+                const int ish = hs.ao_sh[nul + TBX_MAXAO * spi];
+                const int jsh = hs.ao_sh[muj + TBX_MAXAO * spj];
+                const int li = hs.ang[ish + hs.msh * spi], lj = hs.ang[jsh + hs.msh * spj];
+                const int ty = 3 * li + lj, nc = (li + 1) * (li + 2) / 2 * ((lj + 1) * (lj + 2) / 2);
+                const int rel = task0 - v.toff[ty], spx = v.tsp[ty] + rel / nc;
+                if (rel < 0 || rel % nc != 0 || task0 + nc > v.toff[ty + 1] ||
+                    spx >= v.tsp[ty + 1] ||
+                    v.spl[spx] != (p | (jsh << 8) | (ish << 12) | (ty << 16))) {
+                    ++s.bad_elems;
+                    continue;
+                }
+                const long long nu = pr[4] + nul, mu = pr[5] + muj;
+                if (nu >= nao || mu >= nao) {
+                    ++s.bad_elems;
+                    continue;
+                }
+                unsigned char &c = hit[h.dense0 + (mirror ? mu * nao + nu : nu * nao + mu)];
+                if (c < 2) ++c;
+            }
+        }
+        for (int m = 0; m < hm.nmol; ++m) {
+            const int a0 = hm.mol_atom[m];
+            const long long n = hm.mol_atom[m + 1] - a0;
+            for (long long i = 0; i < n; ++i)
+                for (long long j = 0; j <= i; ++j) {
+                    const int want = (fwd || j < i) ? 1 : 0;
+                    if (seen[poff[m] + i * n + j] != want) ++s.bad_pairs;
+                }
+        }
+        for (unsigned char c : hit)
+            if (c != 1) ++s.bad_elems;
+    }
+    *out = s;
+    return TBX_OK;
+}
+
 extern "C" void tbx_batch_delete(tbx_batch *b)
 {
     if (!b) return;

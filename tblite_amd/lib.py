@@ -65,6 +65,13 @@ class Tiles(ct.Structure):
                 ("grad_chunk", ct.c_int)]
 
 
+class PlanStats(ct.Structure):
+    _fields_ = [(n, ct.c_longlong) for n in (
+        "nunit_fwd", "nunit_grad", "npair_fwd", "npair_grad", "ntask_fwd", "ntask_grad",
+        "nelem", "ndense", "plan_bytes_fwd", "plan_bytes_grad", "bad_pairs", "bad_elems",
+        "bad_units")]
+
+
 _SIGS = {
     "tbx_version": (ct.c_int, []),
     "tbx_last_error": (ct.c_char_p, []),
@@ -101,6 +108,8 @@ _SIGS = {
     "tbx_probe_copy": (ct.c_int, [_P, _P, ct.c_longlong, _P]),
     "tbx_batch_set_tiles": (ct.c_int, [_P, ct.POINTER(Tiles)]),
     "tbx_batch_get_tiles": (ct.c_int, [_P, ct.POINTER(Tiles)]),
+    "tbx_batch_plan_stats": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(Tiles),
+                                        ct.POINTER(PlanStats)]),
 }
 
 

@@ -149,3 +149,69 @@ def test_scf_consumers_reject_null_batch():
     assert so.tbx_batch_mulliken_atomic_multipoles(None, 1, 6, buf, buf, buf,
                                                    None) == TBX_ERR_ARG
     assert so.tbx_batch_last_scf_timing(None, buf) == TBX_ERR_ARG
+
+
+_SHELLS = {1: (0,), 6: (0, 1), 7: (0, 1), 8: (0, 1), 9: (0, 1), 16: (0, 1, 2), 17: (0, 1, 2)}
+
+
+def _expected_plan_totals(mols):
+    """Totals the unit plans must have, from the GFN2 shells of each element
+    alone: dense elements sum(nao^2); forward pairs (i, j <= i), gradient
+    pairs (i, j < i); Cartesian-pair tasks sum over those pairs of
+    ncart_i ncart_j = ((sum ncart)^2 +- sum ncart^2) / 2."""
+    tot = dict(ndense=0, npair_fwd=0, npair_grad=0, ntask_fwd=0, ntask_grad=0)
+    for z, _ in mols:
+        nao = np.array([sum(2 * l + 1 for l in _SHELLS[int(a)]) for a in z])
+        nc = np.array([sum((l + 1) * (l + 2) // 2 for l in _SHELLS[int(a)]) for a in z])
+        n = len(z)
+        tot["ndense"] += int(nao.sum()) ** 2
+        tot["npair_fwd"] += n * (n + 1) // 2
+        tot["npair_grad"] += n * (n - 1) // 2
+        tot["ntask_fwd"] += (int(nc.sum()) ** 2 + int((nc ** 2).sum())) // 2
+        tot["ntask_grad"] += (int(nc.sum()) ** 2 - int((nc ** 2).sum())) // 2
+    return tot
+
+
+@pytest.mark.parametrize("tiles", [
+    None,                                                    # library defaults
+    dict(fwd_unit_tasks=100, fwd_unit_pairs=1, fwd_chunk=1,  # one atom pair per unit
+         grad_unit_tasks=100, grad_unit_pairs=1, grad_chunk=1),
+    dict(fwd_unit_tasks=192, fwd_unit_pairs=7, fwd_chunk=2,
+         grad_unit_tasks=112, grad_unit_pairs=13, grad_chunk=4),
+])
+def test_unit_plans_cover_every_pair_and_element_once(tiles):
+    """Host work-unit planner, no GPU: every atom pair (i, j <= i) of every
+    molecule is in exactly one forward unit and every (i, j < i) in exactly one
+    gradient unit; every dense element is written by exactly one forward
+    element descriptor, which reads only its own shell pair's tasks; no unit
+    exceeds a cap.  Lone atoms (H, S), a diatomic, S/Cl d shells and molecules
+    that span many units; totals checked against the closed forms above."""
+    from tblite_amd.batch import plan_stats
+    rng = np.random.default_rng(11)
+    mols = [(np.array([1]), np.zeros((1, 3))), (np.array([16]), np.zeros((1, 3))),
+            (np.array([6, 17]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 3.3]]))]
+    mols += [synthetic.random_molecule(n, rng) for n in (9, 23, 57, 140)]
+    st = plan_stats(mols, tiles)
+    assert st["bad_pairs"] == 0 and st["bad_elems"] == 0 and st["bad_units"] == 0, st
+    for k, v in _expected_plan_totals(mols).items():
+        assert st[k] == v, (k, st[k], v)
+    assert st["nelem"] == st["ndense"]
+    assert st["nunit_fwd"] >= 4 and st["nunit_grad"] >= 3
+    if tiles is not None and tiles["fwd_unit_pairs"] == 1:
+        assert st["nunit_fwd"] == st["npair_fwd"] and st["nunit_grad"] == st["npair_grad"]
+    assert st["plan_bytes_fwd"] > 4 * st["nelem"] and st["plan_bytes_grad"] > 0
+
+
+def test_unit_plans_reject_tiles_beyond_the_kernels():
+    """Gradient units beyond 128 tasks, or below the largest atom pair (an S-S
+    pair has (1 + 3 + 6)^2 = 100 Cartesian-pair tasks), are argument errors."""
+    from tblite_amd.batch import plan_stats
+    mols = [(np.array([16, 16]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 3.9]]))]
+    base = dict(fwd_unit_tasks=256, fwd_unit_pairs=128, fwd_chunk=3,
+                grad_unit_tasks=128, grad_unit_pairs=64, grad_chunk=1)
+    for bad in (dict(grad_unit_tasks=256), dict(grad_unit_tasks=99), dict(fwd_unit_tasks=99),
+                dict(fwd_unit_pairs=129), dict(grad_chunk=0)):
+        with pytest.raises(L.TbxError) as e:
+            plan_stats(mols, {**base, **bad})
+        assert e.value.code == TBX_ERR_ARG
+    assert plan_stats(mols, base)["bad_elems"] == 0
