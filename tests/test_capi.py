@@ -215,3 +215,15 @@ def test_unit_plans_reject_tiles_beyond_the_kernels():
             plan_stats(mols, {**base, **bad})
         assert e.value.code == TBX_ERR_ARG
     assert plan_stats(mols, base)["bad_elems"] == 0
+
+
+def test_unit_plans_of_the_2048_molecule_workload():
+    """The plans the bench runs (2048 molecules of 8-256 atoms, default
+    tiles): no violation, and the totals of the closed forms.  About 10 s."""
+    from tblite_amd.batch import plan_stats
+    mols = synthetic.molecule_batch(2048, 8, 256, seed=1004, workers=8)
+    st = plan_stats(mols)
+    assert st["bad_pairs"] == 0 and st["bad_elems"] == 0 and st["bad_units"] == 0, st
+    for k, v in _expected_plan_totals(mols).items():
+        assert st[k] == v, (k, st[k], v)
+    assert st["ndense"] == 144333216 and st["nelem"] == st["ndense"]
