@@ -110,7 +110,7 @@ struct HostSpecies {
     int nsp = 0, msh = 0, maxnc = 0;
     std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao,
         prim_off, pp_off, ao_info, ao_desc;
-    std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp, pp_mu;
+    std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp, pp_mu, onsite;
     double intcut = 25.0;
 };
 
@@ -233,6 +233,37 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
     hs.pp_off[(size_t)nslot * nslot] = (int)(hs.pp.size() / TBX_PP_N);
     hs.pp_mu.resize(hs.pp.size() / TBX_PP_N);
     for (size_t k = 0; k < hs.pp_mu.size(); ++k) hs.pp_mu[k] = hs.pp[TBX_PP_N * k + TBX_PP_MU];
+    // same-centre overlaps: sum over the primitive pairs of pre * prod_k
+    // <y^(a_k + b_k)>, the even moments (2n-1)!! h^n of the product Gaussian
+    auto moment = [](double h, int t) {
+        switch (t) {
+        case 0: return 1.0;
+        case 2: return h;
+        case 4: return 3.0 * h * h;
+        case 6: return 15.0 * h * h * h;
+        case 8: return 105.0 * h * h * h * h;
+        default: return 0.0;
+        }
+    };
+    hs.onsite.assign((size_t)nsp * TBX_MAXCART * TBX_MAXCART, 0.0);
+    for (int isp = 0; isp < nsp; ++isp)
+        for (int ci = 0; ci < hs.ncart[isp]; ++ci)
+            for (int cj = 0; cj < hs.ncart[isp]; ++cj) {
+                const int ei = hs.cart_exp[isp * TBX_MAXCART + ci];
+                const int ej = hs.cart_exp[isp * TBX_MAXCART + cj];
+                const int si = hs.cart_sh[isp * TBX_MAXCART + ci] + msh * isp;
+                const int sj = hs.cart_sh[isp * TBX_MAXCART + cj] + msh * isp;
+                double sum = 0.0;
+                for (int q = hs.pp_off[(size_t)si * nslot + sj];
+                     q < hs.pp_off[(size_t)si * nslot + sj + 1]; ++q) {
+                    const double h = hs.pp[TBX_PP_N * q + TBX_PP_H];
+                    sum += hs.pp[TBX_PP_N * q + TBX_PP_PRE] *
+                           moment(h, (ei & 15) + (ej & 15)) *
+                           moment(h, ((ei >> 4) & 15) + ((ej >> 4) & 15)) *
+                           moment(h, ((ei >> 8) & 15) + ((ej >> 8) & 15));
+                }
+                hs.onsite[((size_t)isp * TBX_MAXCART + ci) * TBX_MAXCART + cj] = sum;
+            }
     hs.ao_info.assign((size_t)nsp * TBX_MAXAO, 0);
     for (size_t k = 0; k < hs.ao_info.size(); ++k) {
         const int isp = (int)(k / TBX_MAXAO);
@@ -582,7 +613,7 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
                  o_sh_cart = a.add(hs.sh_cart), o_sh_ao = a.add(hs.sh_ao), o_prim = a.add(hs.prim),
                  o_prim_off = a.add(hs.prim_off), o_pp = a.add(hs.pp), o_pp_off = a.add(hs.pp_off),
                  o_pp_mu = a.add(hs.pp_mu), o_ao_info = a.add(hs.ao_info),
-                 o_ao_desc = a.add(hs.ao_desc);
+                 o_ao_desc = a.add(hs.ao_desc), o_onsite = a.add(hs.onsite);
     const hipError_t e = upload_arena(d.species, a, s);
     if (e != hipSuccess) return e;
     const char *base = static_cast<const char *>(d.species.p);
@@ -613,6 +644,7 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     sp.pp_off = reinterpret_cast<const int *>(at(o_pp_off));
     sp.pp = reinterpret_cast<const double *>(at(o_pp));
     sp.pp_mu = reinterpret_cast<const double *>(at(o_pp_mu));
+    sp.onsite = reinterpret_cast<const double *>(at(o_onsite));
     sp.ao_info = reinterpret_cast<const int *>(at(o_ao_info));
     sp.ao_desc = reinterpret_cast<const int *>(at(o_ao_desc));
     sp.intcut = hs.intcut;

@@ -1372,15 +1372,8 @@ __global__ void __launch_bounds__(NTHREADS) k_onsite(GradArgs A)
                         A.Hb[dense0 + (long long)(nui0 + cart_m(li, ki, b)) * nao + muj0 +
                              cart_m(lj, kj, a)];
         if (wHon == 0.0) continue;
-        const int e0 = (ei_ & 15) + (ej_ & 15), e1 = ((ei_ >> 4) & 15) + ((ej_ >> 4) & 15),
-                  e2 = ((ei_ >> 8) & 15) + ((ej_ >> 8) & 15);
-        const int p0 = sp.pp_off[sloti * sp.nslot + slotj];
-        const int p1 = sp.pp_off[sloti * sp.nslot + slotj + 1];
-        double s = 0.0;
-        for (int p = p0; p < p1; ++p) {
-            const Mom M = make_mom(sp.pp[TBX_PP_N * p + TBX_PP_H]);
-            s += sp.pp[TBX_PP_N * p + TBX_PP_PRE] * mom(M, e0) * mom(M, e1) * mom(M, e2);
-        }
+        // the same-centre overlap of the pair (a species constant)
+        const double s = sp.onsite[(spi * TBX_MAXCART + ci) * TBX_MAXCART + cj];
         if (s != 0.0) {
             unsafeAtomicAdd(&A.dse[mt.at_sh[i] + ish], 0.5 * wHon * s);
             unsafeAtomicAdd(&A.dse[mt.at_sh[i] + jsh], 0.5 * wHon * s);
