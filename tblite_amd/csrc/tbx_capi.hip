@@ -233,8 +233,9 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
     hs.pp_off[(size_t)nslot * nslot] = (int)(hs.pp.size() / TBX_PP_N);
     hs.pp_mu.resize(hs.pp.size() / TBX_PP_N);
     for (size_t k = 0; k < hs.pp_mu.size(); ++k) hs.pp_mu[k] = hs.pp[TBX_PP_N * k + TBX_PP_MU];
-    // same-centre overlaps: sum over the primitive pairs of pre * prod_k
-    // <y^(a_k + b_k)>, the even moments (2n-1)!! h^n of the product Gaussian
+    // same-centre moments: sums over the primitive pairs of pre * prod_k
+    // <y^(a_k + b_k + m_k)>, with the even moments (2n-1)!! h^n of the product
+    // Gaussian; m = 0 (S), one unit vector (D), two (Q: xx xy yy xz yz zz)
     auto moment = [](double h, int t) {
         switch (t) {
         case 0: return 1.0;
@@ -245,7 +246,9 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
         default: return 0.0;
         }
     };
-    hs.onsite.assign((size_t)nsp * TBX_MAXCART * TBX_MAXCART, 0.0);
+    static const int kMom[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
+                                    {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}};
+    hs.onsite.assign((size_t)nsp * TBX_MAXCART * TBX_MAXCART * 10, 0.0);
     for (int isp = 0; isp < nsp; ++isp)
         for (int ci = 0; ci < hs.ncart[isp]; ++ci)
             for (int cj = 0; cj < hs.ncart[isp]; ++cj) {
@@ -253,16 +256,17 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
                 const int ej = hs.cart_exp[isp * TBX_MAXCART + cj];
                 const int si = hs.cart_sh[isp * TBX_MAXCART + ci] + msh * isp;
                 const int sj = hs.cart_sh[isp * TBX_MAXCART + cj] + msh * isp;
-                double sum = 0.0;
+                const int e[3] = {(ei & 15) + (ej & 15), ((ei >> 4) & 15) + ((ej >> 4) & 15),
+                                  ((ei >> 8) & 15) + ((ej >> 8) & 15)};
+                double *out = &hs.onsite[(((size_t)isp * TBX_MAXCART + ci) * TBX_MAXCART + cj) * 10];
                 for (int q = hs.pp_off[(size_t)si * nslot + sj];
                      q < hs.pp_off[(size_t)si * nslot + sj + 1]; ++q) {
                     const double h = hs.pp[TBX_PP_N * q + TBX_PP_H];
-                    sum += hs.pp[TBX_PP_N * q + TBX_PP_PRE] *
-                           moment(h, (ei & 15) + (ej & 15)) *
-                           moment(h, ((ei >> 4) & 15) + ((ej >> 4) & 15)) *
-                           moment(h, ((ei >> 8) & 15) + ((ej >> 8) & 15));
+                    const double pre = hs.pp[TBX_PP_N * q + TBX_PP_PRE];
+                    for (int k = 0; k < 10; ++k)
+                        out[k] += pre * moment(h, e[0] + kMom[k][0]) *
+                                  moment(h, e[1] + kMom[k][1]) * moment(h, e[2] + kMom[k][2]);
                 }
-                hs.onsite[((size_t)isp * TBX_MAXCART + ci) * TBX_MAXCART + cj] = sum;
             }
     hs.ao_info.assign((size_t)nsp * TBX_MAXAO, 0);
     for (size_t k = 0; k < hs.ao_info.size(); ++k) {

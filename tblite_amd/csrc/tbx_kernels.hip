@@ -499,7 +499,7 @@ __device__ __forceinline__ int4 decode_task(const US &U, int task, int &spx)
 // hs = hscale * shpoly(R), dsr = (d shpoly/dr) / (r shpoly), sesum =
 // (se_i + se_j)/2.  onsite: the j == i, T = 0 block counts (forward only).
 template <class US>
-__device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const TbxMols &mt,
+__device__ __forceinline__ bool shellpair_setup(const TbxSpecies &sp, const TbxMols &mt,
                                                 const double *xyz, const US &U,
                                                 const double *se, int k, bool onsite_ok,
                                                 int &kept, int &pi0, double &hs, double &sesum,
@@ -516,13 +516,13 @@ __device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const TbxM
     kept = 0;
     hs = sesum = dsr = 0.0;
     const bool onsite = onsite_ok && (pr.i == pr.j) && r2 < kEps;
-    if (!onsite && (r2 < kEps || r2 > sp.cutoff2)) return;
+    if (!onsite && (r2 < kEps || r2 > sp.cutoff2)) return false;
     const int npp = sp.pp_off[sloti * sp.nslot + slotj + 1] - pi0;
     kept = kept_pairs(sp.pp_mu + pi0, npp, onsite ? 0.0 : r2, sp.intcut);
     sesum = 0.5 * (se[pr.shi + ish] + se[pr.shj + jsh]);
     if (onsite) {
         hs = 1.0;  // on-site block: H = S (se_i + se_j)/2
-        return;
+        return true;
     }
     const double hsc = sp.hscale[jsh + sp.msh * (ish + sp.msh * (pr.spj + sp.nsp * pr.spi))];
     const double kpi = sp.shpoly[sloti], kpj = sp.shpoly[slotj];
@@ -531,6 +531,7 @@ __device__ __forceinline__ void shellpair_setup(const TbxSpecies &sp, const TbxM
     const double shp = shi * shj;
     hs = hsc * shp;
     dsr = (shi * kpj + shj * kpi) * 0.5 * rr / (r2 * shp);
+    return false;
 }
 
 // block-wide exclusive scan of one int per lane; returns the total
@@ -667,8 +668,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     for (int k = tid; k < nspair; k += NTHREADS) {
         int kept, pi0;
         double hs, sesum, dsr;
-        shellpair_setup(sp, mt, A.xyz, U, A.se, k, true, kept, pi0, hs, sesum, dsr);
-        spk[k] = kept;
+        const bool on = shellpair_setup(sp, mt, A.xyz, U, A.se, k, true, kept, pi0, hs, sesum, dsr);
+        spk[k] = on ? -1 : kept;  // on-site blocks: moments from the species table
         spp[k] = pi0;
         hfac[k] = sesum * hs;
     }
@@ -680,7 +681,22 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         int spx;
         const int4 d = decode_task(U, tid, spx);
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (spx << 12) | ((U.spl[spx] & 255) << 20);
-        nch = (spk[spx] + chunk - 1) / chunk;
+        const int kept = spk[spx];
+        if (kept >= 0) {
+            nch = (kept + chunk - 1) / chunk;
+        } else {
+            // on-site block (j == i, no lattice shift): the same-centre moments
+            // of the species, no work items
+            const int4 e = unpack_spl(U.spl[spx]);  // pair, jsh, ish, type
+            const int spi = U.pinf[e.x].spi;
+            const int ci = sp.sh_cart[e.z + sp.msh * spi] + d.y;
+            const int cj = sp.sh_cart[e.y + sp.msh * spi] + d.z;
+            const double2 *src = reinterpret_cast<const double2 *>(
+                sp.onsite + ((size_t)(spi * TBX_MAXCART + ci) * TBX_MAXCART + cj) * 10);
+            double2 *dst = reinterpret_cast<double2 *>(acc[tid]);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) dst[q] = src[q];
+        }
     }
     // (3) chunk offsets
     const int nchunk = block_scan<NTHREADS>(nch, coff, U.wsum);
@@ -1373,7 +1389,7 @@ __global__ void __launch_bounds__(NTHREADS) k_onsite(GradArgs A)
                              cart_m(lj, kj, a)];
         if (wHon == 0.0) continue;
         // the same-centre overlap of the pair (a species constant)
-        const double s = sp.onsite[(spi * TBX_MAXCART + ci) * TBX_MAXCART + cj];
+        const double s = sp.onsite[((spi * TBX_MAXCART + ci) * TBX_MAXCART + cj) * 10];
         if (s != 0.0) {
             unsafeAtomicAdd(&A.dse[mt.at_sh[i] + ish], 0.5 * wHon * s);
             unsafeAtomicAdd(&A.dse[mt.at_sh[i] + jsh], 0.5 * wHon * s);

@@ -46,9 +46,10 @@ struct TbxSpecies {
     const int *pp_off;    // [nslot*nslot+1]
     const double *pp;     // [TBX_PP_N * entries]
     const double *pp_mu;  // [entries] reduced exponents alone (screening counts)
-    // same-centre overlap of the Cartesian pair (ci, cj) of a species, the sum
-    // over its primitive pairs (no geometry, no screening): on-site blocks
-    const double *onsite; // [nsp*TBX_MAXCART*TBX_MAXCART]
+    // same-centre moments of the Cartesian pair (ci, cj) of a species, sums
+    // over its primitive pairs (no geometry, no screening): S, D (x, y, z) and
+    // Q (xx, xy, yy, xz, yz, zz) about the centre, the on-site blocks
+    const double *onsite; // [nsp*TBX_MAXCART*TBX_MAXCART*10]
     const int *ao_info;   // [nsp*TBX_MAXAO] shell | m << 4 | l << 8 of each spherical function
     const int *ao_desc;   // [nsp] shells (bits 0-2) | l of shell s << (3 + 2 s)
     double intcut;
