@@ -132,8 +132,9 @@ def main():
         cpu_baseline_child(args.cpu_baseline_child)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
-              "(molecules shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: the batched path runs on one GPU; several GPUs would each take a "
+              "share of the molecules with no exchange between them, and that is not measured "
+              "here", file=sys.stderr)
         sys.exit(2)
 
     cpu = None if args.no_cpu else run_cpu_baseline(args.cpu_budget)

@@ -1,6 +1,6 @@
 """Parity of the gfx950 path (through the C-ABI) with the oracle.
 
-Tolerances: integrals and H0 to 1e-10 absolute, gradient
+Tolerances (the accuracy bar of the path): integrals and H0 to 1e-10 absolute, gradient
 contractions to 1e-8 (Eh/bohr); the kernels use the same FP64 formulas in a
 different order, so the observed differences are ~1e-14.
 """
