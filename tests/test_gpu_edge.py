@@ -131,3 +131,43 @@ def test_misaligned_multipole_output_is_rejected(edge_case):
     assert D.data_ptr() % 16 == 8
     with pytest.raises(L.TbxError):
         b.forward(b.tensor_xyz(), se, out=(S, D, Q, H))
+
+
+def test_batch_of_lone_atoms_only():
+    """A batch of lone atoms has no gradient work units at all: the adjoint is
+    its on-site part alone.  Forward against the oracle; autograd gives zero
+    dE/dR, and dE/dse_s = 1/2 sum over the on-site elements (mu, nu) of
+    Hbar * S * ([shell(mu) == s] + [shell(nu) == s]), with the oracle's S."""
+    from tblite_amd.batch import Batch, h0_integrals
+    mols = [(np.array([1]), np.zeros((1, 3))), (np.array([16]), np.array([[1.0, 2.0, 3.0]])),
+            (np.array([8]), np.array([[-4.0, 0.5, 0.0]])), (np.array([17]), np.zeros((1, 3)))]
+    b = Batch(mols)
+    _say(f"[gpu] lone atoms only nmol={b.nmol}")
+    rng = np.random.default_rng(9)
+    cn = torch.tensor(rng.uniform(0.5, 3.5, b.nat), dtype=torch.float64, device="cuda")
+    se = b.selfenergy(cn)[0].contiguous()
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+    Sb, Hb = t(rng.standard_normal(b.ndense)), t(rng.standard_normal(b.ndense))
+    Db, Qb = t(rng.standard_normal(3 * b.ndense)), t(rng.standard_normal(6 * b.ndense))
+    xyz = b.tensor_xyz().requires_grad_(True)
+    s = se.detach().requires_grad_(True)
+    out = h0_integrals(b, xyz, s)
+    torch.autograd.backward(out, (Sb, Db, Qb, Hb))
+    torch.cuda.synchronize()
+    assert float(xyz.grad.abs().max()) == 0.0
+    se_h, Hb_h, dse = se.cpu().numpy(), Hb.cpu().numpy(), s.grad.cpu().numpy()
+    S = out[0].detach().cpu().numpy()
+    for m, mol in enumerate(mols):
+        st = structure.setup(structure.Structure(*mol))
+        shs = slice(int(b.sh_off[m]), int(b.sh_off[m + 1]))
+        ref = O.get_hamiltonian(st, se_h[shs])
+        n = st.bas.nao
+        a = int(b.dense_off[m])
+        assert np.abs(ref[0].reshape(-1, order="F") - S[a: a + n * n]).max() < TOL_INT
+        # shell of every AO of the atom, in the batch's shell numbering
+        ao_sh = np.concatenate([np.full(2 * c.ang + 1, k) for k, c in
+                                enumerate(b.spec.basis_cgto[b.ids[m]])])
+        w = 0.5 * Hb_h[a: a + n * n].reshape(n, n, order="F") * ref[0]
+        want = np.array([w[ao_sh == k, :].sum() + w[:, ao_sh == k].sum()
+                         for k in range(shs.stop - shs.start)])
+        assert np.abs(dse[shs] - want).max() < TOL_GRAD
