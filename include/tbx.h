@@ -241,6 +241,16 @@ typedef struct {
 int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles);
 int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles);
 
+/* Order of the atom pairs in the work units, no counterpart in the reference:
+ * with a geometry hint xyz (host, bohr, [3*nat]), each molecule's pairs
+ * closer than cutoff + skin there go into units first and the others into
+ * units of their own, so that the gradient kernels skip whole units with no
+ * pair inside the cutoff.  Every pair stays in exactly one unit and every
+ * pair is still screened on the device, so the results do not depend on the
+ * hint, only the speed does.  xyz NULL restores the geometry-independent
+ * order.  Rebuilds the units (host) and waits for the context's stream. */
+int tbx_batch_order_pairs(tbx_batch *b, const double *xyz, double skin);
+
 /* Work-unit plans of a batch description, built on the host only (no
  * context, no device) with `tiles` (NULL: the defaults), and checked against
  * what the kernels assume; no counterpart in the reference (an inspection
@@ -259,10 +269,12 @@ typedef struct {
     long long bad_pairs;  /* pairs missing, repeated, or not of one molecule with j <= i */
     long long bad_elems;  /* elements written 0 or > 1 times, or reading outside the unit */
     long long bad_units;  /* units over a cap */
+    long long nunit_grad_live; /* gradient units with a pair within the cutoff at xyz */
 } tbx_plan_stats;
 
-int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles *tiles,
-                         tbx_plan_stats *out);
+/* xyz, skin: the pair-order hint of tbx_batch_order_pairs (xyz may be NULL). */
+int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles *tiles, const double *xyz,
+                         double skin, tbx_plan_stats *out);
 
 /* Measurement helper, no counterpart in the reference: copies n doubles
  * d_src -> d_dst with a plain streaming kernel (k_copy) on `stream`.  Its

@@ -56,23 +56,36 @@ class _Description:
                                 ct.pointer(self.h0), self.cutoff)
 
 
-def plan_stats(molecules, tiles: dict | None = None) -> dict:
+def plan_stats(molecules, tiles: dict | None = None, hint: bool = False,
+               skin: float = 1.0) -> dict:
     """Work-unit plans of a batch of ``(numbers, xyz_bohr)``, built and checked
     on the host (no device): unit, pair, task and element counts, plan bytes,
     and the number of violations of what the kernels assume (``bad_pairs``,
     ``bad_elems``, ``bad_units``; all zero for a correct planner).  ``tiles``
-    as in ``Batch.set_tiles`` (None: the library defaults)."""
+    as in ``Batch.set_tiles`` (None: the library defaults); ``hint``: order
+    the pairs near-first by the molecules' own geometry (``Batch.order_pairs``)
+    and count the gradient units with a pair inside the cutoff there
+    (``nunit_grad_live``)."""
     d = _Description(molecules)
     t = None if tiles is None else ct.pointer(L.Tiles(**tiles))
+    xyz = None
+    if hint:
+        xyz = np.ascontiguousarray(np.concatenate(
+            [np.asarray(x, dtype=np.float64).reshape(-1) for _, x in molecules]))
     out = L.PlanStats()
-    L.check(L.lib().tbx_batch_plan_stats(ct.byref(d.desc), t, ct.byref(out)))
+    L.check(L.lib().tbx_batch_plan_stats(
+        ct.byref(d.desc), t, None if xyz is None else xyz.ctypes.data_as(ct.c_void_p),
+        skin, ct.byref(out)))
     return {name: int(getattr(out, name)) for name, _ in L.PlanStats._fields_}
 
 
 class Batch:
-    """Batched GFN2 S/D/Q/H0 + gradient for a list of ``(numbers, xyz_bohr)``."""
+    """Batched GFN2 S/D/Q/H0 + gradient for a list of ``(numbers, xyz_bohr)``.
 
-    def __init__(self, molecules, device: int = 0):
+    ``order_pairs``: order the atom pairs of the work units near-first by the
+    given geometry (``order_pairs()``); it changes the speed, not the results."""
+
+    def __init__(self, molecules, device: int = 0, order_pairs: bool = True, skin: float = 1.0):
         self.device = device
         d = _Description(molecules)
         self.nums, ids, self.spec, self.msh = d.nums, d.ids, d.spec, d.msh
@@ -99,6 +112,8 @@ class Batch:
                                           p(self.ao_off), p(self.dense_off)))
         # per-molecule views of the setup, for host-side references
         self._shell_tables()
+        if order_pairs:
+            self.order_pairs(self.xyz0, skin)
 
     def _shell_tables(self):
         spec = self.spec
@@ -226,6 +241,22 @@ class Batch:
         L.check(L.lib().tbx_batch_last_scf_timing(self.handle, v))
         return dict(zip(("add_vao_ms", "add_vmp_ms", "shell_charges_ms", "atomic_multipoles_ms"),
                         list(v)))
+
+    def order_pairs(self, xyz=None, skin: float = 1.0):
+        """Group each molecule's atom pairs closer than cutoff + ``skin`` (bohr)
+        at geometry ``xyz`` ([nat, 3] or [3*nat], bohr) into the first work
+        units and the others into units of their own, which the gradient
+        kernels skip while their pairs stay beyond the cutoff.  Every pair is
+        still screened on the device, so any geometry gives the same results;
+        ``xyz=None`` restores the geometry-independent order."""
+        x = None
+        if xyz is not None:
+            if isinstance(xyz, torch.Tensor):
+                xyz = xyz.detach().cpu().numpy()
+            x = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))
+            assert x.size == 3 * self.nat
+        L.check(L.lib().tbx_batch_order_pairs(
+            self.handle, None if x is None else x.ctypes.data_as(ct.c_void_p), float(skin)))
 
     def tiles(self) -> dict:
         """Current tile shapes (work-unit caps and chunk lengths) of the kernels."""

@@ -291,15 +291,33 @@ struct HostMols {
     std::vector<long long> mol_dense;
     std::vector<TbxSeg> useg, fseg;
     std::vector<TbxUnitHead> uhead, fhead;
+    // optional geometry hint (bohr, [3*nat]) for the order of the pairs in the
+    // units: pairs within sqrt(near2) there come first in each molecule
+    std::vector<double> hint_xyz;
+    double near2 = 0.0;
 };
 
 // Work units (see TbxSeg): the half list of every molecule, cut into units of
 // bounded task count.  Gradient: j < i (j <= i with lattice images); forward:
-// j <= i, so that the on-site blocks are written by the same kernel.
+// j <= i, so that the on-site blocks are written by the same kernel.  With a
+// geometry hint (molecules only), each molecule's pairs within the hint
+// distance come first and the others follow in units of their own: every pair
+// is still in exactly one unit and the kernels still screen every pair, so
+// the hint changes the speed, never the results.
 static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntrans, bool diag,
                              int max_tasks, int max_pairs, std::vector<TbxSeg> &useg,
                              std::vector<int> &unit_seg)
 {
+    const bool hinted = ntrans == 1 && hm.hint_xyz.size() == 3 * (size_t)hm.nat;
+    auto near = [&](int i, int j) {
+        if (!hinted || i == j) return true;
+        double r2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double d = hm.hint_xyz[3 * i + c] - hm.hint_xyz[3 * j + c];
+            r2 += d * d;
+        }
+        return r2 <= hm.near2;
+    };
     useg.clear();
     unit_seg.assign(1, 0);
     int tasks = 0, pairs = 0, segs = 0;
@@ -311,27 +329,33 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
     };
     for (int m = 0; m < hm.nmol; ++m) {
         const int a0 = hm.mol_atom[m], a1 = hm.mol_atom[m + 1];
-        for (int itr = 0; itr < ntrans; ++itr)
-            for (int i = a0; i < a1; ++i) {
-                const int jend = (diag || ntrans > 1) ? i + 1 : i;
-                const int nci = hs.ncart[hm.at_sp[i]];
-                open_seg = false;
-                for (int j = a0; j < jend; ++j) {
-                    const int c = nci * hs.ncart[hm.at_sp[j]];
-                    if (tasks + c > max_tasks || pairs == max_pairs ||
-                        (!open_seg && segs == TBX_UNIT_SEGS))
-                        close_unit();
-                    if (!open_seg) {
-                        useg.push_back(TbxSeg{i, j, j, itr});
-                        ++segs;
-                        open_seg = true;
+        for (int pass = 0; pass < (hinted ? 2 : 1); ++pass) {
+            for (int itr = 0; itr < ntrans; ++itr)
+                for (int i = a0; i < a1; ++i) {
+                    const int jend = (diag || ntrans > 1) ? i + 1 : i;
+                    const int nci = hs.ncart[hm.at_sp[i]];
+                    open_seg = false;
+                    for (int j = a0; j < jend; ++j) {
+                        if (hinted && near(i, j) != (pass == 0)) {
+                            open_seg = false;  // a segment is a run of consecutive j
+                            continue;
+                        }
+                        const int c = nci * hs.ncart[hm.at_sp[j]];
+                        if (tasks + c > max_tasks || pairs == max_pairs ||
+                            (!open_seg && segs == TBX_UNIT_SEGS))
+                            close_unit();
+                        if (!open_seg) {
+                            useg.push_back(TbxSeg{i, j, j, itr});
+                            ++segs;
+                            open_seg = true;
+                        }
+                        useg.back().j1 = j + 1;
+                        tasks += c;
+                        ++pairs;
                     }
-                    useg.back().j1 = j + 1;
-                    tasks += c;
-                    ++pairs;
                 }
-            }
-        close_unit();  // units never span molecules
+            close_unit();  // units never span molecules, nor near and far pairs
+        }
     }
 }
 
@@ -754,6 +778,7 @@ struct tbx_batch {
     HostSpecies hs;
     HostMols hm;
     tbx_tiles tiles = kDefaultTiles;
+    double cutoff = 0.0;  // neighbour cutoff (bohr)
     DevTables tab;
     DevBuf flag, dse_tmp;
     hipEvent_t ev[16] = {};  // pairs (start, end): forward, -, gradient, adjoint, 4 SCF kernels
@@ -1051,6 +1076,7 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     *out = nullptr;
     tbx_batch *b = new tbx_batch;
     b->ctx = ctx;
+    b->cutoff = desc->cutoff;
     int st = build_species(desc->bas, desc->h0, b->hs);
     if (!st) st = build_mols(b->hs, desc->nmol, desc->mol_nat, desc->id, b->hm);
     if (st) {
@@ -1100,6 +1126,42 @@ extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
     return TBX_OK;
 }
 
+// Geometry hint of the pair order (see build_units_list); the cutoff is the
+// batch's neighbour cutoff.
+static int set_hint(HostMols &hm, const double *xyz, double skin, double cutoff)
+{
+    hm.hint_xyz.clear();
+    hm.near2 = 0.0;
+    if (!xyz) return TBX_OK;
+    if (!(skin >= 0.0) || !std::isfinite(skin)) return fail(TBX_ERR_ARG, "skin %g", skin);
+    for (int k = 0; k < 3 * hm.nat; ++k)
+        if (!std::isfinite(xyz[k])) return fail(TBX_ERR_ARG, "coordinate %d is not finite", k);
+    hm.hint_xyz.assign(xyz, xyz + 3 * (size_t)hm.nat);
+    hm.near2 = (cutoff + skin) * (cutoff + skin);
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_order_pairs(tbx_batch *b, const double *xyz, double skin)
+{
+    if (!b) return fail(TBX_ERR_ARG, "null argument");
+    const std::vector<double> old_xyz = b->hm.hint_xyz;
+    const double old_near2 = b->hm.near2;
+    if (int st = set_hint(b->hm, xyz, skin, b->cutoff)) return st;
+    if (int st = build_units(b->hs, b->hm, 1, b->tiles)) {
+        b->hm.hint_xyz = old_xyz;  // keep the host copy of the current units
+        b->hm.near2 = old_near2;
+        (void)build_units(b->hs, b->hm, 1, b->tiles);
+        return st;
+    }
+    std::vector<double> trans(3, 0.0);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));  // no launch may still read the old units
+    HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    b->tab.mol_host.host = std::vector<char>();
+    return TBX_OK;
+}
+
 extern "C" int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles)
 {
     if (!b || !tiles) return fail(TBX_ERR_ARG, "null argument");
@@ -1134,13 +1196,14 @@ static PlanView plan_view(const std::vector<int> &plan, const TbxUnitHead &h)
 }
 
 extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles *tiles,
-                                    tbx_plan_stats *out)
+                                    const double *xyz, double skin, tbx_plan_stats *out)
 {
     if (!desc || !out) return fail(TBX_ERR_ARG, "null argument");
     HostSpecies hs;
     HostMols hm;
     int st = build_species(desc->bas, desc->h0, hs);
     if (!st) st = build_mols(hs, desc->nmol, desc->mol_nat, desc->id, hm);
+    if (!st) st = set_hint(hm, xyz, skin, desc->cutoff);
     if (st) return st;
     const tbx_tiles t = tiles ? *tiles : kDefaultTiles;
     if ((st = check_tiles(hs, &t))) return st;
@@ -1172,8 +1235,15 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
             if (ntask > cap_tasks || v.npair > cap_pairs || v.nseg > TBX_UNIT_SEGS ||
                 v.nspair > ntask || (fwd && h.nel > 2 * t.fwd_unit_tasks))
                 ++s.bad_units;
+            bool live = false;
             for (int p = 0; p < v.npair; ++p) {
                 const int i = v.pairs[10 * p], j = v.pairs[10 * p + 1];
+                if (xyz && i >= 0 && i < hm.nat && j >= 0 && j < hm.nat && i != j) {
+                    double r2 = 0.0;
+                    for (int c = 0; c < 3; ++c) r2 += (xyz[3 * i + c] - xyz[3 * j + c]) *
+                                                       (xyz[3 * i + c] - xyz[3 * j + c]);
+                    live = live || r2 <= desc->cutoff * desc->cutoff;
+                }
                 const bool ok = i >= 0 && i < hm.nat && j >= 0 && j < hm.nat &&
                                 hm.at_mol[i] == hm.at_mol[j] && (fwd ? j <= i : j < i) &&
                                 h.mol == hm.at_mol[i];
@@ -1186,6 +1256,7 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
                 unsigned char &c = seen[poff[m] + (i - a0) * n + (j - a0)];
                 if (c < 2) ++c;
             }
+            if (!fwd && live) ++s.nunit_grad_live;
             if (!fwd) continue;
             const int *ed = plan.data() + 4 * ((size_t)h.plan + v.words);
             const int m = h.mol;

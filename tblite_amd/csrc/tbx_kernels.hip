@@ -1196,14 +1196,20 @@ __global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
     load_unit(H, mt.plan, U);
     __syncthreads();
 
-    // (2) pair vectors; shell pairs: screening counts and H0 factors (one
-    // lane per shell pair)
+    // (2) pair vectors.  A unit with no pair within the cutoff has no work
+    // (the host groups each molecule's distant pairs into units of their own
+    // when given a geometry hint): it ends here, before any output.
+    bool live = false;
     if (tid < npair) {
         double v[3];
         pair_vec(mt, A.xyz, U.pinf[tid], U.pext[tid], v);
 #pragma unroll
         for (int c = 0; c < 3; ++c) U.pv[tid][c] = v[c];
+        const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        live = r2 >= kEps && r2 <= sp.cutoff2;
     }
+    if (!__syncthreads_or(live)) return;
+    // shell pairs: screening counts and H0 factors (one lane per shell pair)
     for (int k = tid; k < nspair; k += GTHREADS) {
         int kept, pi0;
         double hs, sesum, dsr;

@@ -69,7 +69,7 @@ class PlanStats(ct.Structure):
     _fields_ = [(n, ct.c_longlong) for n in (
         "nunit_fwd", "nunit_grad", "npair_fwd", "npair_grad", "ntask_fwd", "ntask_grad",
         "nelem", "ndense", "plan_bytes_fwd", "plan_bytes_grad", "bad_pairs", "bad_elems",
-        "bad_units")]
+        "bad_units", "nunit_grad_live")]
 
 
 _SIGS = {
@@ -108,8 +108,9 @@ _SIGS = {
     "tbx_probe_copy": (ct.c_int, [_P, _P, ct.c_longlong, _P]),
     "tbx_batch_set_tiles": (ct.c_int, [_P, ct.POINTER(Tiles)]),
     "tbx_batch_get_tiles": (ct.c_int, [_P, ct.POINTER(Tiles)]),
-    "tbx_batch_plan_stats": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(Tiles),
-                                        ct.POINTER(PlanStats)]),
+    "tbx_batch_plan_stats": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(Tiles), _P,
+                                        ct.c_double, ct.POINTER(PlanStats)]),
+    "tbx_batch_order_pairs": (ct.c_int, [_P, _P, ct.c_double]),
 }
 
 

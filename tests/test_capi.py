@@ -172,6 +172,7 @@ def _expected_plan_totals(mols):
     return tot
 
 
+@pytest.mark.parametrize("hint", [False, True])
 @pytest.mark.parametrize("tiles", [
     None,                                                    # library defaults
     dict(fwd_unit_tasks=100, fwd_unit_pairs=1, fwd_chunk=1,  # one atom pair per unit
@@ -179,19 +180,21 @@ def _expected_plan_totals(mols):
     dict(fwd_unit_tasks=192, fwd_unit_pairs=7, fwd_chunk=2,
          grad_unit_tasks=112, grad_unit_pairs=13, grad_chunk=4),
 ])
-def test_unit_plans_cover_every_pair_and_element_once(tiles):
+def test_unit_plans_cover_every_pair_and_element_once(tiles, hint):
     """Host work-unit planner, no GPU: every atom pair (i, j <= i) of every
     molecule is in exactly one forward unit and every (i, j < i) in exactly one
     gradient unit; every dense element is written by exactly one forward
     element descriptor, which reads only its own shell pair's tasks; no unit
     exceeds a cap.  Lone atoms (H, S), a diatomic, S/Cl d shells and molecules
-    that span many units; totals checked against the closed forms above."""
+    that span many units; totals checked against the closed forms above.  With
+    the geometry hint (near pairs first, distant ones in units of their own)
+    the same invariants hold."""
     from tblite_amd.batch import plan_stats
     rng = np.random.default_rng(11)
     mols = [(np.array([1]), np.zeros((1, 3))), (np.array([16]), np.zeros((1, 3))),
             (np.array([6, 17]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 3.3]]))]
     mols += [synthetic.random_molecule(n, rng) for n in (9, 23, 57, 140)]
-    st = plan_stats(mols, tiles)
+    st = plan_stats(mols, tiles, hint=hint)
     assert st["bad_pairs"] == 0 and st["bad_elems"] == 0 and st["bad_units"] == 0, st
     for k, v in _expected_plan_totals(mols).items():
         assert st[k] == v, (k, st[k], v)
@@ -219,11 +222,20 @@ def test_unit_plans_reject_tiles_beyond_the_kernels():
 
 def test_unit_plans_of_the_2048_molecule_workload():
     """The plans the bench runs (2048 molecules of 8-256 atoms, default
-    tiles): no violation, and the totals of the closed forms.  About 10 s."""
+    tiles), in the geometry-independent order and near-first by the
+    molecules' geometry: no violation, and the totals of the closed forms.
+    With the hint, a large share of the gradient units holds no pair within
+    the cutoff (the kernel skips them).  About 20 s."""
     from tblite_amd.batch import plan_stats
     mols = synthetic.molecule_batch(2048, 8, 256, seed=1004, workers=8)
-    st = plan_stats(mols)
-    assert st["bad_pairs"] == 0 and st["bad_elems"] == 0 and st["bad_units"] == 0, st
-    for k, v in _expected_plan_totals(mols).items():
-        assert st[k] == v, (k, st[k], v)
-    assert st["ndense"] == 144333216 and st["nelem"] == st["ndense"]
+    want = _expected_plan_totals(mols)
+    for hint in (False, True):
+        st = plan_stats(mols, hint=hint)
+        assert st["bad_pairs"] == 0 and st["bad_elems"] == 0 and st["bad_units"] == 0, st
+        for k, v in want.items():
+            assert st[k] == v, (hint, k, st[k], v)
+        assert st["ndense"] == 144333216 and st["nelem"] == st["ndense"]
+        if hint:
+            print("gradient units", st["nunit_grad"], "with a pair within the cutoff",
+                  st["nunit_grad_live"])
+            assert st["nunit_grad_live"] < 0.8 * st["nunit_grad"]

@@ -262,3 +262,35 @@ def test_tile_shapes_keep_results(batch_case):
             b.set_tiles(grad_chunk=0)
     finally:
         b.set_tiles(**default)
+
+
+def test_pair_order_hint_keeps_results(batch_case):
+    """The pair-order hint (near pairs first, distant ones in units the
+    gradient kernels skip) changes the speed, not the results: the forward,
+    the adjoint and the plain gradient agree without a hint, with the true
+    geometry and with a misleading one (the molecules stretched threefold, so
+    that the units the hint marks distant hold pairs within the cutoff)."""
+    mols, b, cn, se, dse = batch_case
+    _say("[gpu] pair-order hint")
+    rng = np.random.default_rng(23)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+    adj = [t(rng.normal(size=k * b.ndense)) for k in (1, 3, 6, 1)]
+    P, W, vao, vdp, vqp = (t(a) for a in _random_inputs(b, 29))
+    xyz = b.tensor_xyz()
+
+    def run():
+        f = [x.clone() for x in b.forward(xyz, se)]
+        a = [x.clone() for x in b.adjoint(xyz, se, *adj)]
+        g = [x.clone() for x in b.gradient(xyz, se, dse, P, W, vao, vdp, vqp)]
+        torch.cuda.synchronize()
+        return f + a + g
+
+    try:
+        b.order_pairs(None)
+        ref = run()
+        for hint in (b.xyz0, 3.0 * b.xyz0):
+            b.order_pairs(hint, skin=0.5)
+            for k, (r, g) in enumerate(zip(ref, run())):
+                assert torch.abs(r - g).max().item() < 1e-10, k
+    finally:
+        b.order_pairs(b.xyz0)
