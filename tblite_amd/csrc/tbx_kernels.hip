@@ -622,6 +622,33 @@ __device__ __forceinline__ void put(double *p, double v, bool acc)
     if (acc) unsafeAtomicAdd(p, v); else *p = v;
 }
 
+// The elements of a forward unit whose blocks are all zero (no pair inside
+// the cutoff): the addresses and the store pattern of the element phase.
+__device__ __forceinline__ void zero_elements(const FwdArgs &A, const FwdStage &U,
+                                              long long dense0, int nao, int nel, int d0, int d1)
+{
+    for (int base = 0; base < nel; base += NTHREADS) {
+        const int e = base + threadIdx.x;
+        if (e >= nel) continue;
+        const int dsc = base == 0 ? d0 : d1;
+        const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31;
+        const bool mirror = (dsc >> 25) & 1;
+        const int nu = U.pinf[p].aoi + nul, mu = U.pinf[p].aoj + muj;
+        const long long idx = mirror ? dense0 + (long long)mu * nao + nu
+                                     : dense0 + (long long)nu * nao + mu;
+        const double2 z = make_double2(0.0, 0.0);
+        A.S[idx] = 0.0;
+        A.H[idx] = 0.0;
+        const bool odd = (idx & 1) != 0;
+        *reinterpret_cast<double2 *>(A.D + 3 * idx + (odd ? 1 : 0)) = z;
+        A.D[3 * idx + (odd ? 0 : 2)] = 0.0;
+        double2 *q2 = reinterpret_cast<double2 *>(A.Q + 6 * idx);
+        q2[0] = z;
+        q2[1] = z;
+        q2[2] = z;
+    }
+}
+
 // One workgroup per forward unit.  (1) stage pairs and shell pairs; (2) one
 // lane per Cartesian-pair task counts its kept primitive pairs, the first
 // lane of each shell pair also forms its H0 factor; (3) chunks of kept pairs
@@ -657,14 +684,23 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const int d1 = tid + NTHREADS < nel ? edesc[tid + NTHREADS] : 0;
     __syncthreads();
 
-    // (2) pair vectors; shell pairs: screening counts and H0 factors (one
-    // lane per shell pair)
+    // (2) pair vectors.  A unit with no pair inside the cutoff (on-site
+    // pairs are inside) has all-zero blocks: it writes its zeros and ends
+    // (the host groups each molecule's distant pairs into units of their own
+    // when given a geometry hint).
+    bool live = false;
     if (tid < npair) {
         double v[3];
         pair_vec(mt, A.xyz, U.pinf[tid], U.pext[tid], v);
 #pragma unroll
         for (int c = 0; c < 3; ++c) U.pv[tid][c] = v[c];
+        live = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] <= sp.cutoff2;
     }
+    if (!__syncthreads_or(live)) {
+        if (!accum) zero_elements(A, U, dense0, nao, nel, d0, d1);
+        return;
+    }
+    // shell pairs: screening counts and H0 factors (one lane per shell pair)
     for (int k = tid; k < nspair; k += NTHREADS) {
         int kept, pi0;
         double hs, sesum, dsr;
