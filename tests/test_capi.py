@@ -239,3 +239,21 @@ def test_unit_plans_of_the_2048_molecule_workload():
             print("gradient units", st["nunit_grad"], "with a pair within the cutoff",
                   st["nunit_grad_live"])
             assert st["nunit_grad_live"] < 0.8 * st["nunit_grad"]
+
+
+def test_pair_order_hint_validates_arguments():
+    """The pair-order hint refuses a negative or non-finite skin and
+    non-finite coordinates (status TBX_ERR_ARG, nothing built); a null batch
+    is an argument error, not a crash."""
+    from tblite_amd.batch import plan_stats
+    mols = [(np.array([6, 1]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 2.0]]))]
+    assert plan_stats(mols, hint=True, skin=0.0)["bad_elems"] == 0
+    for skin in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(L.TbxError) as e:
+            plan_stats(mols, hint=True, skin=skin)
+        assert e.value.code == TBX_ERR_ARG
+    with pytest.raises(L.TbxError) as e:
+        plan_stats([(np.array([6, 1]), np.array([[0.0, 0.0, 0.0], [0.0, np.nan, 2.0]]))],
+                   hint=True)
+    assert e.value.code == TBX_ERR_ARG
+    assert L.lib().tbx_batch_order_pairs(None, None, 1.0) == TBX_ERR_ARG
