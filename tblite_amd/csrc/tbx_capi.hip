@@ -1130,12 +1130,15 @@ extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
 // batch's neighbour cutoff.
 static int set_hint(HostMols &hm, const double *xyz, double skin, double cutoff)
 {
+    if (xyz) {  // validate before anything changes
+        if (!(skin >= 0.0) || !std::isfinite(skin)) return fail(TBX_ERR_ARG, "skin %g", skin);
+        for (int k = 0; k < 3 * hm.nat; ++k)
+            if (!std::isfinite(xyz[k]))
+                return fail(TBX_ERR_ARG, "coordinate %d is not finite", k);
+    }
     hm.hint_xyz.clear();
     hm.near2 = 0.0;
     if (!xyz) return TBX_OK;
-    if (!(skin >= 0.0) || !std::isfinite(skin)) return fail(TBX_ERR_ARG, "skin %g", skin);
-    for (int k = 0; k < 3 * hm.nat; ++k)
-        if (!std::isfinite(xyz[k])) return fail(TBX_ERR_ARG, "coordinate %d is not finite", k);
     hm.hint_xyz.assign(xyz, xyz + 3 * (size_t)hm.nat);
     hm.near2 = (cutoff + skin) * (cutoff + skin);
     return TBX_OK;
