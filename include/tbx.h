@@ -132,6 +132,23 @@ int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *geom, const t
                                  const double *xmat, double *dEdcn, double *gradient,
                                  double *sigma);
 
+/* Adjoint (backward) of get_hamiltonian (xtb/h0.f90:220-429), differentiated
+ * the way get_hamiltonian_gradient (xtb/h0.f90:432-666) differentiates it.
+ * Given adjoints sbar(nao,nao), hbar(nao,nao), dbar(3,nao,nao) and
+ * qbar(6,nao,nao) of the four outputs, E = <sbar,S> + <hbar,H> + <dbar,D> +
+ * <qbar,Q>; dE/dxyz into gradient(3,nat), dE/dselfenergy into dEdse(nsh)
+ * and the strain derivative into sigma(3,3) (NULL: not wanted) are
+ * ACCUMULATED into.  The integrals and their derivatives are recomputed, no
+ * derivative tensor is stored.  The reference has no such routine: this is
+ * the backward of the differentiable path (tbx_batch_adjoint) for one
+ * structure, host arrays in and out, lattice images included. */
+int tbx_get_hamiltonian_adjoint(tbx_context *ctx, const tbx_geom *geom, const tbx_list *list,
+                                const tbx_basis *bas, const tbx_h0par *h0, const int *ish_at,
+                                const int *iao_sh, int nao, const double *selfenergy,
+                                const double *sbar, const double *hbar, const double *dbar,
+                                const double *qbar, double *gradient, double *dEdse,
+                                double *sigma);
+
 /* --- device-resident batched path ---------------------------------------- */
 
 /* A batch: nmol independent molecules sharing one species table.  Built once
@@ -161,6 +178,14 @@ int tbx_batch_sizes_get(const tbx_batch *b, tbx_batch_sizes *sizes);
  * ao_off[nmol+1], dense_off[nmol+1] (elements of S/H; x3 for D, x6 for Q). */
 int tbx_batch_offsets(const tbx_batch *b, int *atom_off, int *sh_off, int *ao_off,
                       long long *dense_off);
+
+/* Device pointers of the tbx_batch_* entry points must be ordinary
+ * (coarse-grained) device memory, as hipMalloc and torch allocate it: the
+ * gradient outputs (and the forward's with lattice images) are summed with
+ * hardware FP64 atomics, which fine-grained memory (hipMallocManaged,
+ * hipHostMalloc) does not guarantee.  Those sums run in no fixed order, so
+ * dE/dR, dE/dse, dEdcn and sigma may differ in the last bits from run to run
+ * (observed <= 1e-14 relative); every other output is bitwise reproducible. */
 
 /* Forward: dense S, D, Q, H per molecule into device buffers of
  * ndense, 3*ndense, 6*ndense, ndense doubles.  d_xyz: [3*nat], d_se: [nsh],
@@ -231,8 +256,11 @@ int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pai
  * and <= 64); kept primitive pairs are processed in chunks of *_chunk
  * (1..64) per work item.  Defaults: forward 256, 128, chunk 3; gradient
  * 128, 64, chunk 1.
- * tbx_batch_set_tiles rebuilds the units (host) and waits for the context's
- * stream; tblite_amd/tune.py chooses them by timing the kernels. */
+ * tbx_batch_set_tiles rebuilds the units (host) and waits for the whole
+ * device (every stream) before it replaces them, so no launch still reads
+ * the old units; tblite_amd/tune.py chooses them by timing the kernels.  A
+ * HIP graph captured before a tbx_batch_set_tiles or tbx_batch_order_pairs
+ * holds the old unit tables and counts: capture it again afterwards. */
 typedef struct {
     int fwd_unit_tasks, fwd_unit_pairs, fwd_chunk;
     int grad_unit_tasks, grad_unit_pairs, grad_chunk;
@@ -248,7 +276,8 @@ int tbx_batch_get_tiles(const tbx_batch *b, tbx_tiles *tiles);
  * pair inside the cutoff.  Every pair stays in exactly one unit and every
  * pair is still screened on the device, so the results do not depend on the
  * hint, only the speed does.  xyz NULL restores the geometry-independent
- * order.  Rebuilds the units (host) and waits for the context's stream. */
+ * order.  Rebuilds the units (host) and waits for the whole device (every
+ * stream) before it replaces them. */
 int tbx_batch_order_pairs(tbx_batch *b, const double *xyz, double skin);
 
 /* Work-unit plans of a batch description, built on the host only (no

@@ -230,3 +230,26 @@ def get_hamiltonian_gradient(setup, selfenergy, dsedcn, vao, vdp, vqp, pmat, xma
     if st != 0:
         raise ValueError("oracle: unsupported basis")
     return dEdcn, gradient, sigma
+
+
+def get_hamiltonian_adjoint(setup, selfenergy, sbar, hbar, dbar, qbar, gradient=None, dEdse=None,
+                            sigma=None):
+    """Adjoint of ``get_hamiltonian``: for E = <sbar,S> + <hbar,H> + <dbar,D> +
+    <qbar,Q> (Fortran-ordered (nao,nao), (nao,nao), (3,nao,nao), (6,nao,nao)
+    adjoints) accumulates dE/dxyz into ``gradient`` (3,nat), dE/dselfenergy
+    into ``dEdse`` (nsh) and the strain derivative into ``sigma`` (3,3)."""
+    mol, nao = setup.mol, setup.bas.nao
+    sb = np.asfortranarray(np.asarray(sbar, dtype=np.float64).reshape(nao, nao, order="F"))
+    hb = np.asfortranarray(np.asarray(hbar, dtype=np.float64).reshape(nao, nao, order="F"))
+    db = np.asfortranarray(np.asarray(dbar, dtype=np.float64).reshape(3, nao, nao, order="F"))
+    qb = np.asfortranarray(np.asarray(qbar, dtype=np.float64).reshape(6, nao, nao, order="F"))
+    gradient = np.zeros((3, mol.nat), order="F") if gradient is None else gradient
+    dEdse = np.zeros(setup.bas.nsh) if dEdse is None else dEdse
+    sigma = np.zeros((3, 3), order="F") if sigma is None else sigma
+    se = _f64(selfenergy)
+    args, keep = _common_args(setup)
+    st = lib().orc_get_hamiltonian_adjoint(*args, _p(se), _p(sb), _p(hb), _p(db), _p(qb),
+                                           _p(gradient), _p(dEdse), _p(sigma))
+    if st != 0:
+        raise ValueError("oracle: unsupported basis")
+    return gradient, dEdse, sigma

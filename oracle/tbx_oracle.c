@@ -24,6 +24,9 @@
  *   orc_new_adjacency_list    adjlist.f90:96-143
  *   orc_get_hamiltonian       xtb/h0.f90:220-429 (do_diat_scale = .false. for GFN2)
  *   orc_get_hamiltonian_gradient  xtb/h0.f90:432-666
+ *   orc_get_hamiltonian_adjoint   xtb/h0.f90:220-429 differentiated as in :432-666
+ *                             (the adjoint of get_hamiltonian for arbitrary
+ *                             output adjoints; test-only, no reference routine)
  *   orc_get_multipole_integrals_lat  integral/native/integrals.f90:1394-1465
  *                             (its S and S, D are get_overlap_lat :411-470 and
  *                             get_dipole_integrals_lat :994-1059)
@@ -885,5 +888,144 @@ int orc_get_hamiltonian_gradient(int nat, int nid, int msh, int nsh, int nao, co
         free(g_loc);
         free(cn_loc);
     }
+    return 0;
+}
+
+/* Adjoint of get_hamiltonian (xtb/h0.f90:220-429, differentiated the way
+ * get_hamiltonian_gradient differentiates it, h0.f90:432-666).  Given the
+ * adjoints sbar(nao,nao), hbar(nao,nao), dbar(3,nao,nao), qbar(6,nao,nao) of
+ * the four outputs, E = <sbar,S> + <hbar,H> + <dbar,D> + <qbar,Q> and this
+ * accumulates dE/dxyz into gradient(3,nat), dE/dselfenergy into dEdse(nsh)
+ * and the strain derivative into sigma(3,3).  Every half-list pair contributes through the blocks
+ * get_hamiltonian writes for it: block A (rows of jat, columns of iat,
+ * operator on iat) and, for iat /= jat, its mirror (operator on jat). */
+int orc_get_hamiltonian_adjoint(int nat, int nid, int msh, int nsh, int nao, const int *id,
+                                const double *xyz, int ntr, const double *trans, const int *inl,
+                                const int *nnl, const int *nlat, const int *nltr,
+                                const int *nsh_id, const int *ish_at, const int *iao_sh,
+                                const orc_cgto *cgto, double intcut, const double *hscale,
+                                const double *shpoly, const double *rad,
+                                const double *selfenergy, const double *sbar, const double *hbar,
+                                const double *dbar, const double *qbar, double *gradient,
+                                double *dEdse, double *sigma)
+{
+    orc_system sys = {nat, nid, msh, nsh, nao, id, xyz, nsh_id, ish_at, iao_sh, cgto,
+                      intcut, hscale, shpoly, rad};
+    const double eps = 2.220446049250313e-16;
+    for (int isp = 0; isp < nid; ++isp)
+        for (int ish = 0; ish < nsh_id[isp]; ++ish)
+            if (cgto[ish + msh * isp].ang > 2) return -1;
+#define E_(r, c) ((size_t)(r) + (size_t)nao * (c))
+
+#pragma omp parallel
+    {
+        double *g_loc = calloc((size_t)3 * nat, sizeof(double));
+        double *se_loc = calloc((size_t)nsh, sizeof(double));
+        double sig_loc[9] = {0};
+        double stmp[25], dtmp[75], qtmp[150], dstmp[75];
+        double ddtmpi[225], dqtmpi[450], ddtmpj[225], dqtmpj[450];
+        double stmp0[25], dtmp0[75], qtmp0[150];
+#pragma omp for schedule(dynamic)
+        for (int iat = 0; iat < nat; ++iat) {
+            int izp = id[iat], is = ish_at[iat], nsi = nsh_id[izp];
+            for (int img = 0; img < nnl[iat]; ++img) {
+                int jat = nlat[inl[iat] + img], itr = nltr[inl[iat] + img];
+                int jzp = id[jat], js = ish_at[jat], nsj = nsh_id[jzp];
+                double vec[3];
+                for (int k = 0; k < 3; ++k)
+                    vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k] - trans[3 * itr + k];
+                double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+                if (r2 <= eps) continue;
+                double rr = sqrt(sqrt(r2) / (rad[jzp] + rad[izp]));
+                int mirror = iat != jat;
+                double dG[3] = {0, 0, 0};
+                for (int ish = 0; ish < nsi; ++ish) {
+                    int ii = iao_sh[is + ish];
+                    const orc_cgto *ci = &cgto[ish + msh * izp];
+                    double shpolyi = 1.0 + shpoly[ish + msh * izp] * rr;
+                    for (int jsh = 0; jsh < nsj; ++jsh) {
+                        int jj = iao_sh[js + jsh];
+                        const orc_cgto *cj = &cgto[jsh + msh * jzp];
+                        orc_multipole_grad_cgto(cj, ci, r2, vec, intcut, stmp, dtmp, qtmp, dstmp,
+                                                ddtmpj, dqtmpj, ddtmpi, dqtmpi);
+                        double shpolyj = 1.0 + shpoly[jsh + msh * jzp] * rr;
+                        double shp = shpolyi * shpolyj;
+                        double dshpoly = (shpolyi * shpoly[jsh + msh * jzp] +
+                                          shpolyj * shpoly[ish + msh * izp]) *
+                                         0.5 * rr / r2;
+                        double hs = HSCALE(&sys, jsh, ish, jzp, izp) * shp;
+                        double hij = 0.5 * (selfenergy[is + ish] + selfenergy[js + jsh]) * hs;
+                        int nj = nsph(cj->ang), ni = nsph(ci->ang);
+                        double hsum = 0.0;  /* sum of hbar * S over the block (and mirror) */
+                        for (int iao = 0; iao < ni; ++iao)
+                            for (int jao = 0; jao < nj; ++jao) {
+                                int ij = jao + nj * iao;
+                                size_t ea = E_(jj + jao, ii + iao), em = E_(ii + iao, jj + jao);
+                                double sb = sbar[ea], hb = hbar[ea];
+                                if (mirror) {
+                                    sb += sbar[em];
+                                    hb += hbar[em];
+                                }
+                                hsum += hb * stmp[ij];
+                                for (int a = 0; a < 3; ++a) {
+                                    /* dS/dR_i and d(S shpoly)/dR_i */
+                                    double t = (sb + hb * hij) * dstmp[3 * ij + a] +
+                                               hb * hij * stmp[ij] * dshpoly / shp * vec[a];
+                                    for (int c = 0; c < 3; ++c) {
+                                        t += dbar[c + 3 * ea] * ddtmpi[9 * ij + a + 3 * c];
+                                        if (mirror)
+                                            t += dbar[c + 3 * em] * ddtmpj[9 * ij + a + 3 * c];
+                                    }
+                                    for (int c = 0; c < 6; ++c) {
+                                        t += qbar[c + 6 * ea] * dqtmpi[18 * ij + a + 3 * c];
+                                        if (mirror)
+                                            t += qbar[c + 6 * em] * dqtmpj[18 * ij + a + 3 * c];
+                                    }
+                                    dG[a] += t;
+                                }
+                            }
+                        se_loc[is + ish] += 0.5 * hs * hsum;
+                        se_loc[js + jsh] += 0.5 * hs * hsum;
+                    }
+                }
+                for (int k = 0; k < 3; ++k) {
+                    g_loc[3 * iat + k] += dG[k];
+                    g_loc[3 * jat + k] -= dG[k];
+                }
+                /* every block is counted once here (a self-image pair iat == jat has no
+                 * mirror), so each pair adds the symmetrised vec x dE/dvec */
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        sig_loc[a + 3 * b] += 0.5 * (vec[a] * dG[b] + dG[a] * vec[b]);
+            }
+            /* onsite block: H = S (se_i + se_j)/2, no geometry dependence */
+            double vec0[3] = {0.0, 0.0, 0.0};
+            for (int ish = 0; ish < nsi; ++ish) {
+                int ii = iao_sh[is + ish];
+                const orc_cgto *ci = &cgto[ish + msh * izp];
+                for (int jsh = 0; jsh < nsi; ++jsh) {
+                    int jj = iao_sh[is + jsh];
+                    const orc_cgto *cj = &cgto[jsh + msh * izp];
+                    orc_multipole_cgto(cj, ci, 0.0, vec0, intcut, stmp0, dtmp0, qtmp0);
+                    int nj = nsph(cj->ang), ni = nsph(ci->ang);
+                    double hsum = 0.0;
+                    for (int iao = 0; iao < ni; ++iao)
+                        for (int jao = 0; jao < nj; ++jao)
+                            hsum += hbar[E_(jj + jao, ii + iao)] * stmp0[jao + nj * iao];
+                    se_loc[is + ish] += 0.5 * hsum;
+                    se_loc[is + jsh] += 0.5 * hsum;
+                }
+            }
+        }
+#pragma omp critical
+        {
+            for (int k = 0; k < 3 * nat; ++k) gradient[k] += g_loc[k];
+            for (int k = 0; k < nsh; ++k) dEdse[k] += se_loc[k];
+            for (int k = 0; k < 9; ++k) sigma[k] += sig_loc[k];
+        }
+        free(g_loc);
+        free(se_loc);
+    }
+#undef E_
     return 0;
 }

@@ -122,6 +122,8 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
     if (!bas->nsh_id || !bas->ang || !bas->nprim || !bas->alpha || !bas->coeff || !h0->hscale ||
         !h0->shpoly || !h0->rad)
         return fail(TBX_ERR_ARG, "null basis/h0 table");
+    if (!(bas->intcut > 0.0) || !std::isfinite(bas->intcut))
+        return fail(TBX_ERR_ARG, "invalid primitive screening threshold intcut=%g", bas->intcut);
     const int nsp = bas->nid, msh = bas->msh;
     hs.nsp = nsp;
     hs.msh = msh;
@@ -1067,6 +1069,61 @@ extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *ge
     return TBX_OK;
 }
 
+// Adjoint of get_hamiltonian for one structure: the adjoint kernel of the
+// batched path on a one-molecule batch, host arrays in and out.
+extern "C" int tbx_get_hamiltonian_adjoint(tbx_context *ctx, const tbx_geom *geom,
+                                           const tbx_list *list, const tbx_basis *bas,
+                                           const tbx_h0par *h0, const int *ish_at,
+                                           const int *iao_sh, int nao, const double *selfenergy,
+                                           const double *sbar, const double *hbar,
+                                           const double *dbar, const double *qbar,
+                                           double *gradient, double *dEdse, double *sigma)
+{
+    if (!selfenergy || !sbar || !hbar || !dbar || !qbar || !gradient || !dEdse)
+        return fail(TBX_ERR_ARG, "null argument");
+    HostSpecies hs;
+    HostMols hm;
+    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, hs, hm);
+    if (st) return st;
+    hipStream_t s = ctx->stream;
+    const int nat = geom->nat;
+    const size_t n2 = (size_t)nao * nao;
+    HIP_TRY(ctx->se.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(hipMemcpyAsync(ctx->se.p, selfenergy, sizeof(double) * hm.nsh, hipMemcpyHostToDevice, s));
+    // the adjoints go into the buffers of the forward's outputs
+    HIP_TRY(ctx->S.reserve(sizeof(double) * n2));
+    HIP_TRY(ctx->H.reserve(sizeof(double) * n2));
+    HIP_TRY(ctx->D.reserve(sizeof(double) * 3 * n2));
+    HIP_TRY(ctx->Q.reserve(sizeof(double) * 6 * n2));
+    HIP_TRY(hipMemcpyAsync(ctx->S.p, sbar, sizeof(double) * n2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->H.p, hbar, sizeof(double) * n2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->D.p, dbar, sizeof(double) * 3 * n2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->Q.p, qbar, sizeof(double) * 6 * n2, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx->grad.reserve(sizeof(double) * 3 * nat));
+    HIP_TRY(ctx->dse.reserve(sizeof(double) * hm.nsh));
+    HIP_TRY(ctx->sigma.reserve(sizeof(double) * 9));
+    HIP_TRY(tbx_launch_fill(ctx->grad.as<double>(), 3LL * nat, 0.0, s));
+    HIP_TRY(tbx_launch_fill(ctx->dse.as<double>(), hm.nsh, 0.0, s));
+    HIP_TRY(tbx_launch_fill(ctx->sigma.as<double>(), 9, 0.0, s));
+    HIP_TRY(tbx_launch_gradient(1, &ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
+                                ctx->se.as<double>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                1, 0, 0, ctx->S.as<double>(), ctx->D.as<double>(),
+                                ctx->Q.as<double>(), ctx->H.as<double>(), ctx->grad.as<double>(),
+                                ctx->dse.as<double>(), ctx->sigma.as<double>(),
+                                list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), 0, s));
+    std::vector<double> g(3 * nat), e(hm.nsh), sg(9);
+    HIP_TRY(hipMemcpyAsync(g.data(), ctx->grad.p, sizeof(double) * 3 * nat, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(e.data(), ctx->dse.p, sizeof(double) * hm.nsh, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(sg.data(), ctx->sigma.p, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
+    st = check_flag(ctx);  // synchronises the stream
+    if (st) return st;
+    for (int k = 0; k < 3 * nat; ++k) gradient[k] += g[k];
+    for (int k = 0; k < hm.nsh; ++k) dEdse[k] += e[k];
+    if (sigma)
+        for (int k = 0; k < 9; ++k) sigma[k] += sg[k];
+    return TBX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // batched path
 
@@ -1074,6 +1131,8 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
 {
     if (!ctx || !desc || !out) return fail(TBX_ERR_ARG, "null argument");
     *out = nullptr;
+    if (!(desc->cutoff > 0.0) || !std::isfinite(desc->cutoff))
+        return fail(TBX_ERR_ARG, "invalid cutoff %g", desc->cutoff);
     tbx_batch *b = new tbx_batch;
     b->ctx = ctx;
     b->cutoff = desc->cutoff;
@@ -1119,7 +1178,9 @@ extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
     b->tiles = *tiles;
     std::vector<double> trans(3, 0.0);
     HIP_TRY(hipSetDevice(b->ctx->device));
-    HIP_TRY(hipStreamSynchronize(b->ctx->stream));  // no launch may still read the old units
+    // The batch's kernels run on the callers' streams: wait for every stream
+    // of the device, so that no launch still reads the units replaced here.
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     b->tab.mol_host.host = std::vector<char>();
@@ -1158,7 +1219,9 @@ extern "C" int tbx_batch_order_pairs(tbx_batch *b, const double *xyz, double ski
     }
     std::vector<double> trans(3, 0.0);
     HIP_TRY(hipSetDevice(b->ctx->device));
-    HIP_TRY(hipStreamSynchronize(b->ctx->stream));  // no launch may still read the old units
+    // The batch's kernels run on the callers' streams: wait for every stream
+    // of the device, so that no launch still reads the units replaced here.
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     b->tab.mol_host.host = std::vector<char>();
@@ -1202,6 +1265,8 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
                                     const double *xyz, double skin, tbx_plan_stats *out)
 {
     if (!desc || !out) return fail(TBX_ERR_ARG, "null argument");
+    if (!(desc->cutoff > 0.0) || !std::isfinite(desc->cutoff))
+        return fail(TBX_ERR_ARG, "invalid cutoff %g", desc->cutoff);
     HostSpecies hs;
     HostMols hm;
     int st = build_species(desc->bas, desc->h0, hs);

@@ -5,6 +5,8 @@
 !>   get_selfenergy            src/tblite/xtb/h0.f90:113-217
 !>   get_hamiltonian           src/tblite/xtb/h0.f90:220-429
 !>   get_hamiltonian_gradient  src/tblite/xtb/h0.f90:432-666
+!>   (and the adjoint of get_hamiltonian, the backward of the differentiable
+!>   path, which has no reference counterpart)
 !>   get_multipole_integrals, get_dipole_integrals, get_overlap (the full
 !>   double-loop builders)    src/tblite/integral/native/integrals.f90:1394-1465,
 !>                            994-1059, 411-470
@@ -23,7 +25,8 @@ module tbx_h0
 
    public :: tbx_h0_tables, tbx_h0_context, tbx_h0_release, tbx_h0_last_error
    public :: tbx_h0_get_selfenergy, tbx_h0_get_hamiltonian, tbx_h0_get_hamiltonian_gradient
-   public :: tbx_h0_get_multipole_integrals
+   public :: tbx_h0_get_multipole_integrals, tbx_h0_get_hamiltonian_adjoint
+   public :: tbx_h0_pack_tables, tbx_h0_default_context
 
    integer, parameter, public :: tbx_maxg = 12, tbx_maxsh = 4
 
@@ -40,6 +43,9 @@ module tbx_h0
       real(c_double), allocatable :: rad(:)             !< (nid), bohr
       real(c_double) :: intcut = 25.0_c_double
    end type tbx_h0_tables
+
+   !> Context of tbx_h0_default_context (one per process, device 0)
+   type(c_ptr), save :: default_ctx = c_null_ptr
 
    type, bind(C) :: basis_c
       integer(c_int) :: nid, msh
@@ -121,6 +127,21 @@ module tbx_h0
          integer(c_int) :: stat
       end function c_get_hamiltonian_gradient
 
+      function c_get_hamiltonian_adjoint(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, se, &
+            & sbar, hbar, dbar, qbar, gradient, dEdse, sigma) result(stat) &
+            & bind(C, name="tbx_get_hamiltonian_adjoint")
+         import :: c_int, c_ptr, geom_c, basis_c, h0par_c
+         type(c_ptr), value :: ctx
+         type(geom_c), intent(in) :: geom
+         type(c_ptr), value :: list
+         type(basis_c), intent(in) :: bas
+         type(h0par_c), intent(in) :: h0
+         type(c_ptr), value :: ish_at, iao_sh
+         integer(c_int), value :: nao
+         type(c_ptr), value :: se, sbar, hbar, dbar, qbar, gradient, dEdse, sigma
+         integer(c_int) :: stat
+      end function c_get_hamiltonian_adjoint
+
       function c_get_multipole_integrals(ctx, geom, bas, ish_at, iao_sh, nao, overlap, dpint, &
             & qpint) result(stat) bind(C, name="tbx_get_multipole_integrals")
          import :: c_int, c_ptr, geom_c, basis_c
@@ -151,9 +172,70 @@ contains
 
    subroutine tbx_h0_release(ctx)
       type(c_ptr), intent(inout) :: ctx
-      if (c_associated(ctx)) call c_context_delete(ctx)
+      if (c_associated(ctx)) then
+         if (c_associated(ctx, default_ctx)) default_ctx = c_null_ptr
+         call c_context_delete(ctx)
+      end if
       ctx = c_null_ptr
    end subroutine tbx_h0_release
+
+   !> The process's context on HIP device 0, created on first use and kept
+   !> until tbx_h0_release(ctx) is called on it (c_null_ptr on failure).  This
+   !> is the context a tblite-side forwarding body passes on every call.
+   function tbx_h0_default_context() result(ctx)
+      type(c_ptr) :: ctx
+      if (.not. c_associated(default_ctx)) default_ctx = c_context_new(0_c_int)
+      ctx = default_ctx
+   end function tbx_h0_default_context
+
+   !> Fill tab from tblite's basis_type and tb_hamiltonian fields, flattened:
+   !> per shell slot (ish, isp) the cgto's ang, nprim, alpha(1:nprim) and
+   !> coeff(1:nprim) (bas%cgto(ish, isp), basis/type.f90:43-56), bas%nsh_id,
+   !> bas%intcut, and h0%hscale(msh, msh, nid, nid), h0%shpoly(msh, nid),
+   !> h0%rad(nid) (xtb/h0.f90:39-67).  Slots beyond nsh_id(isp) are ignored.
+   subroutine tbx_h0_pack_tables(nsh_id, ang, nprim, alpha, coeff, intcut, hscale, shpoly, &
+         & rad, tab, stat)
+      integer, intent(in) :: nsh_id(:), ang(:, :), nprim(:, :)
+      real(c_double), intent(in) :: alpha(:, :, :), coeff(:, :, :)
+      real(c_double), intent(in) :: intcut
+      real(c_double), intent(in) :: hscale(:, :, :, :), shpoly(:, :), rad(:)
+      type(tbx_h0_tables), intent(out) :: tab
+      integer, intent(out) :: stat
+      integer :: nid, msh, isp, ish, np
+      nid = size(nsh_id)
+      msh = size(ang, 1)
+      stat = 1
+      if (msh < 1 .or. msh > tbx_maxsh .or. size(ang, 2) /= nid) return
+      if (any(shape(nprim) /= [msh, nid]) .or. size(alpha, 2) /= msh .or. &
+         & size(alpha, 3) /= nid .or. any(shape(coeff) /= shape(alpha))) return
+      if (any(shape(hscale) /= [msh, msh, nid, nid]) .or. any(shape(shpoly) /= [msh, nid]) .or. &
+         & size(rad) /= nid) return
+      tab%nid = nid
+      tab%msh = msh
+      tab%intcut = intcut
+      allocate(tab%nsh_id(nid), tab%ang(msh, nid), tab%nprim(msh, nid))
+      allocate(tab%alpha(tbx_maxg, msh, nid), tab%coeff(tbx_maxg, msh, nid))
+      tab%nsh_id = int(nsh_id, c_int)
+      tab%ang = 0
+      tab%nprim = 0
+      tab%alpha = 0.0_c_double
+      tab%coeff = 0.0_c_double
+      do isp = 1, nid
+         if (nsh_id(isp) < 1 .or. nsh_id(isp) > msh) return
+         do ish = 1, nsh_id(isp)
+            np = nprim(ish, isp)
+            if (np < 1 .or. np > tbx_maxg .or. np > size(alpha, 1)) return
+            tab%ang(ish, isp) = int(ang(ish, isp), c_int)
+            tab%nprim(ish, isp) = int(np, c_int)
+            tab%alpha(1:np, ish, isp) = alpha(1:np, ish, isp)
+            tab%coeff(1:np, ish, isp) = coeff(1:np, ish, isp)
+         end do
+      end do
+      tab%hscale = hscale
+      tab%shpoly = shpoly
+      tab%rad = rad
+      stat = 0
+   end subroutine tbx_h0_pack_tables
 
    !> Message of the last failing call ("" if none)
    function tbx_h0_last_error() result(msg)
@@ -320,5 +402,52 @@ contains
          & c_loc(vao), c_loc(vdp), c_loc(vqp), c_loc(pmat), c_loc(xmat), c_loc(dEdcn), &
          & c_loc(gradient), c_loc(sigma))
    end subroutine tbx_h0_get_hamiltonian_gradient
+
+   !> Adjoint of get_hamiltonian: with E = <sbar,S> + <hbar,H> + <dbar,D> +
+   !> <qbar,Q> (sbar, hbar (nao,nao), dbar (3,nao,nao), qbar (6,nao,nao)),
+   !> dE/dxyz into gradient(3,nat), dE/dselfenergy into dEdse(nsh) and the
+   !> strain derivative into sigma(3,3) accumulate.
+   subroutine tbx_h0_get_hamiltonian_adjoint(ctx, tab, id, xyz, trans, cutoff, ish_at, iao_sh, &
+         & nao, selfenergy, sbar, hbar, dbar, qbar, gradient, dEdse, sigma, stat, &
+         & inl, nnl, nlat, nltr)
+      type(c_ptr), intent(in) :: ctx
+      type(tbx_h0_tables), intent(in), target :: tab
+      integer, intent(in) :: id(:)
+      real(c_double), intent(in), target, contiguous :: xyz(:, :), trans(:, :)
+      real(c_double), intent(in) :: cutoff
+      integer(c_int), intent(in), target, contiguous :: ish_at(:), iao_sh(:)
+      integer, intent(in) :: nao
+      real(c_double), intent(in), target, contiguous :: selfenergy(:)
+      real(c_double), intent(in), target, contiguous :: sbar(:, :), hbar(:, :), dbar(:, :, :), &
+         & qbar(:, :, :)
+      real(c_double), intent(inout), target, contiguous :: gradient(:, :), dEdse(:), sigma(:, :)
+      integer, intent(out) :: stat
+      integer, intent(in), optional :: inl(:), nnl(:), nlat(:), nltr(:)
+      type(geom_c) :: geom
+      type(basis_c) :: bas
+      type(h0par_c) :: h0
+      type(list_c), target :: list
+      integer(c_int), allocatable, target :: id0(:), inl0(:), nnl0(:), nlat0(:), nltr0(:)
+      type(c_ptr) :: plist
+
+      id0 = int(id - 1, c_int)
+      bas = basis_c(int(tab%nid, c_int), int(tab%msh, c_int), c_loc(tab%nsh_id), c_loc(tab%ang), &
+         & c_loc(tab%nprim), c_loc(tab%alpha), c_loc(tab%coeff), tab%intcut)
+      h0 = h0par_c(c_loc(tab%hscale), c_loc(tab%shpoly), c_loc(tab%rad))
+      geom = geom_c(int(size(id), c_int), c_loc(id0), c_loc(xyz), int(size(trans, 2), c_int), &
+         & c_loc(trans), cutoff)
+      plist = c_null_ptr
+      if (present(inl) .and. present(nnl) .and. present(nlat) .and. present(nltr)) then
+         inl0 = int(inl, c_int)
+         nnl0 = int(nnl, c_int)
+         nlat0 = int(nlat - 1, c_int)
+         nltr0 = int(nltr - 1, c_int)
+         list = list_c(c_loc(inl0), c_loc(nnl0), int(sum(nnl), c_int), c_loc(nlat0), c_loc(nltr0))
+         plist = c_loc(list)
+      end if
+      stat = c_get_hamiltonian_adjoint(ctx, geom, plist, bas, h0, c_loc(ish_at), c_loc(iao_sh), &
+         & int(nao, c_int), c_loc(selfenergy), c_loc(sbar), c_loc(hbar), c_loc(dbar), &
+         & c_loc(qbar), c_loc(gradient), c_loc(dEdse), c_loc(sigma))
+   end subroutine tbx_h0_get_hamiltonian_adjoint
 
 end module tbx_h0

@@ -156,3 +156,33 @@ def get_hamiltonian_gradient(setup: Setup, selfenergy, dsedcn, vao, vdp, vqp, pm
         _p(a.ish_at), _p(a.iao_sh), nao, _p(se), _p(dse), nspin, _p(va), _p(vd), _p(vq), _p(pm),
         _p(xm), _p(dEdcn), _p(gradient), _p(sigma)))
     return dEdcn, gradient, sigma
+
+
+def get_hamiltonian_adjoint(setup: Setup, selfenergy, sbar, hbar, dbar, qbar, gradient, dEdse,
+                            sigma=None, device: int = 0):
+    """Adjoint (backward) of ``get_hamiltonian`` for one structure: with
+    ``E = <sbar,S> + <hbar,H> + <dbar,D> + <qbar,Q>`` (adjoints Fortran-ordered
+    ``(nao,nao)``, ``(nao,nao)``, ``(3,nao,nao)``, ``(6,nao,nao)``),
+    accumulates dE/dxyz into ``gradient(3,nat)``, dE/dselfenergy into
+    ``dEdse(nsh)`` and, if given, the strain derivative into ``sigma(3,3)``.
+    The integrals are recomputed on the device (no derivative tensors)."""
+    a = _Args(setup)
+    nao, nat, nsh = a.nao, setup.mol.nat, setup.bas.nsh
+
+    def fa(x, shape):
+        return np.asfortranarray(np.asarray(x, dtype=np.float64).reshape(shape, order="F"))
+
+    sb, hb = fa(sbar, (nao, nao)), fa(hbar, (nao, nao))
+    db, qb = fa(dbar, (3, nao, nao)), fa(qbar, (6, nao, nao))
+    outs = [(gradient, (3, nat)), (dEdse, (nsh,))]
+    if sigma is not None:
+        outs.append((sigma, (3, 3)))
+    for arr, shape in outs:
+        if arr.shape != shape or arr.dtype != np.float64 or not arr.flags.f_contiguous:
+            raise ValueError(f"output must be a Fortran-ordered float64 array of shape {shape}")
+    se = np.ascontiguousarray(selfenergy, dtype=np.float64)
+    L.check(L.lib().tbx_get_hamiltonian_adjoint(
+        L.context(device), ct.byref(a.geom), ct.byref(a.list), ct.byref(a.basis), ct.byref(a.h0),
+        _p(a.ish_at), _p(a.iao_sh), nao, _p(se), _p(sb), _p(hb), _p(db), _p(qb), _p(gradient),
+        _p(dEdse), None if sigma is None else _p(sigma)))
+    return gradient, dEdse, sigma

@@ -90,6 +90,9 @@ _SIGS = {
                                                 ct.POINTER(Basis), ct.POINTER(H0Par), _P, _P,
                                                 ct.c_int, _P, _P, ct.c_int, _P, _P, _P, _P, _P,
                                                 _P, _P, _P]),
+    "tbx_get_hamiltonian_adjoint": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(List),
+                                               ct.POINTER(Basis), ct.POINTER(H0Par), _P, _P,
+                                               ct.c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
     "tbx_batch_add_vao_to_h1": (ct.c_int, [_P, ct.c_int, _P, _P, _P, _P]),
     "tbx_batch_add_vmp_to_h1": (ct.c_int, [_P, ct.c_int, ct.c_int, _P, ct.c_int, _P, _P, _P]),
     "tbx_batch_mulliken_shell_charges": (ct.c_int, [_P, ct.c_int, _P, _P, _P, _P, _P]),

@@ -1,7 +1,8 @@
 """Stream-file exchange with the Fortran host driver of the tbx_h0 binding
 (tblite_amd/fortran/tbx_h0_driver.f90): the input carries one structure with
-its flattened GFN2 tables, density and potential in the driver's read order;
-the output the dense integrals and the accumulated gradient terms."""
+its flattened GFN2 tables, density, potential and output adjoints in the
+driver's read order; the output the dense integrals, the accumulated gradient
+terms and the adjoint's dE/dxyz, dE/dse and sigma."""
 from __future__ import annotations
 
 import numpy as np
@@ -11,7 +12,7 @@ from tblite_amd import lib as L
 DRIVER = L.LIB_PATH.parent / "tbx_h0_driver"
 
 
-def write_input(path, st, se, dse, vao, vdp, vqp, P, W):
+def write_input(path, st, se, dse, vao, vdp, vqp, P, W, bars):
     mol, bas, spec = st.mol, st.bas, st.spec
     nat, nid, msh, nao, nsh = mol.nat, mol.nid, st.msh, bas.nao, bas.nsh
     nspin = P.shape[2]
@@ -31,7 +32,7 @@ def write_input(path, st, se, dse, vao, vdp, vqp, P, W):
         np.ascontiguousarray(se, dtype="<f8"), np.ascontiguousarray(dse, dtype="<f8"),
         f64F(vao), f64F(vdp), f64F(vqp), f64F(P), f64F(W),
         i32(st.inl), i32(st.nnl), i32(np.asarray(st.nlat) + 1), i32(np.asarray(st.nltr) + 1),
-    ]
+    ] + [f64F(b) for b in bars]
     with open(path, "wb") as f:
         for p in parts:
             f.write(p.tobytes())
@@ -40,12 +41,12 @@ def write_input(path, st, se, dse, vao, vdp, vqp, P, W):
 def read_output(path, st):
     nat, nao = st.mol.nat, st.bas.nao
     raw = open(path, "rb").read()
-    stat = np.frombuffer(raw[:12], dtype="<i4")
-    v = np.frombuffer(raw[12:], dtype="<f8")
-    sizes = [nao * nao, 3 * nao * nao, 6 * nao * nao, nao * nao, nat, 3 * nat, 9,
-             nao * nao, 3 * nao * nao, 6 * nao * nao]
+    nsh = st.bas.nsh
+    stat = np.frombuffer(raw[:16], dtype="<i4")
+    v = np.frombuffer(raw[16:], dtype="<f8")
     shapes = [(nao, nao), (3, nao, nao), (6, nao, nao), (nao, nao), (nat,), (3, nat), (3, 3),
-              (nao, nao), (3, nao, nao), (6, nao, nao)]
+              (nao, nao), (3, nao, nao), (6, nao, nao), (3, nat), (nsh,), (3, 3)]
+    sizes = [int(np.prod(sh)) for sh in shapes]
     out, o = [], 0
     for n, sh in zip(sizes, shapes):
         out.append(v[o:o + n].reshape(sh, order="F"))

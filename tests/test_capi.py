@@ -110,12 +110,23 @@ def test_fortran_driver_reads_input_and_reports_missing_device(tmp_path):
     se, dse = st.selfenergy(np.full(st.mol.nat, 2.0))
     nao, nat = st.bas.nao, st.mol.nat
     P = np.asfortranarray(np.eye(nao)[:, :, None])
+    bars = (np.zeros((nao, nao)), np.zeros((nao, nao)), np.zeros((3, nao, nao)),
+            np.zeros((6, nao, nao)))
     write_input(tmp_path / "in.bin", st, se, dse, np.zeros((nao, 1)), np.zeros((3, nat, 1)),
-                np.zeros((6, nat, 1)), P, P)
+                np.zeros((6, nat, 1)), P, P, bars)
     r = subprocess.run([str(DRIVER), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 2, r.stdout + r.stderr
     assert "no context" in r.stdout and len(r.stdout.strip()) > len("tbx_h0_driver: no context:")
+    # tbx_h0_pack_tables rejects a shell with more primitives than a cgto holds
+    # (basis/type.f90:32), before any device call (exit status 3)
+    st.nprim = st.nprim.copy()
+    st.nprim[0] = 13
+    write_input(tmp_path / "bad.bin", st, se, dse, np.zeros((nao, 1)), np.zeros((3, nat, 1)),
+                np.zeros((6, nat, 1)), P, P, bars)
+    r = subprocess.run([str(DRIVER), str(tmp_path / "bad.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "rejected the tables" in r.stdout, r.stdout + r.stderr
 
 
 def test_full_builders_validate_arguments():

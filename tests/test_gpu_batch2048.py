@@ -1,0 +1,142 @@
+"""Parity of the batched path on the benchmark workload itself: the 2048
+molecules of 8-256 atoms (log-uniform, H/C/N/O/F/S/Cl, seed 1004) that
+bench.py times, with the same near-first pair order.  Every molecule's
+forward (S, D, Q, H0), plain gradient (dE/dR, dEdcn, sigma) and adjoint
+(dE/dR, dE/dse for non-symmetric random adjoints of the four outputs) is
+compared with the oracle.
+
+Tolerances: integrals and H0 1e-10, gradients 1e-8 (absolute).  Device
+results are copied to the host one molecule at a time, so host memory holds
+one molecule's dense blocks beside the oracle's."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tblite_amd import structure, synthetic
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL_INT = 1e-10
+TOL_GRAD = 1e-8
+NMOL, SEED = 2048, 1004
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def workload():
+    from tblite_amd.batch import Batch
+    t = time.perf_counter()
+    mols = synthetic.molecule_batch(NMOL, seed=SEED, workers=8)
+    b = Batch(mols)
+    sts = [structure.setup(structure.Structure(z, x)) for z, x in mols]
+    _say(f"[gpu] 2048-molecule workload: nat={b.nat} nao={b.nao} ndense={b.ndense} "
+         f"({time.perf_counter() - t:.1f} s set-up)")
+    rng = np.random.default_rng(SEED)
+    cn = torch.tensor(rng.uniform(0.5, 3.5, b.nat), dtype=torch.float64, device="cuda")
+    se, dse = b.selfenergy(cn)
+    return mols, sts, b, se.contiguous(), dse.contiguous()
+
+
+def _slices(b, m):
+    a0, a1 = int(b.atom_off[m]), int(b.atom_off[m + 1])
+    s0, s1 = int(b.sh_off[m]), int(b.sh_off[m + 1])
+    o0, o1 = int(b.ao_off[m]), int(b.ao_off[m + 1])
+    d0, d1 = int(b.dense_off[m]), int(b.dense_off[m + 1])
+    return (a0, a1), (s0, s1), (o0, o1), (d0, d1)
+
+
+def _dense(t, comps, d0, d1, n):
+    """Molecule block of a packed dense device tensor -> Fortran (comps, n, n)."""
+    v = t[comps * d0: comps * d1].cpu().numpy()
+    return v.reshape((comps, n, n), order="F") if comps > 1 else v.reshape((n, n), order="F")
+
+
+def test_forward_every_molecule(workload):
+    mols, sts, b, se, _ = workload
+    S, D, Q, H = b.forward(b.tensor_xyz(), se)
+    torch.cuda.synchronize()
+    se_h = se.cpu().numpy()
+    worst = 0.0
+    for m, st in enumerate(sts):
+        _, (s0, s1), (o0, o1), (d0, d1) = _slices(b, m)
+        n = o1 - o0
+        ref = O.get_hamiltonian(st, se_h[s0:s1])
+        for r, t, c, lab in zip(ref, (S, D, Q, H), (1, 3, 6, 1), "SDQH"):
+            err = float(np.abs(r - _dense(t, c, d0, d1, n)).max())
+            worst = max(worst, err)
+            assert err < TOL_INT, (m, lab, err)
+    _say(f"    forward: {b.nmol} molecules, max |GPU - oracle| = {worst:.2e}")
+
+
+def test_plain_gradient_every_molecule(workload):
+    mols, sts, b, se, dse = workload
+    rng = np.random.default_rng(SEED + 1)
+    P = np.empty(b.ndense)
+    W = np.empty(b.ndense)
+    for m in range(b.nmol):
+        _, _, (o0, o1), (d0, d1) = _slices(b, m)
+        n = o1 - o0
+        g = rng.normal(size=(n, n))
+        P[d0:d1] = (0.5 * (g + g.T)).ravel(order="F")
+        h = rng.normal(size=(n, n))
+        W[d0:d1] = (0.05 * (h + h.T)).ravel(order="F")
+    vao = rng.normal(scale=1e-2, size=b.nao)
+    vdp = rng.normal(scale=1e-3, size=3 * b.nat)
+    vqp = rng.normal(scale=1e-3, size=6 * b.nat)
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+    dEdcn, grad, sigma = b.gradient(b.tensor_xyz(), se, dse, dev(P), dev(W), dev(vao), dev(vdp),
+                                    dev(vqp))
+    torch.cuda.synchronize()
+    dEdcn, grad, sigma = dEdcn.cpu().numpy(), grad.cpu().numpy(), sigma.cpu().numpy()
+    se_h, dse_h = se.cpu().numpy(), dse.cpu().numpy()
+    worst = 0.0
+    for m, st in enumerate(sts):
+        (a0, a1), (s0, s1), (o0, o1), (d0, d1) = _slices(b, m)
+        n = o1 - o0
+        cr, gr, sr = O.get_hamiltonian_gradient(
+            st, se_h[s0:s1], dse_h[s0:s1], vao[o0:o1].reshape(n, 1),
+            vdp[3 * a0: 3 * a1].reshape(3, -1, 1, order="F"),
+            vqp[6 * a0: 6 * a1].reshape(6, -1, 1, order="F"),
+            P[d0:d1].reshape(n, n, 1, order="F"), W[d0:d1].reshape(n, n, 1, order="F"))
+        errs = (np.abs(grad[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
+                np.abs(dEdcn[a0:a1] - cr).max(),
+                np.abs(sigma[9 * m: 9 * m + 9].reshape(3, 3, order="F") - sr).max())
+        worst = max(worst, *errs)
+        assert max(errs) < TOL_GRAD, (m, errs)
+    _say(f"    plain gradient: {b.nmol} molecules, max |GPU - oracle| = {worst:.2e}")
+
+
+def test_adjoint_every_molecule(workload):
+    """Backward kernel (the autograd path) with non-symmetric standard-normal
+    adjoints of S, H0, D, Q: dE/dR and dE/dse of every molecule against the
+    oracle's adjoint of get_hamiltonian."""
+    mols, sts, b, se, _ = workload
+    g = torch.Generator(device="cuda")
+    g.manual_seed(SEED + 2)
+    bars = [torch.randn(c * b.ndense, generator=g, dtype=torch.float64, device="cuda")
+            for c in (1, 1, 3, 6)]
+    Sb, Hb, Db, Qb = bars
+    dxyz, dse_out = b.adjoint(b.tensor_xyz(), se, Sb, Db, Qb, Hb)
+    torch.cuda.synchronize()
+    dxyz, dse_out = dxyz.cpu().numpy(), dse_out.cpu().numpy()
+    se_h = se.cpu().numpy()
+    worst = 0.0
+    for m, st in enumerate(sts):
+        (a0, a1), (s0, s1), (o0, o1), (d0, d1) = _slices(b, m)
+        n = o1 - o0
+        mb = [_dense(t, c, d0, d1, n) for t, c in ((Sb, 1), (Hb, 1), (Db, 3), (Qb, 6))]
+        gr, er, _ = O.get_hamiltonian_adjoint(st, se_h[s0:s1], *mb)
+        errs = (np.abs(dxyz[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
+                np.abs(dse_out[s0:s1] - er).max())
+        worst = max(worst, *errs)
+        assert max(errs) < TOL_GRAD, (m, errs)
+    _say(f"    adjoint: {b.nmol} molecules, max |GPU - oracle| = {worst:.2e}")

@@ -208,3 +208,94 @@ def test_diffset_autograd_equals_plain_gradient(diff_batch):
     err = torch.abs(x.grad - grad).max().item()
     _say(f"    autograd vs plain gradient: max |diff| = {err:.2e}")
     assert err < TOL_GRAD
+
+
+def test_diffset_autograd_vs_oracle_gradient(diff_batch):
+    """Autograd dE/dR with the adjoints of the functional
+    get_hamiltonian_gradient differentiates, against the oracle's
+    get_hamiltonian_gradient (xtb/h0.f90:432-666): the 999-atom water cluster
+    and every one of the 512 molecules."""
+    from test_gpu_batch import _plain_adjoints
+
+    mols, b, cn = diff_batch
+    rng = np.random.default_rng(9)
+    P = np.zeros(b.ndense)
+    for m in range(b.nmol):
+        n = int(b.ao_off[m + 1] - b.ao_off[m])
+        g = rng.normal(size=(n, n))
+        P[int(b.dense_off[m]): int(b.dense_off[m]) + n * n] = (0.5 * (g + g.T)).ravel()
+    W = 0.05 * P
+    vao = rng.normal(scale=1e-2, size=b.nao)
+    vdp = rng.normal(scale=1e-3, size=3 * b.nat)
+    vqp = rng.normal(scale=1e-3, size=6 * b.nat)
+    Sb, Hb, Db, Qb = _plain_adjoints(b, P, W, vao, vdp, vqp)
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+    se, dse = b.selfenergy(cn)
+    x = b.tensor_xyz().requires_grad_(True)
+    o = h0_integrals(b, x, se.detach().contiguous())
+    torch.autograd.backward(o, (dev(Sb), dev(Db), dev(Qb), dev(Hb)))
+    torch.cuda.synchronize()
+    gx = x.grad.cpu().numpy()
+    se_h, dse_h = se.detach().cpu().numpy(), dse.cpu().numpy()
+    worst = 0.0
+    for m in range(b.nmol):
+        st = structure.setup(structure.Structure(*mols[m]))
+        a0, a1 = int(b.atom_off[m]), int(b.atom_off[m + 1])
+        s0, s1 = int(b.sh_off[m]), int(b.sh_off[m + 1])
+        o0, o1 = int(b.ao_off[m]), int(b.ao_off[m + 1])
+        d0, n = int(b.dense_off[m]), o1 - o0
+        _, gr, _ = O.get_hamiltonian_gradient(
+            st, se_h[s0:s1], dse_h[s0:s1], vao[o0:o1].reshape(n, 1),
+            vdp[3 * a0: 3 * a1].reshape(3, -1, 1, order="F"),
+            vqp[6 * a0: 6 * a1].reshape(6, -1, 1, order="F"),
+            P[d0: d0 + n * n].reshape(n, n, 1, order="F"),
+            W[d0: d0 + n * n].reshape(n, n, 1, order="F"))
+        err = float(np.abs(gx[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max())
+        worst = max(worst, err)
+        assert err < TOL_GRAD, (m, err)
+    _say(f"    autograd vs oracle gradient: {b.nmol} molecules, max |diff| = {worst:.2e}")
+
+
+def test_diffset_autograd_vs_oracle_adjoint(diff_batch):
+    """Autograd with symmetrised standard-normal adjoints (the FD test's
+    functional) against the oracle's adjoint of get_hamiltonian: dE/dR and
+    dE/dCN (through get_selfenergy) of every molecule and the water cluster."""
+    mols, b, cn = diff_batch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1005)
+
+    def sym_adjoint(comps):
+        out = torch.empty(comps * b.ndense, dtype=torch.float64, device="cuda")
+        for v in b.molecule_views(out, comps):
+            r = torch.randn(tuple(v.shape), generator=g, dtype=torch.float64, device="cuda")
+            v.copy_(0.5 * (r + r.transpose(0, 1)))
+        return out
+    Sb, Db, Qb, Hb = sym_adjoint(1), sym_adjoint(3), sym_adjoint(6), sym_adjoint(1)
+    x = b.tensor_xyz().requires_grad_(True)
+    cnv = cn.clone().requires_grad_(True)
+    se, _ = b.selfenergy(cnv)
+    S, D, Q, H = h0_integrals(b, x, se)
+    E = (Sb * S).sum() + (Db * D).sum() + (Qb * Q).sum() + (Hb * H).sum()
+    E.backward()
+    torch.cuda.synchronize()
+    gx, gcn = x.grad.cpu().numpy(), cnv.grad.cpu().numpy()
+    se_h = se.detach().cpu().numpy()
+    worst = 0.0
+    for m in range(b.nmol):
+        st = structure.setup(structure.Structure(*mols[m]))
+        a0, a1 = int(b.atom_off[m]), int(b.atom_off[m + 1])
+        s0, s1 = int(b.sh_off[m]), int(b.sh_off[m + 1])
+        o0, o1 = int(b.ao_off[m]), int(b.ao_off[m + 1])
+        d0, d1, n = int(b.dense_off[m]), int(b.dense_off[m + 1]), o1 - o0
+        bars = [t[c * d0: c * d1].cpu().numpy().reshape((c, n, n) if c > 1 else (n, n),
+                                                         order="F")
+                for t, c in ((Sb, 1), (Hb, 1), (Db, 3), (Qb, 6))]
+        gr, er, _ = O.get_hamiltonian_adjoint(st, se_h[s0:s1], *bars)
+        _, dsedcn = st.selfenergy(np.ones(st.mol.nat))  # -kcn, independent of cn
+        sh_at = np.repeat(np.arange(st.mol.nat), st.spec.nsh_id[st.mol.id])
+        cr = np.bincount(sh_at, weights=dsedcn * er, minlength=st.mol.nat)
+        errs = (np.abs(gx[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
+                np.abs(gcn[a0:a1] - cr).max())
+        worst = max(worst, *errs)
+        assert max(errs) < TOL_GRAD, (m, errs)
+    _say(f"    autograd vs oracle adjoint: {b.nmol} molecules, max |diff| = {worst:.2e}")

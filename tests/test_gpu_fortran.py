@@ -1,7 +1,9 @@
 """The Fortran iso_c_binding module (tblite_amd/fortran/tbx_h0.f90), driven by
 a Fortran host program, against the oracle: the drop-in boundary exercised
 from Fortran exactly as tblite's xtb_singlepoint would call it (tblite's
-1-based adjacency list, Fortran-ordered dense outputs, accumulated gradient)."""
+1-based adjacency list, Fortran-ordered dense outputs, accumulated gradient),
+with the tables packed by tbx_h0_pack_tables on the default context, and the
+adjoint of get_hamiltonian against the oracle's."""
 import subprocess
 import sys
 
@@ -38,14 +40,16 @@ def test_fortran_driver_parity(name, tmp_path):
     vao = np.asfortranarray(rng.normal(scale=1e-2, size=(nao, 1)))
     vdp = np.asfortranarray(rng.normal(scale=1e-3, size=(3, nat, 1)))
     vqp = np.asfortranarray(rng.normal(scale=1e-3, size=(6, nat, 1)))
+    bars = (rng.normal(size=(nao, nao)), rng.normal(size=(nao, nao)),
+            rng.normal(size=(3, nao, nao)), rng.normal(size=(6, nao, nao)))
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    write_input(fin, st, se, dse, vao, vdp, vqp, P, W)
+    write_input(fin, st, se, dse, vao, vdp, vqp, P, W, bars)
     r = subprocess.run([str(DRIVER), str(fin), str(fout)], capture_output=True, text=True,
                        timeout=120)
     sys.stdout.write(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    stat, (S, D, Q, H, dEdcn, grad, sigma, S2, D2, Q2) = read_output(fout, st)
-    assert list(stat) == [0, 0, 0]
+    stat, (S, D, Q, H, dEdcn, grad, sigma, S2, D2, Q2, ag, adse, asig) = read_output(fout, st)
+    assert list(stat) == [0, 0, 0, 0]
     for ref, got, lab in zip(O.get_hamiltonian(st, se), (S, D, Q, H), "SDQH"):
         assert np.abs(ref - got).max() < 1e-10, lab
     cr, gr, sr = O.get_hamiltonian_gradient(st, se, dse, vao, vdp, vqp, P, W)
@@ -55,3 +59,8 @@ def test_fortran_driver_parity(name, tmp_path):
     # tbx_h0_get_multipole_integrals against the oracle's full double loop
     for ref, got in zip(O.get_multipole_integrals_lat(st), (S2, D2, Q2)):
         assert np.abs(ref - got).max() < 1e-10
+    # tbx_h0_get_hamiltonian_adjoint against the oracle's adjoint
+    rg, rdse, rsig = O.get_hamiltonian_adjoint(st, se, *[np.asfortranarray(b) for b in bars])
+    assert np.abs(rg - ag).max() < 1e-8
+    assert np.abs(rdse - adse).max() < 1e-8
+    assert np.abs(rsig - asig).max() < 1e-8

@@ -118,6 +118,41 @@ def test_periodic_images_parity():
     assert np.abs(sr - sg).max() < TOL_GRAD
 
 
+def _periodic_case():
+    z = np.array([8, 1, 1, 6, 1, 16])
+    x = np.array([[0.0, 0.0, 0.0], [1.8, 0.0, 0.0], [-0.5, 1.7, 0.0], [4.0, 3.0, 3.0],
+                  [5.9, 3.2, 3.4], [1.0, 5.0, 6.5]])
+    trans = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)],
+                     dtype=float) * 14.0
+    return structure.setup(structure.Structure(z, x), trans=trans)
+
+
+@pytest.mark.parametrize("name", ["sulfur_dimer", "octane", "mixed80", "periodic"])
+def test_get_hamiltonian_adjoint_parity(name):
+    """Host-pointer adjoint of get_hamiltonian against the oracle's adjoint
+    with non-symmetric random adjoints (dE/dxyz, dE/dse, sigma; accumulated)."""
+    st = _periodic_case() if name == "periodic" else _case(name)
+    nao, nat, nsh = st.bas.nao, st.mol.nat, st.bas.nsh
+    _flush(f"[gpu] get_hamiltonian_adjoint {name} nat={nat} nao={nao}")
+    rng = np.random.default_rng(13)
+    se, _ = st.selfenergy(rng.uniform(0.5, 3.5, nat))
+    bars = [np.asfortranarray(rng.normal(size=sh)) for sh in
+            ((nao, nao), (nao, nao), (3, nao, nao), (6, nao, nao))]
+    rg, rdse, rsig = O.get_hamiltonian_adjoint(st, se, *bars)
+    g0 = rng.normal(size=(3, nat))
+    e0 = rng.normal(size=nsh)
+    s0 = rng.normal(size=(3, 3))
+    gg, gdse, gsig = (np.asfortranarray(g0.copy()), e0.copy(), np.asfortranarray(s0.copy()))
+    h0.get_hamiltonian_adjoint(st, se, *bars, gg, gdse, gsig)
+    assert np.abs(rg - (gg - g0)).max() < TOL_GRAD
+    assert np.abs(rdse - (gdse - e0)).max() < TOL_GRAD
+    assert np.abs(rsig - (gsig - s0)).max() < TOL_GRAD
+    # sigma is optional
+    g2, d2 = np.zeros((3, nat), order="F"), np.zeros(nsh)
+    h0.get_hamiltonian_adjoint(st, se, *bars, g2, d2)
+    assert np.abs(g2 - rg).max() < TOL_GRAD and np.abs(d2 - rdse).max() < TOL_GRAD
+
+
 @pytest.mark.parametrize("name", ["sulfur_dimer", "octane", "periodic"])
 def test_full_builders_parity(name):
     """get_multipole_integrals / get_dipole_integrals / get_overlap against the

@@ -491,3 +491,120 @@ def test_lat_builders_match_half_list(periodic):
         lhs = D - np.transpose(D, (0, 2, 1))
         rhs = (R[:, None, :] - R[None, :, :]).transpose(2, 0, 1) * S[None]
         assert np.abs(lhs - rhs).max() < 1e-10
+
+
+# -- adjoint of get_hamiltonian --------------------------------------------------
+
+def _with_geometry(st, xyz, trans=None):
+    """``st`` with new coordinates (and lattice vectors), the same neighbour list."""
+    import dataclasses
+
+    from tblite_amd import structure
+    mol = structure.Structure(st.mol.numbers, xyz)
+    return dataclasses.replace(st, mol=mol, trans=st.trans if trans is None else trans)
+
+
+def _adjoint_energy(st, se, bars):
+    S, D, Q, H = O.get_hamiltonian(st, se)
+    sb, hb, db, qb = bars
+    return np.sum(sb * S) + np.sum(hb * H) + np.sum(db * D) + np.sum(qb * Q)
+
+
+def _random_bars(rng, nao):
+    """Non-symmetric adjoints: a transposed block or a swapped operator side
+    changes E."""
+    return (np.asfortranarray(rng.normal(size=(nao, nao))),
+            np.asfortranarray(rng.normal(size=(nao, nao))),
+            np.asfortranarray(rng.normal(size=(3, nao, nao))),
+            np.asfortranarray(rng.normal(size=(6, nao, nao))))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_hamiltonian_adjoint_fd(periodic):
+    """The oracle's adjoint of get_hamiltonian against central differences
+    of E = <sb,S> + <hb,H> + <db,D> + <qb,Q> through the forward oracle
+    (pinned above by the reference's literal matrices): dE/dR at fixed
+    neighbour list, the strain derivative (positions and lattice vectors
+    strained together) and dE/dse (E is linear in se)."""
+    import dataclasses
+
+    from tblite_amd import structure
+    if periodic:
+        st = _lat_case(True)
+        # a strain moves every image pair at once; with the primitive screen
+        # lifted no primitive pair crosses the intcut threshold inside the
+        # difference stencil, so the differences stay smooth
+        st = dataclasses.replace(st, bas=dataclasses.replace(st.bas, intcut=1e5))
+    else:
+        st = structure.setup(_small_system(5))
+    rng = np.random.default_rng(31 + periodic)
+    nao, nat = st.bas.nao, st.mol.nat
+    se, _ = st.selfenergy(rng.uniform(0.5, 3.5, nat))
+    bars = _random_bars(rng, nao)
+    grad, dse, sigma = O.get_hamiltonian_adjoint(st, se, *bars)
+    x0 = st.mol.xyz.copy()
+    num = np.zeros((3, nat))
+    for iat in range(nat):
+        for k in range(3):
+            def f(x):
+                xyz = x0.copy()
+                xyz[iat, k] = x[0]
+                return (_adjoint_energy(_with_geometry(st, xyz), se, bars),)
+            num[k, iat] = _richardson(f, np.array([x0[iat, k]]), 0, h=1e-3)[0]
+    scale = max(1.0, np.abs(num).max())
+    assert np.abs(grad - num).max() < 1e-8 * scale
+    nsig = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(3):
+            def f(x):
+                eps = np.eye(3)
+                eps[a, b] += x[0]
+                return (_adjoint_energy(_with_geometry(st, x0 @ eps.T, st.trans @ eps.T), se,
+                                        bars),)
+            nsig[a, b] = _richardson(f, np.array([0.0]), 0, h=1e-3)[0]
+    assert np.abs(sigma - 0.5 * (nsig + nsig.T)).max() < 1e-8 * max(1.0, np.abs(nsig).max())
+    e0 = _adjoint_energy(st, se, bars)
+    for sh in range(0, st.bas.nsh, 3):
+        s2 = se.copy()
+        s2[sh] += 1.0
+        assert abs((_adjoint_energy(st, s2, bars) - e0) - dse[sh]) < 1e-9 * max(1.0, abs(e0))
+    # D is not symmetric (the operator sits on the column function): a
+    # transposed dipole adjoint is a different functional, and the pin sees it
+    dbt = np.asfortranarray(np.transpose(bars[2], (0, 2, 1)))
+    g2, _, _ = O.get_hamiltonian_adjoint(st, se, bars[0], bars[1], dbt, bars[3])
+    assert np.abs(g2 - num).max() > 1e-4
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_hamiltonian_adjoint_equals_gradient(periodic):
+    """With the adjoints of the functional get_hamiltonian_gradient
+    differentiates (h0.f90:552-567: H -> P, S -> -W - P (v_mu + v_nu)/2,
+    D(k,mu,nu) -> -P vdp(k, atom(nu)), Q likewise with vqp), the adjoint's
+    dE/dR and sigma are that routine's gradient and sigma, and its dE/dse
+    contracted with dsedcn is dEdcn up to the on-site term the reference
+    keeps as diag(P) (h0.f90:654-663)."""
+    from tblite_amd import structure
+    st = _lat_case(True) if periodic else structure.setup(_small_system(7))
+    rng = np.random.default_rng(41 + periodic)
+    nao, nat = st.bas.nao, st.mol.nat
+    se, dsedcn = st.selfenergy(rng.uniform(0.5, 3.5, nat))
+    P = _sym(rng, nao)
+    W = _sym(rng, nao, 0.1)
+    vao = rng.normal(scale=1e-2, size=nao)
+    vdp = rng.normal(scale=1e-3, size=(3, nat))
+    vqp = rng.normal(scale=1e-3, size=(6, nat))
+    ao2at = st.bas.ao2at
+    sb = np.asfortranarray(-W - 0.5 * P * (vao[:, None] + vao[None, :]))
+    db = np.asfortranarray(-P[None, :, :] * vdp[:, ao2at][:, None, :])
+    qb = np.asfortranarray(-P[None, :, :] * vqp[:, ao2at][:, None, :])
+    grad, dse, sigma = O.get_hamiltonian_adjoint(st, se, sb, P, db, qb)
+    cr, gr, sr = O.get_hamiltonian_gradient(
+        st, se, dsedcn, vao.reshape(nao, 1), vdp.reshape(3, nat, 1), vqp.reshape(6, nat, 1),
+        P.reshape(nao, nao, 1), W.reshape(nao, nao, 1))
+    assert np.abs(grad - gr).max() < 1e-12 * max(1.0, np.abs(gr).max())
+    assert np.abs(sigma - sr).max() < 1e-12 * max(1.0, np.abs(sr).max())
+    # dEdcn = sum over shells dsedcn * dE/dse, with the on-site block reduced
+    # to diag(P) in the reference
+    sh_at = np.repeat(np.arange(nat), st.spec.nsh_id[st.mol.id])
+    cn_adj = np.bincount(sh_at, weights=dsedcn * dse, minlength=nat)
+    assert np.abs(cn_adj - cr).max() < 1e-6 * max(1.0, np.abs(cr).max())
