@@ -9,12 +9,15 @@ molecules (atom counts log-uniform in [8, 256], H/C/N/O/F/S/Cl), FP64:
             equal to the plain gradient (one launch)
 
 Inputs are resident in HBM when the timed region starts.  ``value`` is
-molecules per second of that batched-autograd step.  Beside it the line
-reports the batched plain-gradient step, the plain per-molecule GPU path (the
-drop-in host-pointer entry points, PCIe copies included, on a sample), the
-roofline of the dominant kernel (HIP events on the launch stream) and the CPU
-baseline (the oracle, a C restatement of the reference loop nest with OpenMP,
-timed in a child process on a bounded sample).
+molecules per second of that batched-autograd step (eager launches through
+torch.autograd).  Beside it the line reports the same step's two launches
+replayed as a captured HIP graph, the batched plain-gradient step, the plain
+per-molecule GPU path (one molecule per launch, inputs resident in HBM, on a
+sample of the molecules) and the drop-in host-pointer entry points (PCIe
+copies included), the roofline of the dominant kernel (HIP events on the
+launch stream) and the CPU baseline (the oracle, a C restatement of the
+reference loop nest with OpenMP on the physical cores, timed in a child
+process on a bounded sample).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -51,12 +54,49 @@ def profiled_traffic(kernel: str, nmol: int):
     return None
 
 
-def _cpu_threads():
+def _cpu_info():
+    """Host CPUs this process may run on: logical CPUs of the affinity mask,
+    the physical cores among them (/proc/cpuinfo core ids) and the cgroup CPU
+    quota, if one is set.  The CPU baseline runs one OpenMP thread per
+    physical core, no more than the quota allows."""
     try:
-        n = len(os.sched_getaffinity(0))
+        cpus = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        cpus = list(range(os.cpu_count() or 1))
+    core_of = {}
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+            elif cur:
+                if "processor" in cur:
+                    core_of[int(cur["processor"])] = (cur.get("physical id", "0"),
+                                                      cur.get("core id", cur["processor"]))
+                cur = {}
+        if "processor" in cur:
+            core_of[int(cur["processor"])] = (cur.get("physical id", "0"),
+                                              cur.get("core id", cur["processor"]))
+    except OSError:
+        pass
+    physical = len({core_of.get(c, ("0", str(c))) for c in cpus})
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    threads = physical
+    if quota is not None:
+        threads = max(1, min(threads, int(quota)))
+    return {"logical": len(cpus), "physical": physical, "cgroup_quota_cpus": quota,
+            "threads": threads}
+
+
+def _cpu_threads():
+    return _cpu_info()["threads"]
 
 
 # --------------------------------------------------------------------------
@@ -97,7 +137,8 @@ def cpu_baseline_child(budget_s: float):
 
 
 def run_cpu_baseline(budget_s=15.0):
-    threads = _cpu_threads()
+    info = _cpu_info()
+    threads = info["threads"]
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_SCHEDULE="dynamic",
                OMP_PROC_BIND="spread", OMP_STACKSIZE="4G")
     out = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-baseline-child",
@@ -107,10 +148,12 @@ def run_cpu_baseline(budget_s=15.0):
         return None
     r = json.loads(out.stdout.strip().splitlines()[-1])
     return {"value": r["molecules"] / r["seconds"], "unit": "molecules/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "logical_cpus": info["logical"],
+            "physical_cores": info["physical"], "cgroup_quota_cpus": info["cgroup_quota_cpus"],
             "sample": f"first {r['molecules']} of the 2048 molecules ({r['atoms']} atoms), "
                       f"oracle get_selfenergy+adjacency list+get_hamiltonian+"
-                      f"get_hamiltonian_gradient, OpenMP {threads} threads, "
+                      f"get_hamiltonian_gradient, OpenMP {threads} threads (one per physical "
+                      f"core of {info['physical']}, {info['logical']} logical CPUs), "
                       f"{r['seconds']:.1f} s"}
 
 
@@ -238,31 +281,99 @@ def main():
     log(f"plain batched: {el_p / args.steps * 1e3:.2f} ms/step (forward {fwd_p:.2f} ms, "
         f"gradient {grad_p:.2f} ms)")
 
-    # agreement of the two batched gradients (autograd vs plain kernel)
+    # the same step as one captured HIP graph (forward + adjoint launches,
+    # outputs preallocated), replayed; captured on a side stream
+    aouts = (torch.empty(3 * b.nat, dtype=torch.float64, device=dev),
+             torch.empty(b.nsh, dtype=torch.float64, device=dev))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        b.forward(xyz, se, out=outs)
+        b.adjoint(xyz, se, Sb, Db, Qb, Hb, out=aouts)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        b.forward(xyz, se, out=outs)
+        b.adjoint(xyz, se, Sb, Db, Qb, Hb, out=aouts)
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    el_g = time.perf_counter() - t
+    log(f"graph-replayed step: {el_g / args.steps * 1e3:.2f} ms/step")
+
+    # agreement of the two batched gradients (autograd vs plain kernel), and
+    # of the graph's adjoint with both
     gx, _ = step_autograd()
     _, gp, _ = step_plain()
+    graph.replay()
     torch.cuda.synchronize()
     agree = float((gx - gp).abs().max())
+    agree_graph = float((aouts[0] - gx).abs().max())
 
-    # plain per-molecule GPU path (drop-in host-pointer entry points) on a sample
+    # plain per-molecule GPU path: one molecule per launch, inputs and
+    # outputs resident in HBM (views into the batch's packed tensors), on
+    # every 8th molecule of the workload (all sizes); each molecule is its own
+    # one-molecule batch, created before the timed loop
+    sample = list(range(0, b.nmol, 8))
+    singles = []
+    for m in sample:
+        a0, a1 = int(b.atom_off[m]), int(b.atom_off[m + 1])
+        s0, s1 = int(b.sh_off[m]), int(b.sh_off[m + 1])
+        o0, o1 = int(b.ao_off[m]), int(b.ao_off[m + 1])
+        d0, d1 = int(b.dense_off[m]), int(b.dense_off[m + 1])
+        bm = Batch([mols[m]])
+        ins = (xyz[3 * a0: 3 * a1], se[s0:s1], dsedcn[s0:s1], P[d0:d1], W[d0:d1],
+               vao[o0:o1], vdp[3 * a0: 3 * a1], vqp[6 * a0: 6 * a1])
+        fo = (outs[0][d0:d1], outs[1][3 * d0: 3 * d1], outs[2][6 * d0: 6 * d1], outs[3][d0:d1])
+        go = (gouts[0][a0:a1], gouts[1][3 * a0: 3 * a1], gouts[2][9 * m: 9 * m + 9])
+        singles.append((bm, ins, fo, go))
+
+    def step_single():
+        for bm, (x_, s_, ds_, p_, w_, va_, vd_, vq_), fo, go in singles:
+            bm.forward(x_, s_, out=fo)
+            bm.gradient(x_, s_, ds_, p_, w_, va_, vd_, vq_, out=go)
+
+    step_single()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        step_single()
+    torch.cuda.synchronize()
+    plain_rate = reps * len(singles) / (time.perf_counter() - t)
+    log(f"plain per-molecule GPU (device-resident): {plain_rate:.0f} molecules/s")
+    del singles
+
+    # drop-in host-pointer entry points (PCIe copies, host work-unit plans
+    # cached across calls of the same molecule) on the first 64 molecules
     nsample = min(64, len(mols))
     rng = np.random.default_rng(3)
-    t = 0.0
+    dropin = []
     for z, x in mols[:nsample]:
         st = structure.setup(structure.Structure(z, x))
         nao_m, nat_m = st.bas.nao, st.mol.nat
         sem, dsem = st.selfenergy(rng.uniform(0.5, 3.5, nat_m))
         a = rng.normal(size=(nao_m, nao_m))
         Pm = np.asfortranarray((0.5 * (a + a.T))[:, :, None])
-        t1 = time.perf_counter()
-        h0mod.get_hamiltonian(st, sem)
-        h0mod.get_hamiltonian_gradient(st, sem, dsem, np.zeros((nao_m, 1), order="F"),
-                                       np.zeros((3, nat_m, 1), order="F"),
-                                       np.zeros((6, nat_m, 1), order="F"), Pm, 0.05 * Pm,
-                                       np.zeros(nat_m), np.zeros((3, nat_m), order="F"),
-                                       np.zeros((3, 3), order="F"))
-        t += time.perf_counter() - t1
-    plain_rate = nsample / t
+        dropin.append((st, sem, dsem, Pm))
+
+    def run_dropin():
+        for st, sem, dsem, Pm in dropin:
+            nao_m, nat_m = st.bas.nao, st.mol.nat
+            h0mod.get_hamiltonian(st, sem)
+            h0mod.get_hamiltonian_gradient(st, sem, dsem, np.zeros((nao_m, 1), order="F"),
+                                           np.zeros((3, nat_m, 1), order="F"),
+                                           np.zeros((6, nat_m, 1), order="F"), Pm, 0.05 * Pm,
+                                           np.zeros(nat_m), np.zeros((3, nat_m), order="F"),
+                                           np.zeros((3, 3), order="F"))
+    t = time.perf_counter()
+    run_dropin()
+    dropin_rate = nsample / (time.perf_counter() - t)
 
     # SCF-loop consumers of the resident S / Q (one HBM pass each, nspin = 1):
     # kernel time from the batch's events, algorithmic bytes per dense element
@@ -296,15 +407,21 @@ def main():
 
     ms_step = el_a / args.steps * 1e3
     value = b.nmol / (el_a / args.steps)
-    # dominant kernel of the autograd step
+    # dominant kernel of the autograd step.  The adjoint's derivative work is
+    # over the off-site primitive pairs only (its on-site part reads a
+    # per-species table), so its algorithmic FLOP count uses K.c off-site.
     fwd_bytes = roofline.BYTES_FWD_PER_ELEM * b.ndense
-    adj_flop = roofline.FLOP_BWD_PER_KC * kc
+    adj_flop = roofline.FLOP_BWD_PER_KC * kc_off
     if adj_a >= fwd_a:
         ach = adj_flop / (adj_a * 1e-3) / 1e12
-        roof = roofline.roofline_entry("mfma", ach, roofline.PEAK_FP64_TFLOPS, "TFLOP/s",
+        roof = roofline.roofline_entry("fp64", ach, roofline.PEAK_FP64_TFLOPS, "TFLOP/s",
                                        profiled_traffic("k_gradient<adjoint>", b.nmol))
+        roof["bound_note"] = ("FP64 vector ALU (the kernel issues no MFMA); peak = MI355X FP64 "
+                              "78.6 TFLOP/s")
         roof["kernel"] = "k_gradient<adjoint> (FP64 vector)"
         roof["kernel_ms"] = adj_a
+        roof["algorithmic_flop"] = adj_flop
+        roof["achieved_incl_onsite"] = roofline.FLOP_BWD_PER_KC * kc / (adj_a * 1e-3) / 1e12
     else:
         ach = fwd_bytes / (fwd_a * 1e-3) / 1e9
         roof = roofline.roofline_entry("hbm", ach, roofline.PEAK_HBM_GBS, "GB/s",
@@ -330,15 +447,24 @@ def main():
                    "molecules": b.nmol, "atoms": b.nat, "nao": b.nao,
                    "dense_elements": b.ndense, "kept_prim_x_cart_pairs": kc,
                    "parallelism": "single GPU"},
+        "graph_ms_per_step": el_g / args.steps * 1e3,
+        "graph_molecules_per_s": b.nmol / (el_g / args.steps),
         "batched_plain_molecules_per_s": b.nmol / (el_p / args.steps),
         "plain_gpu_molecules_per_s": plain_rate,
-        "plain_gpu_sample": f"first {nsample} molecules, host-pointer get_hamiltonian + "
-                            f"get_hamiltonian_gradient (PCIe copies included)",
+        "plain_gpu_sample": f"every 8th molecule ({len(sample)}), one molecule per launch: "
+                            f"forward + plain gradient kernels on inputs resident in HBM",
+        "batched_autograd_over_plain_gpu": value / plain_rate,
+        "dropin_pcie_molecules_per_s": dropin_rate,
+        "dropin_pcie_sample": f"first {nsample} molecules, host-pointer get_hamiltonian + "
+                              f"get_hamiltonian_gradient (PCIe copies included, work-unit "
+                              f"plans built once per molecule)",
         "kernel_ms": {"forward": fwd_a, "adjoint": adj_a, "gradient": grad_p},
         "forward_hbm_GBs": fwd_bytes / (fwd_a * 1e-3) / 1e9,
         "adjoint_fp64_TFLOPs": adj_flop / (adj_a * 1e-3) / 1e12,
         "gradient_fp64_TFLOPs": adj_flop / (grad_p * 1e-3) / 1e12,
+        "kept_prim_x_cart_pairs_offsite": kc_off,
         "autograd_vs_plain_max_abs_diff": agree,
+        "graph_vs_autograd_max_abs_diff": agree_graph,
         "tiles": b.tiles(),
         "scf_consumers": scf,
         "roofline": roof,

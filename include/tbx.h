@@ -215,6 +215,27 @@ int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
                       const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
                       const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream);
 
+/* --- coordination number (device) ----------------------------------------
+ * The GFN2 coordination number, mctc-lib's double-exponential counting
+ * function as tblite creates it (xtb/gfn2.f90:640) and evaluates it
+ * (xtb/singlepoint.f90:224):
+ *   cn(i) = sum_{j,T; 0 < r <= cutoff} 1/(1+exp(-ka (r0/r-1))) *
+ *           1/(1+exp(-kb ((r0+rshift)/r-1))),  r0 = rcov(i) + rcov(j),
+ * over the atoms of i's molecule and the molecule's translations.  GFN2:
+ * ka = 10, kb = 20, rshift = 2 bohr, cutoff 25 bohr, rcov = 4/3 of the D3
+ * covalent radii (bohr).  tbx_batch_set_ncoord stores the radii ([nid],
+ * per species) and constants; it waits for the device. */
+int tbx_batch_set_ncoord(tbx_batch *b, const double *rcov, double ka, double kb, double rshift,
+                         double cutoff);
+/* d_cn [nat] OVERWRITTEN. */
+int tbx_batch_ncoord(tbx_batch *b, const double *d_xyz, double *d_cn, void *stream);
+/* Chain rule of the coordination number, what xtb/singlepoint.f90:375-378
+ * does with dcndr and dcndL: d_grad [3*nat] += dcn/dxyz^T d_dEdcn and
+ * d_sigma [9*nmol] += dcn/dstrain^T d_dEdcn per molecule (ACCUMULATED;
+ * FP64 atomics for sigma). */
+int tbx_batch_ncoord_gradient(tbx_batch *b, const double *d_xyz, const double *d_dEdcn,
+                              double *d_grad, double *d_sigma, void *stream);
+
 /* --- SCF-loop consumers of the resident integrals (batch layout) ---------
  * Dense spin arrays (d_P, d_h1) hold nspin (1 or 2) blocks of ndense
  * doubles; d_vao is [nao*nspin], d_vmp [vdim*nat*nspin].  The multipole

@@ -253,3 +253,24 @@ def get_hamiltonian_adjoint(setup, selfenergy, sbar, hbar, dbar, qbar, gradient=
     if st != 0:
         raise ValueError("oracle: unsupported basis")
     return gradient, dEdse, sigma
+
+
+def ncoord_dexp(ids, xyz, trans, rcov, ka=10.0, kb=20.0, rshift=2.0, cutoff=25.0,
+                derivatives=True):
+    """Double-exponential coordination number (mctc-lib ncoord "dexp", GFN2):
+    cn (nat), and with ``derivatives`` dcndr (3, nat, nat) = d cn(i)/d xyz(:, j)
+    as [c, j, i] in Fortran order, and dcndL (3, 3, nat)."""
+    ids = _i32(ids)
+    x = _f64(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+    tr = _f64(np.asarray(trans, dtype=np.float64).reshape(-1, 3))
+    rc = _f64(rcov)
+    nat = len(ids)
+    cn = np.zeros(nat)
+    dcndr = np.zeros((3, nat, nat), order="F") if derivatives else None
+    dcndL = np.zeros((3, 3, nat), order="F") if derivatives else None
+    lib().orc_ncoord_dexp(ct.c_int(nat), _p(ids), _p(x), ct.c_int(len(tr)), _p(tr), _p(rc),
+                          ct.c_double(ka), ct.c_double(kb), ct.c_double(rshift),
+                          ct.c_double(cutoff), _p(cn),
+                          _p(dcndr) if derivatives else None,
+                          _p(dcndL) if derivatives else None)
+    return (cn, dcndr, dcndL) if derivatives else cn

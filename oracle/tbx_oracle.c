@@ -30,6 +30,9 @@
  *   orc_get_multipole_integrals_lat  integral/native/integrals.f90:1394-1465
  *                             (its S and S, D are get_overlap_lat :411-470 and
  *                             get_dipole_integrals_lat :994-1059)
+ *   orc_ncoord_dexp           mctc-lib ncoord "dexp" as called at xtb/gfn2.f90:640,
+ *                             xtb/singlepoint.f90:224 (third-party, absent:
+ *                             restated from its published formula; unpinned)
  *
  * Conventions kept from the reference: all arrays are Fortran column-major,
  * indices are 0-based on this side; Cartesian components in CCA order;
@@ -1028,4 +1031,69 @@ int orc_get_hamiltonian_adjoint(int nat, int nid, int msh, int nsh, int nao, con
     }
 #undef E_
     return 0;
+}
+
+/* Double-exponential coordination number of GFN2 (mctc-lib v0.5 ncoord
+ * "dexp", created at xtb/gfn2.f90:640 and evaluated at
+ * xtb/singlepoint.f90:224):
+ *   cn(i) = sum_{j, T; 0 < r_ijT <= cutoff} f(r_ijT),
+ *   f(r) = 1/(1+exp(-ka (r0/r - 1))) * 1/(1+exp(-kb ((r0+rshift)/r - 1))),
+ *   r0 = rcov(id(i)) + rcov(id(j)),
+ * with its derivatives dcndr(3, nat, nat) (d cn(i) / d xyz(:, j), stored
+ * dcndr[c + 3*(j + nat*i)]) and dcndL(3, 3, nat) (strain derivative), the
+ * arrays singlepoint.f90:375-378 contracts with dEdcn.  dcndr / dcndL may
+ * be NULL.  mctc-lib is not part of the reference tree: its formula and the
+ * GFN2 constants (ka = 10, kb = 20, rshift = 2 bohr, cutoff 25 bohr) are
+ * restated from its published source; parity with it is unpinned. */
+static void dexp_count(double ka, double kb, double rshift, double r, double r0, double *f,
+                       double *df)
+{
+    double e1 = exp(-ka * (r0 / r - 1.0)), e2 = exp(-kb * ((r0 + rshift) / r - 1.0));
+    double c1 = 1.0 / (1.0 + e1), c2 = 1.0 / (1.0 + e2);
+    /* d/dr 1/(1+exp(-k (a/r - 1))) = -k a e / (r^2 (1+e)^2) */
+    double d1 = -ka * r0 * e1 / (r * r * (1.0 + e1) * (1.0 + e1));
+    double d2 = -kb * (r0 + rshift) * e2 / (r * r * (1.0 + e2) * (1.0 + e2));
+    *f = c1 * c2;
+    *df = d1 * c2 + c1 * d2;
+}
+
+void orc_ncoord_dexp(int nat, const int *id, const double *xyz, int ntr, const double *trans,
+                     const double *rcov, double ka, double kb, double rshift, double cutoff,
+                     double *cn, double *dcndr, double *dcndL)
+{
+    const double eps = 2.220446049250313e-16, cut2 = cutoff * cutoff;
+    memset(cn, 0, sizeof(double) * nat);
+    if (dcndr) memset(dcndr, 0, sizeof(double) * 3 * (size_t)nat * nat);
+    if (dcndL) memset(dcndL, 0, sizeof(double) * 9 * (size_t)nat);
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int t = 0; t < ntr; ++t) {
+                double v[3], r2 = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = xyz[3 * i + k] - xyz[3 * j + k] - trans[3 * t + k];
+                    r2 += v[k] * v[k];
+                }
+                if (r2 > cut2 || r2 < eps) continue;
+                double r = sqrt(r2), f, df;
+                dexp_count(ka, kb, rshift, r, rcov[id[i]] + rcov[id[j]], &f, &df);
+                cn[i] += f;
+                if (i != j) cn[j] += f;
+                double g[3];
+                for (int k = 0; k < 3; ++k) g[k] = df * v[k] / r;
+                if (dcndr) {
+                    for (int k = 0; k < 3; ++k) {
+                        dcndr[k + 3 * (i + (size_t)nat * i)] += g[k];
+                        dcndr[k + 3 * (j + (size_t)nat * j)] -= g[k];
+                        dcndr[k + 3 * (j + (size_t)nat * i)] -= g[k];
+                        dcndr[k + 3 * (i + (size_t)nat * j)] += g[k];
+                    }
+                }
+                if (dcndL) {
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            dcndL[a + 3 * b + 9 * i] += g[a] * v[b];
+                            if (i != j) dcndL[a + 3 * b + 9 * j] += g[a] * v[b];
+                        }
+                }
+            }
 }

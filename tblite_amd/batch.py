@@ -112,6 +112,9 @@ class Batch:
                                           p(self.ao_off), p(self.dense_off)))
         # per-molecule views of the setup, for host-side references
         self._shell_tables()
+        # the GFN2 coordination number (xtb/gfn2.f90:640): D3 covalent radii
+        self.rcov = np.array([params.covalent_rad(int(z)) for z in self.nums])
+        self.set_ncoord(self.rcov)
         if order_pairs:
             self.order_pairs(self.xyz0, skin)
 
@@ -200,6 +203,31 @@ class Batch:
                                           _ptr(Qb), _ptr(Hb), _ptr(dxyz), _ptr(dse),
                                           self._stream()))
         return dxyz, dse
+
+    # -- coordination number (mctc-lib ncoord "dexp", xtb/singlepoint.f90:224) --
+    def set_ncoord(self, rcov, ka=params.CN_KA, kb=params.CN_KB, rshift=params.CN_RSHIFT,
+                   cutoff=params.CN_CUTOFF):
+        """Covalent radii per species (bohr) and constants of the counting
+        function; the defaults are GFN2's."""
+        r = np.ascontiguousarray(rcov, dtype=np.float64)
+        assert r.size == len(self.nums)
+        L.check(L.lib().tbx_batch_set_ncoord(self.handle, r.ctypes.data_as(ct.c_void_p),
+                                             float(ka), float(kb), float(rshift), float(cutoff)))
+        self._rcov_keep = r
+
+    def ncoord(self, xyz, out=None):
+        """Coordination number of every atom [nat] (device)."""
+        if out is None:
+            out = torch.empty(self.nat, dtype=torch.float64, device=self.torch_device)
+        L.check(L.lib().tbx_batch_ncoord(self.handle, _ptr(xyz), _ptr(out), self._stream()))
+        return out
+
+    def ncoord_gradient(self, xyz, dEdcn, grad, sigma):
+        """grad [3*nat] += (dcn/dxyz)^T dEdcn, sigma [9*nmol] += the strain
+        part (xtb/singlepoint.f90:375-378), in place."""
+        L.check(L.lib().tbx_batch_ncoord_gradient(self.handle, _ptr(xyz), _ptr(dEdcn), _ptr(grad),
+                                                  _ptr(sigma), self._stream()))
+        return grad, sigma
 
     # -- SCF-loop consumers of the resident integrals ------------------------
     # (scf/potential.f90:192-255, wavefunction/mulliken.f90:37-98).  Spin
@@ -309,3 +337,39 @@ class H0Integrals(torch.autograd.Function):
 def h0_integrals(batch: Batch, xyz, se):
     """Differentiable batched S, D, Q, H0 (see ``H0Integrals``)."""
     return H0Integrals.apply(xyz, se, batch)
+
+
+class CoordinationNumber(torch.autograd.Function):
+    """xyz -> cn over a ``Batch`` (GFN2 double-exponential counting function);
+    the backward is the device chain-rule kernel (no dcn/dR array)."""
+
+    @staticmethod
+    def forward(ctx, xyz, batch):
+        ctx.batch = batch
+        ctx.save_for_backward(xyz)
+        return batch.ncoord(xyz.detach())
+
+    @staticmethod
+    def backward(ctx, dcn):
+        (xyz,) = ctx.saved_tensors
+        b = ctx.batch
+        dev = xyz.device
+        grad = torch.zeros(3 * b.nat, dtype=torch.float64, device=dev)
+        sigma = torch.zeros(9 * b.nmol, dtype=torch.float64, device=dev)
+        b.ncoord_gradient(xyz.detach(), dcn.to(torch.float64).contiguous(), grad, sigma)
+        return grad, None
+
+
+def coordination_number(batch: Batch, xyz):
+    """Differentiable coordination number (see ``CoordinationNumber``)."""
+    return CoordinationNumber.apply(xyz, batch)
+
+
+def gfn2_integrals(batch: Batch, xyz):
+    """The differentiable GFN2 one-electron path as tblite's singlepoint runs
+    it (xtb/singlepoint.f90:219-241): cn(xyz) -> get_selfenergy -> S, D, Q,
+    H0.  The gradient of any function of the outputs with respect to xyz
+    includes the coordination-number chain (singlepoint.f90:375-378)."""
+    cn = coordination_number(batch, xyz)
+    se, _ = batch.selfenergy(cn)
+    return h0_integrals(batch, xyz, se.contiguous())

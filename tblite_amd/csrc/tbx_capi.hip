@@ -45,6 +45,13 @@ extern "C" hipError_t tbx_launch_scf(int what, const TbxMols *mol, int nspin, lo
                                      hipStream_t stream);
 extern "C" hipError_t tbx_launch_copy(const double *src, double *dst, long long n,
                                       hipStream_t stream);
+extern "C" hipError_t tbx_launch_ncoord(const TbxMols *mol, const double *rcov, double ka,
+                                        double kb, double rshift, double cutoff,
+                                        const double *xyz, double *cn, hipStream_t stream);
+extern "C" hipError_t tbx_launch_ncoord_grad(const TbxMols *mol, const double *rcov, double ka,
+                                             double kb, double rshift, double cutoff,
+                                             const double *xyz, const double *dEdcn,
+                                             double *grad, double *sigma, hipStream_t stream);
 
 namespace {
 
@@ -287,6 +294,8 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
 
 struct HostMols {
     int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
+    // translations of each molecule (first, count) into the translation table
+    std::vector<int2> mol_tr;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
         cart_atom, ao_atom, sh_off, unit_seg, funit_seg, uplan, fplan, sh_aog, tiles;
@@ -687,6 +696,11 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
 {
     Arena &a = d.mol_host;
     a.host.clear();
+    // every molecule has the whole translation table unless the batch says otherwise
+    std::vector<int2> mol_tr = hm.mol_tr;
+    if (mol_tr.size() != (size_t)hm.nmol)
+        mol_tr.assign(hm.nmol, make_int2(0, (int)(trans.size() / 3)));
+    const size_t o_mol_tr = a.add(mol_tr);
     const size_t o_trans = a.add(trans), o_mol_atom = a.add(hm.mol_atom),
                  o_mol_nao = a.add(hm.mol_nao), o_mol_cart = a.add(hm.mol_cart),
                  o_mol_dense = a.add(hm.mol_dense), o_at_sp = a.add(hm.at_sp),
@@ -704,6 +718,7 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vecto
     mt.nat = hm.nat;
     mt.ntrans = (int)(trans.size() / 3);
     mt.trans = reinterpret_cast<const double *>(at(o_trans));
+    mt.mol_tr = reinterpret_cast<const int2 *>(at(o_mol_tr));
     mt.mol_atom = reinterpret_cast<const int *>(at(o_mol_atom));
     mt.mol_nao = reinterpret_cast<const int *>(at(o_mol_nao));
     mt.mol_cart = reinterpret_cast<const int *>(at(o_mol_cart));
@@ -744,6 +759,13 @@ struct tbx_context {
     HostSpecies hs;
     std::vector<char> hs_raw;
     bool hs_valid = false;
+    // molecule tables and work-unit plans of the last call (they depend on the
+    // species, the atom order and the translations, not on the coordinates):
+    // a call with the same species tables, ids and translations reuses them
+    // and their device copy
+    HostMols hm;
+    std::vector<char> hm_raw;
+    bool hm_valid = false;
 };
 
 // The raw bytes of a basis + H0 parameter set + cutoff (empty if a table is
@@ -785,6 +807,10 @@ struct tbx_batch {
     DevBuf flag, dse_tmp;
     hipEvent_t ev[16] = {};  // pairs (start, end): forward, -, gradient, adjoint, 4 SCF kernels
     bool timed = false;
+    // coordination number: covalent radii per species (device) and constants
+    DevBuf rcov;
+    bool cn_set = false;
+    double cn_ka = 0.0, cn_kb = 0.0, cn_rshift = 0.0, cn_cutoff = 0.0;
 };
 
 extern "C" {
@@ -848,10 +874,28 @@ int tbx_get_selfenergy(int nat, const int *id, const int *ish_at, const int *nsh
 
 } // extern "C"
 
-// Shared set-up of the single-molecule host path.
+// The raw bytes of a structure's species ids and translations.
+static std::vector<char> mols_bytes(const tbx_geom *geom)
+{
+    std::vector<char> r;
+    auto put = [&r](const void *p, size_t n) {
+        const char *c = static_cast<const char *>(p);
+        r.insert(r.end(), c, c + n);
+    };
+    put(&geom->nat, sizeof(int));
+    put(&geom->ntrans, sizeof(int));
+    put(geom->id, sizeof(int) * (size_t)geom->nat);
+    put(geom->trans, sizeof(double) * 3 * (size_t)geom->ntrans);
+    return r;
+}
+
+// Shared set-up of the single-molecule host path: the context's species
+// tables (ctx->hs) and molecule tables with work-unit plans (ctx->hm), built
+// and uploaded only when the parametrisation, the species of the atoms or the
+// translations change.
 static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list *list,
                           const tbx_basis *bas, const tbx_h0par *h0, const int *ish_at,
-                          const int *iao_sh, int nao, HostSpecies &hs, HostMols &hm)
+                          const int *iao_sh, int nao)
 {
     if (!ctx) return fail(TBX_ERR_ARG, "null context");
     if (!geom || geom->nat <= 0 || !geom->id || !geom->xyz)
@@ -865,14 +909,21 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
     std::vector<char> raw = species_bytes(bas, h0, geom->cutoff);
     const bool reuse = ctx->hs_valid && !raw.empty() && raw == ctx->hs_raw;
     int st = TBX_OK;
-    if (reuse) {
-        hs = ctx->hs;
-    } else {
+    HostSpecies hs_new;
+    if (!reuse) {
         ctx->hs_valid = false;
-        if ((st = build_species(bas, h0, hs))) return st;
+        ctx->hm_valid = false;
+        if ((st = build_species(bas, h0, hs_new))) return st;
     }
-    st = build_mols(hs, 1, &geom->nat, geom->id, hm);
-    if (st) return st;
+    const HostSpecies &hs = reuse ? ctx->hs : hs_new;
+    std::vector<char> mraw = mols_bytes(geom);
+    const bool reuse_mols = reuse && ctx->hm_valid && mraw == ctx->hm_raw;
+    HostMols hm_new;
+    if (!reuse_mols) {
+        ctx->hm_valid = false;
+        if ((st = build_mols(hs, 1, &geom->nat, geom->id, hm_new))) return st;
+    }
+    const HostMols &hm = reuse_mols ? ctx->hm : hm_new;
     if (hm.nao != nao) return fail(TBX_ERR_ARG, "nao=%d but the basis gives %d", nao, hm.nao);
     if (ish_at && iao_sh) {
         for (int a = 0; a < geom->nat; ++a) {
@@ -889,13 +940,21 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
     hipStream_t s = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     if (!reuse) {
-        HIP_TRY(upload_species(ctx->tab, hs, geom->cutoff, s));
-        ctx->hs = hs;
+        // no launch of this context may still read the tables replaced here
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(upload_species(ctx->tab, hs_new, geom->cutoff, s));
+        ctx->hs = std::move(hs_new);
         ctx->hs_raw = std::move(raw);
         ctx->hs_valid = true;
     }
-    if ((st = build_units(hs, hm, geom->ntrans, kDefaultTiles))) return st;
-    HIP_TRY(upload_mols(ctx->tab, hm, trans, s));
+    if (!reuse_mols) {
+        if ((st = build_units(ctx->hs, hm_new, geom->ntrans, kDefaultTiles))) return st;
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(upload_mols(ctx->tab, hm_new, trans, s));
+        ctx->hm = std::move(hm_new);
+        ctx->hm_raw = std::move(mraw);
+        ctx->hm_valid = true;
+    }
     HIP_TRY(ctx->xyz.reserve(sizeof(double) * 3 * geom->nat));
     HIP_TRY(hipMemcpyAsync(ctx->xyz.p, geom->xyz, sizeof(double) * 3 * geom->nat,
                            hipMemcpyHostToDevice, s));
@@ -926,10 +985,9 @@ extern "C" int tbx_get_hamiltonian(tbx_context *ctx, const tbx_geom *geom, const
                                    double *overlap, double *dpint, double *qpint,
                                    double *hamiltonian)
 {
-    HostSpecies hs;
-    HostMols hm;
-    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, hs, hm);
+    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao);
     if (st) return st;
+    const HostMols &hm = ctx->hm;
     if (!selfenergy || !overlap || !dpint || !qpint || !hamiltonian)
         return fail(TBX_ERR_ARG, "null output array");
     hipStream_t s = ctx->stream;
@@ -966,10 +1024,9 @@ extern "C" int tbx_get_multipole_integrals(tbx_context *ctx, const tbx_geom *geo
     std::vector<double> hscale((size_t)msh * msh * nid * nid, 0.0), shpoly((size_t)msh * nid, 0.0),
         rad(nid, 1.0);
     const tbx_h0par h0 = {hscale.data(), shpoly.data(), rad.data()};
-    HostSpecies hs;
-    HostMols hm;
-    int st = prepare_single(ctx, geom, nullptr, bas, &h0, ish_at, iao_sh, nao, hs, hm);
+    int st = prepare_single(ctx, geom, nullptr, bas, &h0, ish_at, iao_sh, nao);
     if (st) return st;
+    const HostMols &hm = ctx->hm;
     hipStream_t s = ctx->stream;
     const size_t n2 = (size_t)nao * nao;
     const std::vector<double> se(hm.nsh, 0.0);
@@ -1016,10 +1073,9 @@ extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *ge
                                             const double *pmat, const double *xmat,
                                             double *dEdcn, double *gradient, double *sigma)
 {
-    HostSpecies hs;
-    HostMols hm;
-    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, hs, hm);
+    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao);
     if (st) return st;
+    const HostMols &hm = ctx->hm;
     if (nspin < 1 || nspin > 2) return fail(TBX_ERR_ARG, "nspin=%d", nspin);
     if (!selfenergy || !dsedcn || !vao || !vdp || !vqp || !pmat || !xmat || !dEdcn || !gradient ||
         !sigma)
@@ -1081,10 +1137,9 @@ extern "C" int tbx_get_hamiltonian_adjoint(tbx_context *ctx, const tbx_geom *geo
 {
     if (!selfenergy || !sbar || !hbar || !dbar || !qbar || !gradient || !dEdse)
         return fail(TBX_ERR_ARG, "null argument");
-    HostSpecies hs;
-    HostMols hm;
-    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao, hs, hm);
+    int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao);
     if (st) return st;
+    const HostMols &hm = ctx->hm;
     hipStream_t s = ctx->stream;
     const int nat = geom->nat;
     const size_t n2 = (size_t)nao * nao;
@@ -1391,6 +1446,7 @@ extern "C" void tbx_batch_delete(tbx_batch *b)
     b->tab.release();
     b->flag.release();
     b->dse_tmp.release();
+    b->rcov.release();
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete b;
@@ -1492,6 +1548,52 @@ extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double
                                 d_Dbar, d_Qbar, d_Hbar, d_dxyz, d_dse, b->dse_tmp.as<double>(),
                                 nullptr, b->flag.as<int>(), b->tiles.grad_chunk, s));
     HIP_TRY(mark(b, 7, s));
+    return TBX_OK;
+}
+
+// Coordination number of the batch (mctc-lib ncoord "dexp" as GFN2 uses it,
+// xtb/gfn2.f90:640, xtb/singlepoint.f90:224) and its chain rule.
+extern "C" int tbx_batch_set_ncoord(tbx_batch *b, const double *rcov, double ka, double kb,
+                                    double rshift, double cutoff)
+{
+    if (!b || !rcov) return fail(TBX_ERR_ARG, "null argument");
+    for (double v : {ka, kb, cutoff})
+        if (!(v > 0.0) || !std::isfinite(v))
+            return fail(TBX_ERR_ARG, "ka, kb and cutoff must be positive and finite");
+    if (!(rshift >= 0.0) || !std::isfinite(rshift)) return fail(TBX_ERR_ARG, "rshift %g", rshift);
+    for (int k = 0; k < b->hs.nsp; ++k)
+        if (!(rcov[k] > 0.0) || !std::isfinite(rcov[k]))
+            return fail(TBX_ERR_ARG, "covalent radius of species %d is %g", k, rcov[k]);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());  // no launch may still read the radii replaced here
+    HIP_TRY(b->rcov.reserve(sizeof(double) * b->hs.nsp));
+    HIP_TRY(hipMemcpy(b->rcov.p, rcov, sizeof(double) * b->hs.nsp, hipMemcpyHostToDevice));
+    b->cn_ka = ka;
+    b->cn_kb = kb;
+    b->cn_rshift = rshift;
+    b->cn_cutoff = cutoff;
+    b->cn_set = true;
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_ncoord(tbx_batch *b, const double *d_xyz, double *d_cn, void *stream)
+{
+    if (!b || !d_xyz || !d_cn) return fail(TBX_ERR_ARG, "null argument");
+    if (!b->cn_set) return fail(TBX_ERR_ARG, "coordination number not set up (tbx_batch_set_ncoord)");
+    HIP_TRY(tbx_launch_ncoord(&b->tab.mt, b->rcov.as<double>(), b->cn_ka, b->cn_kb, b->cn_rshift,
+                              b->cn_cutoff, d_xyz, d_cn, pick_stream(b, stream)));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_ncoord_gradient(tbx_batch *b, const double *d_xyz,
+                                         const double *d_dEdcn, double *d_grad, double *d_sigma,
+                                         void *stream)
+{
+    if (!b || !d_xyz || !d_dEdcn || !d_grad || !d_sigma) return fail(TBX_ERR_ARG, "null argument");
+    if (!b->cn_set) return fail(TBX_ERR_ARG, "coordination number not set up (tbx_batch_set_ncoord)");
+    HIP_TRY(tbx_launch_ncoord_grad(&b->tab.mt, b->rcov.as<double>(), b->cn_ka, b->cn_kb,
+                                   b->cn_rshift, b->cn_cutoff, d_xyz, d_dEdcn, d_grad, d_sigma,
+                                   pick_stream(b, stream)));
     return TBX_OK;
 }
 

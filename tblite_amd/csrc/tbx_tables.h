@@ -1,6 +1,7 @@
 // tbx_tables.h -- layout shared by the host set-up (tbx_capi.hip) and the
 // gfx950 kernels (tbx_kernels.hip).  Internal, not part of the C-ABI.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define TBX_MAXCART 24 /* Cartesian functions per species: <= 4 shells, l <= 2 */
@@ -109,7 +110,8 @@ struct TbxUnitHead {
 struct TbxMols {
     int nmol, nat;
     int ntrans;
-    const double *trans;           // [3*ntrans]
+    const double *trans;           // [3*ntrans] translation vectors of all molecules
+    const int2 *mol_tr;            // [nmol] (first, count): translations of molecule m
     const int *mol_atom;           // [nmol+1] first atom
     const int *mol_nao;            // [nmol]
     const int *mol_cart;           // [nmol+1] first Cartesian function (global)

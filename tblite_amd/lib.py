@@ -114,6 +114,10 @@ _SIGS = {
     "tbx_batch_plan_stats": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(Tiles), _P,
                                         ct.c_double, ct.POINTER(PlanStats)]),
     "tbx_batch_order_pairs": (ct.c_int, [_P, _P, ct.c_double]),
+    "tbx_batch_set_ncoord": (ct.c_int, [_P, _P, ct.c_double, ct.c_double, ct.c_double,
+                                        ct.c_double]),
+    "tbx_batch_ncoord": (ct.c_int, [_P, _P, _P, _P]),
+    "tbx_batch_ncoord_gradient": (ct.c_int, [_P, _P, _P, _P, _P, _P]),
 }
 
 

@@ -61,6 +61,34 @@ def atomic_rad(z: int) -> float:
     return _ATOMIC_RAD_AA[z - 1] * AATOAU
 
 
+# Covalent radii of the coordination number (Angstrom), Z = 1..86: the
+# DFT-D3 table, i.e. the single-bond radii of P. Pyykko, M. Atsumi, Chem. Eur.
+# J. 15, 186 (2009) with the values of metals reduced.  mctc-lib's
+# get_covalent_rad (the radii of the GFN2 coordination number,
+# xtb/gfn2.f90:640) returns 4/3 of these in bohr.  mctc-lib is not part of the
+# reference tree; the H, Li, S and Si entries are fixed by the reference's
+# literal H0 matrices (test/unit/test_hamiltonian.f90), whose CN comes from it.
+_COVALENT_RAD_D3_AA = [
+    0.32, 0.46, 1.20, 0.94, 0.77, 0.75, 0.71, 0.63, 0.64, 0.67, 1.40, 1.25, 1.13, 1.04, 1.10,
+    1.02, 0.99, 0.96, 1.76, 1.54, 1.33, 1.22, 1.21, 1.10, 1.07, 1.04, 1.00, 0.99, 1.01, 1.09,
+    1.12, 1.09, 1.15, 1.10, 1.14, 1.17, 1.89, 1.67, 1.47, 1.39, 1.32, 1.24, 1.15, 1.13, 1.13,
+    1.08, 1.15, 1.23, 1.28, 1.26, 1.26, 1.23, 1.32, 1.31, 2.09, 1.76, 1.62, 1.47, 1.58, 1.57,
+    1.56, 1.55, 1.51, 1.52, 1.51, 1.50, 1.49, 1.49, 1.48, 1.53, 1.46, 1.37, 1.31, 1.23, 1.18,
+    1.16, 1.11, 1.12, 1.13, 1.32, 1.30, 1.30, 1.36, 1.31, 1.38, 1.42,
+]
+
+# GFN2 double-exponential coordination number (mctc-lib ncoord "dexp"):
+# steepness of the two counting functions, offset of the second (bohr) and
+# the real-space cutoff (bohr, xtb/singlepoint.f90:55)
+CN_KA, CN_KB, CN_RSHIFT, CN_CUTOFF = 10.0, 20.0, 2.0, 25.0
+
+
+def covalent_rad(z: int) -> float:
+    """Covalent radius of the GFN2 coordination number in bohr (4/3 of the
+    D3 covalent radius, mctc-lib ``get_covalent_rad``)."""
+    return 4.0 / 3.0 * _COVALENT_RAD_D3_AA[z - 1] * AATOAU
+
+
 @lru_cache(maxsize=None)
 def _sto_table():
     return json.loads((DATA / "sto_ng.json").read_text())["table"]
