@@ -93,6 +93,15 @@ int tbx_get_selfenergy(int nat, const int *id, const int *ish_at, const int *nsh
                        const double *selfenergy_sp, const double *kcn, const double *cn,
                        double *selfenergy, double *dsedcn);
 
+/* get_lattice_points_cutoff (cutoff.f90:97-121): the lattice points within
+ * rthr of the cell lattice(3,3) (column k = lattice vector k, bohr), origin
+ * first, in the reference's order; one zero vector when no direction of
+ * periodic[3] is set.  On entry *ntrans is the room in trans [3 * *ntrans];
+ * on return the number of points.  Too little room: TBX_ERR_ARG, with
+ * *ntrans the number needed (call with *ntrans = 0 to ask).  Host only. */
+int tbx_get_lattice_points(const int *periodic, const double *lattice, double rthr, int *ntrans,
+                           double *trans);
+
 /* get_hamiltonian (xtb/h0.f90:220-429): dense S(nao,nao), D(3,nao,nao),
  * Q(6,nao,nao) (traceless, operator on the column function) and H0(nao,nao),
  * all intent(out).  `list` may be NULL: neighbours are the pairs within
@@ -172,6 +181,18 @@ typedef struct {
 } tbx_batch_sizes;
 
 int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tbx_batch **out);
+/* A batch of periodic structures (and molecules): molecule m has its own
+ * mol_ntrans[m] >= 1 translation vectors, stored one molecule after the other
+ * in trans [3 * sum mol_ntrans] (a molecule: one zero vector; a crystal: the
+ * lattice points of get_lattice_points, cutoff.f90:97-121, e.g. from
+ * tbx_get_lattice_points).  Pairs (i, j <= i, T) of every translation within
+ * desc->cutoff enter the integrals, as new_adjacency_list puts them in the
+ * half list (adjlist.f90:96-143); the coordination number sums over the same
+ * translations within its own cutoff, so they should reach the larger of the
+ * two.  The forward then adds the blocks of all images into zeroed outputs
+ * with FP64 atomics. */
+int tbx_batch_create_periodic(tbx_context *ctx, const tbx_batch_desc *desc,
+                              const int *mol_ntrans, const double *trans, tbx_batch **out);
 void tbx_batch_delete(tbx_batch *b);
 int tbx_batch_sizes_get(const tbx_batch *b, tbx_batch_sizes *sizes);
 /* Offsets into the packed per-molecule arrays: atom_off[nmol+1], sh_off[nmol+1],

@@ -274,3 +274,25 @@ def ncoord_dexp(ids, xyz, trans, rcov, ka=10.0, kb=20.0, rshift=2.0, cutoff=25.0
                           _p(dcndr) if derivatives else None,
                           _p(dcndL) if derivatives else None)
     return (cn, dcndr, dcndL) if derivatives else cn
+
+
+def get_lattice_points(periodic, lattice, rthr):
+    """get_lattice_points_cutoff (cutoff.f90:97-121) in plain Python: lattice
+    vectors are the rows of ``lattice``; returns (ntr, 3), origin first."""
+    lat = np.asarray(lattice, dtype=np.float64).reshape(3, 3)
+    if not any(periodic):
+        return np.zeros((1, 3))
+    rep = []
+    for k in range(3):  # get_translations, cutoff.f90:127-160
+        n = np.cross(lat[(k + 1) % 3], lat[(k + 2) % 3])
+        n = n / np.sqrt(np.sum(n * n))
+        rep.append(int(np.ceil(abs(rthr / np.sum(n * lat[k])))))
+    out = []
+    for ix in range(rep[0] + 1):  # get_lattice_points_rep_3d, cutoff.f90:39-94
+        for iy in range(rep[1] + 1):
+            for iz in range(rep[2] + 1):
+                for jx in ((1, -1) if ix > 0 else (1,)):
+                    for jy in ((1, -1) if iy > 0 else (1,)):
+                        for jz in ((1, -1) if iz > 0 else (1,)):
+                            out.append(lat[0] * ix * jx + lat[1] * iy * jy + lat[2] * iz * jz)
+    return np.array(out)

@@ -83,9 +83,15 @@ class Batch:
     """Batched GFN2 S/D/Q/H0 + gradient for a list of ``(numbers, xyz_bohr)``.
 
     ``order_pairs``: order the atom pairs of the work units near-first by the
-    given geometry (``order_pairs()``); it changes the speed, not the results."""
+    given geometry (``order_pairs()``); it changes the speed, not the results.
+    ``lattices``: optional, one entry per structure, ``None`` for a molecule
+    or ``(periodic[3], lattice)`` with the lattice vectors as the rows of
+    ``lattice`` (bohr); each periodic structure then sums over its lattice
+    points within the larger of the integral and coordination-number cutoffs
+    (``structure.get_lattice_points``, cutoff.f90:97-121)."""
 
-    def __init__(self, molecules, device: int = 0, order_pairs: bool = True, skin: float = 1.0):
+    def __init__(self, molecules, device: int = 0, order_pairs: bool = True, skin: float = 1.0,
+                 lattices=None):
         self.device = device
         d = _Description(molecules)
         self.nums, ids, self.spec, self.msh = d.nums, d.ids, d.spec, d.msh
@@ -97,7 +103,20 @@ class Batch:
         desc = d.desc
         p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
         handle = ct.c_void_p()
-        L.check(L.lib().tbx_batch_create(L.context(device), ct.byref(desc), ct.byref(handle)))
+        self.trans = None
+        if lattices is None:
+            L.check(L.lib().tbx_batch_create(L.context(device), ct.byref(desc), ct.byref(handle)))
+        else:
+            assert len(lattices) == len(molecules)
+            rthr = max(self.cutoff, params.CN_CUTOFF)
+            per = [np.zeros((1, 3)) if lat is None else
+                   structure.get_lattice_points(lat[0], lat[1], rthr) for lat in lattices]
+            self.mol_ntrans = np.array([len(t) for t in per], dtype=np.int32)
+            self.trans = np.ascontiguousarray(np.concatenate(per))
+            self.trans_off = np.concatenate([[0], np.cumsum(self.mol_ntrans)])
+            L.check(L.lib().tbx_batch_create_periodic(L.context(device), ct.byref(desc),
+                                                      p(self.mol_ntrans), p(self.trans),
+                                                      ct.byref(handle)))
         self.handle = handle
         sz = L.BatchSizes()
         L.check(L.lib().tbx_batch_sizes_get(handle, ct.byref(sz)))
@@ -149,6 +168,13 @@ class Batch:
         se0 = torch.as_tensor(self.selfenergy_sp, device=cn.device)
         kcn = torch.as_tensor(self.kcn_sh, device=cn.device)
         return se0 - kcn * cn_sh, -kcn
+
+    def molecule_translations(self, m):
+        """Translation vectors (ntr, 3) of structure ``m`` (one zero vector for
+        a molecule)."""
+        if self.trans is None:
+            return np.zeros((1, 3))
+        return self.trans[self.trans_off[m]: self.trans_off[m + 1]]
 
     def molecule_views(self, flat, comps=1):
         """Split a packed dense tensor into per-molecule (comps, nao, nao)

@@ -294,7 +294,9 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
 
 struct HostMols {
     int nmol = 0, nat = 0, nsh = 0, nao = 0, ncart = 0;
-    // translations of each molecule (first, count) into the translation table
+    // translation vectors (3 per entry) and, per molecule, the (first, count)
+    // of its own into that table: one zero vector for molecules
+    std::vector<double> trans;
     std::vector<int2> mol_tr;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
@@ -315,19 +317,19 @@ struct HostMols {
 // distance come first and the others follow in units of their own: every pair
 // is still in exactly one unit and the kernels still screen every pair, so
 // the hint changes the speed, never the results.
-static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntrans, bool diag,
+static void build_units_list(const HostSpecies &hs, const HostMols &hm, bool diag,
                              int max_tasks, int max_pairs, std::vector<TbxSeg> &useg,
                              std::vector<int> &unit_seg)
 {
-    const bool hinted = ntrans == 1 && hm.hint_xyz.size() == 3 * (size_t)hm.nat;
-    auto near = [&](int i, int j) {
-        if (!hinted || i == j) return true;
+    const bool hinted = hm.hint_xyz.size() == 3 * (size_t)hm.nat;
+    auto near = [&](int i, int j, int itr) {
+        if (!hinted) return true;
         double r2 = 0.0;
         for (int c = 0; c < 3; ++c) {
-            const double d = hm.hint_xyz[3 * i + c] - hm.hint_xyz[3 * j + c];
+            const double d = hm.hint_xyz[3 * i + c] - hm.hint_xyz[3 * j + c] - hm.trans[3 * itr + c];
             r2 += d * d;
         }
-        return r2 <= hm.near2;
+        return i == j && r2 < 1e-12 ? true : r2 <= hm.near2;
     };
     useg.clear();
     unit_seg.assign(1, 0);
@@ -340,14 +342,16 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, int ntra
     };
     for (int m = 0; m < hm.nmol; ++m) {
         const int a0 = hm.mol_atom[m], a1 = hm.mol_atom[m + 1];
+        const int2 tr = hm.mol_tr[m];
         for (int pass = 0; pass < (hinted ? 2 : 1); ++pass) {
-            for (int itr = 0; itr < ntrans; ++itr)
+            for (int itr = tr.x; itr < tr.x + tr.y; ++itr)
                 for (int i = a0; i < a1; ++i) {
-                    const int jend = (diag || ntrans > 1) ? i + 1 : i;
+                    // with translations the self images (i, i, T) are pairs too
+                    const int jend = (diag || tr.y > 1) ? i + 1 : i;
                     const int nci = hs.ncart[hm.at_sp[i]];
                     open_seg = false;
                     for (int j = a0; j < jend; ++j) {
-                        if (hinted && near(i, j) != (pass == 0)) {
+                        if (hinted && near(i, j, itr) != (pass == 0)) {
                             open_seg = false;  // a segment is a run of consecutive j
                             continue;
                         }
@@ -504,12 +508,10 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
     return true;
 }
 
-static int build_units(const HostSpecies &hs, HostMols &hm, int ntrans, const tbx_tiles &t)
+static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
 {
-    build_units_list(hs, hm, ntrans, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg,
-                     hm.unit_seg);
-    build_units_list(hs, hm, ntrans, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg,
-                     hm.funit_seg);
+    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg, hm.unit_seg);
+    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg, hm.funit_seg);
     if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, TBX_GRAD_UNIT_TASKS,
                      TBX_GRAD_UNIT_PAIRS, hm.uhead, hm.uplan) ||
         !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, TBX_UNIT_TASKS, TBX_UNIT_PAIRS,
@@ -548,6 +550,8 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
 {
     if (nmol <= 0 || !mol_nat || !id) return fail(TBX_ERR_ARG, "empty batch");
     hm.nmol = nmol;
+    hm.trans.assign(3, 0.0);
+    hm.mol_tr.assign(nmol, make_int2(0, 1));
     hm.mol_atom.assign(nmol + 1, 0);
     for (int m = 0; m < nmol; ++m) {
         if (mol_nat[m] <= 0) return fail(TBX_ERR_ARG, "molecule %d has %d atoms", m, mol_nat[m]);
@@ -691,16 +695,12 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     return hipSuccess;
 }
 
-static hipError_t upload_mols(DevTables &d, const HostMols &hm, const std::vector<double> &trans,
-                              hipStream_t s)
+static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
 {
     Arena &a = d.mol_host;
     a.host.clear();
-    // every molecule has the whole translation table unless the batch says otherwise
-    std::vector<int2> mol_tr = hm.mol_tr;
-    if (mol_tr.size() != (size_t)hm.nmol)
-        mol_tr.assign(hm.nmol, make_int2(0, (int)(trans.size() / 3)));
-    const size_t o_mol_tr = a.add(mol_tr);
+    const std::vector<double> &trans = hm.trans;
+    const size_t o_mol_tr = a.add(hm.mol_tr);
     const size_t o_trans = a.add(trans), o_mol_atom = a.add(hm.mol_atom),
                  o_mol_nao = a.add(hm.mol_nao), o_mol_cart = a.add(hm.mol_cart),
                  o_mol_dense = a.add(hm.mol_dense), o_at_sp = a.add(hm.at_sp),
@@ -872,6 +872,55 @@ int tbx_get_selfenergy(int nat, const int *id, const int *ish_at, const int *nsh
     return TBX_OK;
 }
 
+// get_lattice_points_cutoff (cutoff.f90:97-121): the repetitions along each
+// lattice vector follow from the distances between lattice planes
+// (get_translations, :127-160) and the points are listed in the order of
+// get_lattice_points_rep_3d with the origin first (:39-94).
+int tbx_get_lattice_points(const int *periodic, const double *lattice, double rthr, int *ntrans,
+                           double *trans)
+{
+    if (!periodic || !lattice || !ntrans || (*ntrans > 0 && !trans))
+        return fail(TBX_ERR_ARG, "tbx_get_lattice_points: null argument");
+    const int cap = *ntrans;
+    if (!periodic[0] && !periodic[1] && !periodic[2]) {
+        *ntrans = 1;
+        if (cap < 1) return fail(TBX_ERR_ARG, "room for 1 translation needed");
+        trans[0] = trans[1] = trans[2] = 0.0;
+        return TBX_OK;
+    }
+    if (!(rthr >= 0.0) || !std::isfinite(rthr)) return fail(TBX_ERR_ARG, "cutoff %g", rthr);
+    const double *a[3] = {lattice, lattice + 3, lattice + 6};
+    int rep[3];
+    for (int k = 0; k < 3; ++k) {
+        const double *u = a[(k + 1) % 3], *v = a[(k + 2) % 3];
+        const double n[3] = {u[1] * v[2] - v[1] * u[2], u[2] * v[0] - v[2] * u[0],
+                             u[0] * v[1] - v[0] * u[1]};
+        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        const double c = (n[0] * a[k][0] + n[1] * a[k][1] + n[2] * a[k][2]) / nn;
+        const double r = std::ceil(std::fabs(rthr / c));
+        if (!std::isfinite(r) || r > 1000.0)
+            return fail(TBX_ERR_ARG, "degenerate lattice or cutoff: %g repetitions", r);
+        rep[k] = (int)r;
+    }
+    const long long need = (2LL * rep[0] + 1) * (2LL * rep[1] + 1) * (2LL * rep[2] + 1);
+    if (need > INT_MAX) return fail(TBX_ERR_ARG, "too many lattice points");
+    *ntrans = (int)need;
+    if (cap < need) return fail(TBX_ERR_ARG, "room for %lld translations needed", need);
+    int itr = 0;
+    for (int ix = 0; ix <= rep[0]; ++ix)
+        for (int iy = 0; iy <= rep[1]; ++iy)
+            for (int iz = 0; iz <= rep[2]; ++iz)
+                for (int jx = 1; jx >= (ix > 0 ? -1 : 1); jx -= 2)
+                    for (int jy = 1; jy >= (iy > 0 ? -1 : 1); jy -= 2)
+                        for (int jz = 1; jz >= (iz > 0 ? -1 : 1); jz -= 2) {
+                            for (int c = 0; c < 3; ++c)
+                                trans[3 * itr + c] = a[0][c] * ix * jx + a[1][c] * iy * jy +
+                                                     a[2][c] * iz * jz;
+                            ++itr;
+                        }
+    return TBX_OK;
+}
+
 } // extern "C"
 
 // The raw bytes of a structure's species ids and translations.
@@ -936,7 +985,6 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
         }
     }
     if (list && (!list->nnl || !list->inl)) return fail(TBX_ERR_ARG, "incomplete neighbour list");
-    std::vector<double> trans(geom->trans, geom->trans + 3 * geom->ntrans);
     hipStream_t s = ctx->stream;
     HIP_TRY(hipSetDevice(ctx->device));
     if (!reuse) {
@@ -948,9 +996,11 @@ static int prepare_single(tbx_context *ctx, const tbx_geom *geom, const tbx_list
         ctx->hs_valid = true;
     }
     if (!reuse_mols) {
-        if ((st = build_units(ctx->hs, hm_new, geom->ntrans, kDefaultTiles))) return st;
+        hm_new.trans.assign(geom->trans, geom->trans + 3 * geom->ntrans);
+        hm_new.mol_tr.assign(1, make_int2(0, geom->ntrans));
+        if ((st = build_units(ctx->hs, hm_new, kDefaultTiles))) return st;
         HIP_TRY(hipStreamSynchronize(s));
-        HIP_TRY(upload_mols(ctx->tab, hm_new, trans, s));
+        HIP_TRY(upload_mols(ctx->tab, hm_new, s));
         ctx->hm = std::move(hm_new);
         ctx->hm_raw = std::move(mraw);
         ctx->hm_valid = true;
@@ -1182,7 +1232,10 @@ extern "C" int tbx_get_hamiltonian_adjoint(tbx_context *ctx, const tbx_geom *geo
 // ---------------------------------------------------------------------------
 // batched path
 
-extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tbx_batch **out)
+// Batch set-up shared by tbx_batch_create and tbx_batch_create_periodic:
+// mol_ntrans / trans NULL for molecules (one zero translation each).
+static int batch_create(tbx_context *ctx, const tbx_batch_desc *desc, const int *mol_ntrans,
+                        const double *trans, tbx_batch **out)
 {
     if (!ctx || !desc || !out) return fail(TBX_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1193,18 +1246,36 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     b->cutoff = desc->cutoff;
     int st = build_species(desc->bas, desc->h0, b->hs);
     if (!st) st = build_mols(b->hs, desc->nmol, desc->mol_nat, desc->id, b->hm);
+    if (!st && mol_ntrans) {
+        long long ntot = 0;
+        for (int m = 0; m < desc->nmol && !st; ++m) {
+            if (mol_ntrans[m] < 1)
+                st = fail(TBX_ERR_ARG, "molecule %d has %d translations (>= 1)", m, mol_ntrans[m]);
+            ntot += mol_ntrans[m];
+        }
+        if (!st && (!trans || ntot > INT_MAX / 3)) st = fail(TBX_ERR_ARG, "invalid translations");
+        for (long long k = 0; !st && k < 3 * ntot; ++k)
+            if (!std::isfinite(trans[k])) st = fail(TBX_ERR_ARG, "non-finite translation %lld", k);
+        if (!st) {
+            b->hm.trans.assign(trans, trans + 3 * ntot);
+            int first = 0;
+            for (int m = 0; m < desc->nmol; ++m) {
+                b->hm.mol_tr[m] = make_int2(first, mol_ntrans[m]);
+                first += mol_ntrans[m];
+            }
+        }
+    }
     if (st) {
         delete b;
         return st;
     }
     hipError_t e = hipSetDevice(ctx->device);
-    std::vector<double> trans(3, 0.0);
     if (e == hipSuccess) e = upload_species(b->tab, b->hs, desc->cutoff, ctx->stream);
-    if ((st = build_units(b->hs, b->hm, 1, b->tiles))) {
+    if ((st = build_units(b->hs, b->hm, b->tiles))) {
         tbx_batch_delete(b);
         return st;
     }
-    if (e == hipSuccess) e = upload_mols(b->tab, b->hm, trans, ctx->stream);
+    if (e == hipSuccess) e = upload_mols(b->tab, b->hm, ctx->stream);
     if (e == hipSuccess) e = b->flag.reserve(sizeof(int));
     // scratch of the gradient / adjoint launches: dE/dse (plain) or sigma (adjoint)
     if (e == hipSuccess)
@@ -1216,27 +1287,40 @@ extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tb
     b->tab.mol_host.host = std::vector<char>();
     if (e != hipSuccess) {
         tbx_batch_delete(b);
-        return fail(TBX_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e));
+        return fail(e == hipErrorOutOfMemory ? TBX_ERR_ALLOC : TBX_ERR_DEVICE,
+                    "batch upload: %s", hipGetErrorString(e));
     }
     *out = b;
     return TBX_OK;
+}
+
+extern "C" int tbx_batch_create(tbx_context *ctx, const tbx_batch_desc *desc, tbx_batch **out)
+{
+    return batch_create(ctx, desc, nullptr, nullptr, out);
+}
+
+extern "C" int tbx_batch_create_periodic(tbx_context *ctx, const tbx_batch_desc *desc,
+                                         const int *mol_ntrans, const double *trans,
+                                         tbx_batch **out)
+{
+    if (!mol_ntrans || !trans) return fail(TBX_ERR_ARG, "null translations");
+    return batch_create(ctx, desc, mol_ntrans, trans, out);
 }
 
 extern "C" int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles)
 {
     if (!b || !tiles) return fail(TBX_ERR_ARG, "null argument");
     if (int st = check_tiles(b->hs, tiles)) return st;
-    if (int st = build_units(b->hs, b->hm, 1, *tiles)) {
-        (void)build_units(b->hs, b->hm, 1, b->tiles);  // keep the host copy of the current units
+    if (int st = build_units(b->hs, b->hm, *tiles)) {
+        (void)build_units(b->hs, b->hm, b->tiles);  // keep the host copy of the current units
         return st;
     }
     b->tiles = *tiles;
-    std::vector<double> trans(3, 0.0);
     HIP_TRY(hipSetDevice(b->ctx->device));
     // The batch's kernels run on the callers' streams: wait for every stream
     // of the device, so that no launch still reads the units replaced here.
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
+    HIP_TRY(upload_mols(b->tab, b->hm, b->ctx->stream));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     b->tab.mol_host.host = std::vector<char>();
     return TBX_OK;
@@ -1266,18 +1350,17 @@ extern "C" int tbx_batch_order_pairs(tbx_batch *b, const double *xyz, double ski
     const std::vector<double> old_xyz = b->hm.hint_xyz;
     const double old_near2 = b->hm.near2;
     if (int st = set_hint(b->hm, xyz, skin, b->cutoff)) return st;
-    if (int st = build_units(b->hs, b->hm, 1, b->tiles)) {
+    if (int st = build_units(b->hs, b->hm, b->tiles)) {
         b->hm.hint_xyz = old_xyz;  // keep the host copy of the current units
         b->hm.near2 = old_near2;
-        (void)build_units(b->hs, b->hm, 1, b->tiles);
+        (void)build_units(b->hs, b->hm, b->tiles);
         return st;
     }
-    std::vector<double> trans(3, 0.0);
     HIP_TRY(hipSetDevice(b->ctx->device));
     // The batch's kernels run on the callers' streams: wait for every stream
     // of the device, so that no launch still reads the units replaced here.
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(upload_mols(b->tab, b->hm, trans, b->ctx->stream));
+    HIP_TRY(upload_mols(b->tab, b->hm, b->ctx->stream));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     b->tab.mol_host.host = std::vector<char>();
     return TBX_OK;
@@ -1330,7 +1413,7 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
     if (st) return st;
     const tbx_tiles t = tiles ? *tiles : kDefaultTiles;
     if ((st = check_tiles(hs, &t))) return st;
-    if ((st = build_units(hs, hm, 1, t))) return st;
+    if ((st = build_units(hs, hm, t))) return st;
     tbx_plan_stats s = {};
     s.ndense = hm.ndense;
     s.nunit_fwd = (long long)hm.fhead.size();

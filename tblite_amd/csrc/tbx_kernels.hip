@@ -1446,10 +1446,12 @@ __global__ void k_listcheck(TbxSpecies sp, TbxMols mt, const double *xyz, const 
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= mt.nat) return;
-    const int a0 = mt.mol_atom[mt.at_mol[i]];
+    const int m = mt.at_mol[i];
+    const int a0 = mt.mol_atom[m];
+    const int2 tr = mt.mol_tr[m];
     int n = 0;
     for (int j = a0; j <= i; ++j)
-        for (int itr = 0; itr < mt.ntrans; ++itr) {
+        for (int itr = tr.x; itr < tr.x + tr.y; ++itr) {
             const double vx = xyz[3 * i] - xyz[3 * j] - mt.trans[3 * itr];
             const double vy = xyz[3 * i + 1] - xyz[3 * j + 1] - mt.trans[3 * itr + 1];
             const double vz = xyz[3 * i + 2] - xyz[3 * j + 2] - mt.trans[3 * itr + 2];

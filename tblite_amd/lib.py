@@ -78,6 +78,8 @@ _SIGS = {
     "tbx_context_new": (_P, [ct.c_int]),
     "tbx_context_delete": (None, [_P]),
     "tbx_get_selfenergy": (ct.c_int, [ct.c_int, _P, _P, _P, ct.c_int, _P, _P, _P, _P, _P]),
+    "tbx_get_lattice_points": (ct.c_int, [_P, _P, ct.c_double, ct.POINTER(ct.c_int), _P]),
+    "tbx_batch_create_periodic": (ct.c_int, [_P, ct.POINTER(BatchDesc), _P, _P, ct.POINTER(_P)]),
     "tbx_get_hamiltonian": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(List), ct.POINTER(Basis),
                                        ct.POINTER(H0Par), _P, _P, ct.c_int, _P, _P, _P, _P, _P]),
     "tbx_get_multipole_integrals": (ct.c_int, [_P, ct.POINTER(Geom), ct.POINTER(Basis), _P, _P,

@@ -37,6 +37,26 @@ class Structure:
         return len(self.nums)
 
 
+def get_lattice_points(periodic, lattice, rthr: float) -> np.ndarray:
+    """``get_lattice_points_cutoff`` (``cutoff.f90:97-121``) through the C-ABI:
+    the lattice points within ``rthr`` (bohr) of the cell whose lattice
+    vectors are the rows of ``lattice`` (3, 3), origin first, shape (ntr, 3).
+    No periodic direction: the single zero translation."""
+    import ctypes as ct
+
+    from . import lib as L
+    per = np.ascontiguousarray([1 if p else 0 for p in periodic], dtype=np.int32)
+    lat = np.ascontiguousarray(np.asarray(lattice, dtype=np.float64).reshape(3, 3))
+    p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+    n = ct.c_int(0)  # no room: the call reports the number of points
+    st = L.lib().tbx_get_lattice_points(p(per), p(lat), float(rthr), ct.byref(n), None)
+    if n.value <= 0:
+        L.check(st)
+    out = np.zeros((n.value, 3))
+    L.check(L.lib().tbx_get_lattice_points(p(per), p(lat), float(rthr), ct.byref(n), p(out)))
+    return out
+
+
 def lattice_points_molecule() -> np.ndarray:
     """``get_lattice_points_cutoff`` for a non-periodic system (``cutoff.f90:97-121``):
     a single zero translation, shape (ntr, 3)."""
