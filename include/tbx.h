@@ -228,6 +228,22 @@ int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
                        const double *d_vao, const double *d_vdp, const double *d_vqp,
                        double *d_dEdcn, double *d_grad, double *d_sigma, void *stream);
 
+/* tbx_batch_gradient for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Same arguments and results
+ * as tbx_batch_gradient, but its dE/dse partial sums live in a static device
+ * array of TBX_NOALLOC_MAX_SHELLS doubles (8 MiB, held by every device the
+ * library runs on until the process ends) instead of the batch's own
+ * buffer; it calls no hipMalloc, hipFree or allocating library routine.
+ * Batches with more shells are rejected with TBX_ERR_ARG.  Every call on one
+ * device shares that array: calls that overlap on one device (different
+ * streams, or batches used from several threads) race. */
+#define TBX_NOALLOC_MAX_SHELLS (1 << 20)
+int tbx_batch_gradient_noalloc(tbx_batch *b, const double *d_xyz, const double *d_se,
+                               const double *d_dsedcn, const double *d_P, const double *d_W,
+                               const double *d_vao, const double *d_vdp, const double *d_vqp,
+                               double *d_dEdcn, double *d_grad, double *d_sigma, void *stream);
+
 /* Adjoint (backward) of tbx_batch_forward: given dense adjoints d_Sbar,
  * d_Hbar (ndense), d_Dbar (3*ndense), d_Qbar (6*ndense) of the forward
  * outputs, write dE/dxyz [3*nat] and dE/dse [nsh] (OVERWRITTEN).  The

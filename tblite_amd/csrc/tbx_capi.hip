@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -108,6 +109,41 @@ hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s)
     return hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
 }
 
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Static device scratch of tbx_batch_gradient_noalloc: dE/dse per shell
+// (TBX_NOALLOC_MAX_SHELLS doubles, 8 MiB; every device the library runs on
+// holds one copy until the process ends).  In host code `scratch` is only a
+// stand-in for the symbol: its device address, one per device, is looked up
+// once with hipGetSymbolAddress when the first context on that device is
+// created, and only that address is passed to kernels.
+static __device__ __align__(256) double scratch[TBX_NOALLOC_MAX_SHELLS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // by device
+
+// Device address of `scratch` on `device` (which must be current).
+int scratch_init(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return TBX_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(TBX_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return TBX_OK;
+}
+
+double *scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -830,6 +866,7 @@ tbx_context *tbx_context_new(int device)
         fail(TBX_ERR_DEVICE, "hipSetDevice(%d) failed", device);
         return nullptr;
     }
+    if (scratch_init(device) != TBX_OK) return nullptr;
     tbx_context *ctx = new tbx_context;
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1614,6 +1651,39 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
                              d_dEdcn, s));
     HIP_TRY(mark(b, 5, s));
+    return TBX_OK;
+}
+
+// tbx_batch_gradient with the dE/dse partial sums in the static `scratch`
+// (through its device address) instead of the batch's buffer: no device
+// allocation and no allocating library call.
+extern "C" int tbx_batch_gradient_noalloc(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                          const double *d_dsedcn, const double *d_P,
+                                          const double *d_W, const double *d_vao,
+                                          const double *d_vdp, const double *d_vqp,
+                                          double *d_dEdcn, double *d_grad, double *d_sigma,
+                                          void *stream)
+{
+    if (!b || !d_xyz || !d_se || !d_dsedcn || !d_P || !d_W || !d_vao || !d_vdp || !d_vqp ||
+        !d_dEdcn || !d_grad || !d_sigma)
+        return fail(TBX_ERR_ARG, "null argument");
+    if (b->hm.nsh > TBX_NOALLOC_MAX_SHELLS)
+        return fail(TBX_ERR_ARG,
+                    "tbx_batch_gradient_noalloc: %d shells exceed the static scratch of %d; "
+                    "use tbx_batch_gradient",
+                    b->hm.nsh, TBX_NOALLOC_MAX_SHELLS);
+    double *dse = scratch_ptr(b->ctx->device);
+    if (!dse) return fail(TBX_ERR_DEVICE, "no static scratch on device %d", b->ctx->device);
+    hipStream_t s = pick_stream(b, stream);
+    // the scratch keeps the previous call's sums: clear it (a fill kernel)
+    HIP_TRY(tbx_launch_fill(dse, b->hm.nsh, 0.0, s));
+    HIP_TRY(tbx_launch_fill(d_grad, 3LL * b->hm.nat, 0.0, s));
+    HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
+    HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp,
+                                d_vqp, 1, b->hm.ndense, b->hm.nao, nullptr, nullptr, nullptr,
+                                nullptr, d_grad, dse, d_sigma, nullptr, b->flag.as<int>(),
+                                b->tiles.grad_chunk, s));
+    HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, dse, d_dsedcn, d_P, d_dEdcn, s));
     return TBX_OK;
 }
 

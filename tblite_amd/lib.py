@@ -107,6 +107,8 @@ _SIGS = {
     "tbx_batch_offsets": (ct.c_int, [_P, _P, _P, _P, _P]),
     "tbx_batch_forward": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "tbx_batch_gradient": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "tbx_batch_gradient_noalloc": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                              _P]),
     "tbx_batch_adjoint": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "tbx_batch_last_timing": (ct.c_int, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
                                          ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
