@@ -315,10 +315,10 @@ def main():
     agree = float((gx - gp).abs().max())
     agree_graph = float((aouts[0] - gx).abs().max())
 
-    # plain per-molecule GPU path: one molecule per launch, inputs and
-    # outputs resident in HBM (views into the batch's packed tensors), on
-    # every 8th molecule of the workload (all sizes); each molecule is its own
-    # one-molecule batch, created before the timed loop
+    # plain per-molecule GPU path: one molecule per launch, inputs resident in
+    # HBM (views into the batch's packed tensors) and outputs preallocated per
+    # molecule, on every 8th molecule of the workload (all sizes); each
+    # molecule is its own one-molecule batch, created before the timed loop
     sample = list(range(0, b.nmol, 8))
     singles = []
     for m in sample:
@@ -329,8 +329,10 @@ def main():
         bm = Batch([mols[m]])
         ins = (xyz[3 * a0: 3 * a1], se[s0:s1], dsedcn[s0:s1], P[d0:d1], W[d0:d1],
                vao[o0:o1], vdp[3 * a0: 3 * a1], vqp[6 * a0: 6 * a1])
-        fo = (outs[0][d0:d1], outs[1][3 * d0: 3 * d1], outs[2][6 * d0: 6 * d1], outs[3][d0:d1])
-        go = (gouts[0][a0:a1], gouts[1][3 * a0: 3 * a1], gouts[2][9 * m: 9 * m + 9])
+        nd = d1 - d0
+        fo = tuple(torch.empty(c * nd, dtype=torch.float64, device=dev) for c in (1, 3, 6, 1))
+        go = tuple(torch.empty(n, dtype=torch.float64, device=dev)
+                   for n in (a1 - a0, 3 * (a1 - a0), 9))
         singles.append((bm, ins, fo, go))
 
     def step_single():
