@@ -1192,6 +1192,41 @@ __device__ __forceinline__ void seg_scan_step(int key, double g[3], double &ds)
     }
 }
 
+// Segmented inclusive scan of N doubles over the 64 lanes of a wavefront,
+// runs of equal key (the same DPP steps as the work-item sums): the last
+// lane of each run ends with the run's total.
+template <int CTRL, int ROWMASK, int N>
+__device__ __forceinline__ void seg_scan_step_n(int key, double (&v)[N])
+{
+    const int k2 = __builtin_amdgcn_update_dpp(INT_MIN, key, CTRL, ROWMASK, 0xf, false);
+    double y[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) y[n] = dpp_move<CTRL, ROWMASK>(v[n]);
+    if (k2 == key) {
+#pragma unroll
+        for (int n = 0; n < N; ++n) v[n] += y[n];
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void seg_scan_n(int key, double (&v)[N])
+{
+    seg_scan_step_n<0x111, 0xf>(key, v);  // row_shr:1
+    seg_scan_step_n<0x112, 0xf>(key, v);  // row_shr:2
+    seg_scan_step_n<0x114, 0xf>(key, v);  // row_shr:4
+    seg_scan_step_n<0x118, 0xf>(key, v);  // row_shr:8
+    seg_scan_step_n<0x142, 0xa>(key, v);  // row_bcast:15 into rows 1 and 3
+    seg_scan_step_n<0x143, 0xc>(key, v);  // row_bcast:31 into rows 2 and 3
+}
+
+// sum over the 64 lanes of a wavefront (every lane gets the total)
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
 // One workgroup per work unit (up to TBX_UNIT_PAIRS atom pairs of one
 // molecule, at most TBX_UNIT_TASKS Cartesian-pair tasks).  (1) The unit's
 // pairs are staged in LDS and their shell pairs laid out by type (l_i, l_j);
@@ -1207,8 +1242,6 @@ __global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
     __shared__ GradStage U;
     __shared__ double rec[NREC * GTHREADS];
     __shared__ double accP[TBX_GRAD_UNIT_PAIRS][3 + 2 * MAXSH];  // G (3), dse_i, dse_j per shell
-    __shared__ double accS[TBX_UNIT_SEGS][3 + MAXSH];      // atom-i sums per segment
-    __shared__ double sig[6];
     __shared__ int tinfo[GTHREADS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
     __shared__ int tpp[GTHREADS];    // first primitive pair of the task
     __shared__ int tkept[GTHREADS];
@@ -1226,8 +1259,6 @@ __global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
     const int ao_m = H.ao_m;
     const int chunk = A.chunk;
     for (int k = tid; k < TBX_GRAD_UNIT_PAIRS * (3 + 2 * MAXSH); k += GTHREADS) (&accP[0][0])[k] = 0.0;
-    for (int k = tid; k < TBX_UNIT_SEGS * (3 + MAXSH); k += GTHREADS) (&accS[0][0])[k] = 0.0;
-    if (tid < 6) sig[tid] = 0.0;
     // (1) the unit's plan
     load_unit(H, mt.plan, U);
     __syncthreads();
@@ -1354,48 +1385,62 @@ __global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
     }
     __syncthreads();
 
-    // (5) pair sums -> atom j (HBM), segment sums (atom i) and sigma (LDS)
-    if (tid < npair) {
-        const PairInfo pr = U.pinf[tid];
-        const int k = U.pext[tid] & 31;
-        const bool self = (pr.i == pr.j);
-        const double G0 = accP[tid][0], G1 = accP[tid][1], G2 = accP[tid][2];
-        const double vx = U.pv[tid][0], vy = U.pv[tid][1], vz = U.pv[tid][2];
-        if (!self && (G0 != 0.0 || G1 != 0.0 || G2 != 0.0)) {
-            unsafeAtomicAdd(&A.grad[3 * pr.j + 0], -G0);
-            unsafeAtomicAdd(&A.grad[3 * pr.j + 1], -G1);
-            unsafeAtomicAdd(&A.grad[3 * pr.j + 2], -G2);
-            atomicAdd(&accS[k][0], G0);
-            atomicAdd(&accS[k][1], G1);
-            atomicAdd(&accS[k][2], G2);
+    // (5) one lane per pair (npair <= 64: the first wavefront).  The pair's
+    // sums go to atom j (HBM atomics).  The atom-i sums of each segment (a
+    // run of consecutive pairs with one row atom i) are reduced by a
+    // segmented scan across the wavefront and added by the segment's last
+    // lane; the unit's sigma is a wavefront sum added by nine lanes.
+    if (tid < 64) {
+        const bool on = tid < npair;
+        const int key = on ? (U.pext[tid] & 31) : 64 + tid;  // segment; unique on idle lanes
+        double gi[3 + MAXSH], sg[6];
+#pragma unroll
+        for (int q = 0; q < 3 + MAXSH; ++q) gi[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sg[q] = 0.0;
+        PairInfo pr{};
+        if (on) {
+            pr = U.pinf[tid];
+            const bool self = (pr.i == pr.j);
+            const double G0 = accP[tid][0], G1 = accP[tid][1], G2 = accP[tid][2];
+            const double vx = U.pv[tid][0], vy = U.pv[tid][1], vz = U.pv[tid][2];
+            if (!self && (G0 != 0.0 || G1 != 0.0 || G2 != 0.0)) {
+                unsafeAtomicAdd(&A.grad[3 * pr.j + 0], -G0);
+                unsafeAtomicAdd(&A.grad[3 * pr.j + 1], -G1);
+                unsafeAtomicAdd(&A.grad[3 * pr.j + 2], -G2);
+                gi[0] = G0;
+                gi[1] = G1;
+                gi[2] = G2;
+            }
+            const double fs = self ? 0.25 : 0.5;
+            sg[0] = fs * 2.0 * vx * G0;
+            sg[1] = fs * (vx * G1 + G0 * vy);
+            sg[2] = fs * 2.0 * vy * G1;
+            sg[3] = fs * (vx * G2 + G0 * vz);
+            sg[4] = fs * (vy * G2 + G1 * vz);
+            sg[5] = fs * 2.0 * vz * G2;
+            for (int q = 0; q < (pr.dj & 7); ++q)
+                if (accP[tid][3 + MAXSH + q] != 0.0)
+                    unsafeAtomicAdd(&A.dse[pr.shj + q], accP[tid][3 + MAXSH + q]);
+#pragma unroll
+            for (int q = 0; q < MAXSH; ++q) gi[3 + q] = accP[tid][3 + q];
         }
-        const double fs = self ? 0.25 : 0.5;
-        if (G0 != 0.0 || G1 != 0.0 || G2 != 0.0) {
-            atomicAdd(&sig[0], fs * 2.0 * vx * G0);
-            atomicAdd(&sig[1], fs * (vx * G1 + G0 * vy));
-            atomicAdd(&sig[2], fs * 2.0 * vy * G1);
-            atomicAdd(&sig[3], fs * (vx * G2 + G0 * vz));
-            atomicAdd(&sig[4], fs * (vy * G2 + G1 * vz));
-            atomicAdd(&sig[5], fs * 2.0 * vz * G2);
+        seg_scan_n<3 + MAXSH>(key, gi);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sg[q] = wave_sum_d(sg[q]);
+        const int knext = __shfl_down(key, 1, 64);
+        if (on && (tid == npair - 1 || knext != key)) {  // last lane of its segment
+            for (int q = 0; q < 3; ++q)
+                if (gi[q] != 0.0) unsafeAtomicAdd(&A.grad[3 * pr.i + q], gi[q]);
+            for (int q = 0; q < (pr.di & 7); ++q)
+                if (gi[3 + q] != 0.0) unsafeAtomicAdd(&A.dse[pr.shi + q], gi[3 + q]);
         }
-        for (int q = 0; q < (pr.dj & 7); ++q)
-            if (accP[tid][3 + MAXSH + q] != 0.0)
-                unsafeAtomicAdd(&A.dse[pr.shj + q], accP[tid][3 + MAXSH + q]);
-        for (int q = 0; q < (pr.di & 7); ++q)
-            if (accP[tid][3 + q] != 0.0) atomicAdd(&accS[k][3 + q], accP[tid][3 + q]);
-    }
-    __syncthreads();
-    if (tid < nseg) {
-        const PairInfo &first = U.pinf[U.sinfo[tid].x];
-        const int i = first.i;
-        for (int q = 0; q < 3; ++q)
-            if (accS[tid][q] != 0.0) unsafeAtomicAdd(&A.grad[3 * i + q], accS[tid][q]);
-        for (int q = 0; q < (first.di & 7); ++q)
-            if (accS[tid][3 + q] != 0.0) unsafeAtomicAdd(&A.dse[first.shi + q], accS[tid][3 + q]);
-    } else if (tid >= 64 && tid < 64 + 9) {
-        const int k9 = tid - 64;
-        const double v = sig[kSigPack[k9]];
-        if (v != 0.0) unsafeAtomicAdd(&A.sigma[9 * m + k9], v);
+        if (tid < 9) {
+            double v = 0.0;
+#pragma unroll
+            for (int k9 = 0; k9 < 9; ++k9) v = (tid == k9) ? sg[kSigPack[k9]] : v;
+            if (v != 0.0) unsafeAtomicAdd(&A.sigma[9 * m + tid], v);
+        }
     }
 }
 
