@@ -343,7 +343,8 @@ struct HostMols {
     // optional geometry hint (bohr, [3*nat]) for the order of the pairs in the
     // units: pairs within sqrt(near2) there come first in each molecule
     std::vector<double> hint_xyz;
-    double near2 = 0.0;
+    double near2 = 0.0, skin = 0.0;
+    int nunit_near = 0;  // gradient units holding a pair within cutoff + skin at the hint
 };
 
 // Work units (see TbxSeg): the half list of every molecule, cut into units of
@@ -355,7 +356,7 @@ struct HostMols {
 // the hint changes the speed, never the results.
 static void build_units_list(const HostSpecies &hs, const HostMols &hm, bool diag,
                              int max_tasks, int max_pairs, std::vector<TbxSeg> &useg,
-                             std::vector<int> &unit_seg)
+                             std::vector<int> &unit_seg, std::vector<char> *unit_far = nullptr)
 {
     const bool hinted = hm.hint_xyz.size() == 3 * (size_t)hm.nat;
     auto near = [&](int i, int j, int itr) {
@@ -369,10 +370,14 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, bool dia
     };
     useg.clear();
     unit_seg.assign(1, 0);
-    int tasks = 0, pairs = 0, segs = 0;
+    if (unit_far) unit_far->clear();
+    int tasks = 0, pairs = 0, segs = 0, cur_pass = 0;
     bool open_seg = false;
     auto close_unit = [&]() {
-        if (segs > 0) unit_seg.push_back((int)useg.size());
+        if (segs > 0) {
+            unit_seg.push_back((int)useg.size());
+            if (unit_far) unit_far->push_back(cur_pass == 1 ? 1 : 0);
+        }
         tasks = pairs = segs = 0;
         open_seg = false;
     };
@@ -380,6 +385,7 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, bool dia
         const int a0 = hm.mol_atom[m], a1 = hm.mol_atom[m + 1];
         const int2 tr = hm.mol_tr[m];
         for (int pass = 0; pass < (hinted ? 2 : 1); ++pass) {
+            cur_pass = pass;
             for (int itr = tr.x; itr < tr.x + tr.y; ++itr)
                 for (int i = a0; i < a1; ++i) {
                     // with translations the self images (i, i, T) are pairs too
@@ -546,13 +552,22 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
 
 static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
 {
-    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg, hm.unit_seg);
+    std::vector<char> far;
+    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg, hm.unit_seg,
+                     &far);
     build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg, hm.funit_seg);
     if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, TBX_GRAD_UNIT_TASKS,
                      TBX_GRAD_UNIT_PAIRS, hm.uhead, hm.uplan) ||
         !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, TBX_UNIT_TASKS, TBX_UNIT_PAIRS,
                      hm.fhead, hm.fplan))
         return fail(TBX_ERR_ARG, "work-unit plan outside the kernel limits");
+    // gradient units of pairs beyond cutoff + skin at the hint go last: the
+    // kernels skip them while every atom stays within skin/2 of the hint
+    std::vector<TbxUnitHead> near, farh;
+    for (size_t u = 0; u < hm.uhead.size(); ++u) (far[u] ? farh : near).push_back(hm.uhead[u]);
+    hm.nunit_near = (int)near.size();
+    near.insert(near.end(), farh.begin(), farh.end());
+    hm.uhead.swap(near);
     return TBX_OK;
 }
 
@@ -744,7 +759,8 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
                  o_at_cart = a.add(hm.at_cart), o_cart_atom = a.add(hm.cart_atom),
                  o_ao_atom = a.add(hm.ao_atom), o_mol_ao = a.add(hm.mol_ao),
                  o_uhead = a.add(hm.uhead), o_uplan = a.add(hm.uplan), o_fhead = a.add(hm.fhead),
-                 o_fplan = a.add(hm.fplan), o_sh_aog = a.add(hm.sh_aog), o_tiles = a.add(hm.tiles);
+                 o_fplan = a.add(hm.fplan), o_sh_aog = a.add(hm.sh_aog), o_tiles = a.add(hm.tiles),
+                 o_hint = a.add(hm.hint_xyz);
     const hipError_t e = upload_arena(d.mols, a, s);
     if (e != hipSuccess) return e;
     const char *base = static_cast<const char *>(d.mols.p);
@@ -778,6 +794,12 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
     mt.sh_aog = reinterpret_cast<const int *>(at(o_sh_aog));
     mt.ntile = (int)(hm.tiles.size() / 4);
     mt.tiles = reinterpret_cast<const int *>(at(o_tiles));
+    // the far gradient units may be skipped while no atom moved more than
+    // (a little less than) half the skin away from the hint
+    const bool hinted = hm.hint_xyz.size() == 3 * (size_t)hm.nat && hm.nat > 0;
+    mt.nunit_near = hinted ? hm.nunit_near : mt.nunit;
+    mt.hint = hinted ? reinterpret_cast<const double *>(at(o_hint)) : nullptr;
+    mt.hint_r2 = 0.25 * 0.999 * hm.skin * hm.skin;
     return hipSuccess;
 }
 
@@ -1374,10 +1396,11 @@ static int set_hint(HostMols &hm, const double *xyz, double skin, double cutoff)
                 return fail(TBX_ERR_ARG, "coordinate %d is not finite", k);
     }
     hm.hint_xyz.clear();
-    hm.near2 = 0.0;
+    hm.near2 = hm.skin = 0.0;
     if (!xyz) return TBX_OK;
     hm.hint_xyz.assign(xyz, xyz + 3 * (size_t)hm.nat);
     hm.near2 = (cutoff + skin) * (cutoff + skin);
+    hm.skin = skin;
     return TBX_OK;
 }
 

@@ -508,8 +508,7 @@ __device__ __forceinline__ bool shellpair_setup(const TbxSpecies &sp, const TbxM
     const int4 e = unpack_spl(U.spl[k]);
     const PairInfo pr = U.pinf[e.x];
     const int ish = e.z, jsh = e.y;
-    double v[3];
-    pair_vec(mt, xyz, pr, U.pext[e.x], v);  // the same values as U.pv[e.x]
+    const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
     const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     const int sloti = ish + sp.msh * pr.spi, slotj = jsh + sp.msh * pr.spj;
     pi0 = sp.pp_off[sloti * sp.nslot + slotj];
@@ -1237,7 +1236,7 @@ __device__ __forceinline__ double wave_sum_d(double v)
 // the results to its pair's LDS sums; (5) the pair sums go to atoms j, the
 // segment sums to atoms i and the unit sum to sigma, with FP64 atomics.
 template <int MODE>
-__global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
+__device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
 {
     __shared__ GradStage U;
     __shared__ double rec[NREC * GTHREADS];
@@ -1251,7 +1250,7 @@ __global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
-    const TbxUnitHead H = mt.unit[blockIdx.x];
+    const TbxUnitHead H = mt.unit[unit];
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
     const int m = H.mol;
     const int nao = H.nao;
@@ -1444,6 +1443,44 @@ __global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
     }
 }
 
+template <int MODE>
+__global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
+{
+    gradient_unit<MODE>(A, blockIdx.x);
+}
+
+// The far gradient units (no pair within cutoff + skin at the pair-order
+// hint): nothing to do while every atom is within skin/2 of its hint
+// position (flag 0, set by k_hint_check), since then every such pair is
+// still beyond the cutoff.  Otherwise the workgroups walk them.
+template <int MODE>
+__global__ void __launch_bounds__(GTHREADS, 2) k_gradient_far(GradArgs A, const int *stale)
+{
+    if (*stale == 0) return;
+    for (int u = A.mol.nunit_near + blockIdx.x; u < A.mol.nunit; u += gridDim.x) {
+        __syncthreads();  // the previous unit's LDS is read until its last phase
+        gradient_unit<MODE>(A, u);
+    }
+}
+
+// stale = 1 if any atom moved more than sqrt(hint_r2) from its hint position
+__global__ void k_hint_check(TbxMols mt, const double *xyz, int *stale)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    if (i < mt.nat) {
+        const double dx = xyz[3 * i] - mt.hint[3 * i], dy = xyz[3 * i + 1] - mt.hint[3 * i + 1],
+                     dz = xyz[3 * i + 2] - mt.hint[3 * i + 2];
+        moved = !(dx * dx + dy * dy + dz * dz <= mt.hint_r2);  // NaN counts as moved
+    }
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(stale, 1);
+}
+
+__global__ void k_fill_int(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 // On-site blocks of the adjoint: H = S (se_i + se_j)/2 contributes to dE/dse
 // only.  16 lanes per atom, looping over its Cartesian pairs.
 __global__ void __launch_bounds__(NTHREADS) k_onsite(GradArgs A)
@@ -1579,13 +1616,29 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag,
                chunk > 0 ? chunk : GRAD_CHUNK_DEFAULT};
     (void)hipGetLastError();
-    if (mol->nunit > 0) {
+    const int nfar = mol->nunit - mol->nunit_near;
+    if (nfar > 0 && nnl) return hipErrorInvalidValue;  // the flag serves one purpose per launch
+    if (nfar > 0) {  // is the pair-order hint still good for the far units?
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, flag, 0);
+        hipLaunchKernelGGL(k_hint_check, dim3((mol->nat + 255) / 256), dim3(256), 0, stream,
+                           *mol, xyz, flag);
+    }
+    if (mol->nunit_near > 0) {
         if (mode == GRAD_PLAIN)
-            hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nunit), dim3(GTHREADS), 0, stream,
-                               a);
-        else
-            hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nunit), dim3(GTHREADS), 0,
+            hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nunit_near), dim3(GTHREADS), 0,
                                stream, a);
+        else
+            hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nunit_near), dim3(GTHREADS),
+                               0, stream, a);
+    }
+    if (nfar > 0) {
+        const int grid = nfar < 2048 ? nfar : 2048;
+        if (mode == GRAD_PLAIN)
+            hipLaunchKernelGGL(k_gradient_far<GRAD_PLAIN>, dim3(grid), dim3(GTHREADS), 0, stream,
+                               a, (const int *)flag);
+        else
+            hipLaunchKernelGGL(k_gradient_far<GRAD_ADJOINT>, dim3(grid), dim3(GTHREADS), 0, stream,
+                               a, (const int *)flag);
     }
     const int per = NTHREADS / 16;
     if (mode == GRAD_ADJOINT)

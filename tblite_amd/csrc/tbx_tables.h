@@ -125,7 +125,12 @@ struct TbxMols {
     const int *ao_atom;            // [total AO] atom (global)
     const int *mol_ao;             // [nmol+1] first AO (global)
     int nunit;
-    const TbxUnitHead *unit;       // [nunit] gradient units (j < i; j <= i with images)
+    const TbxUnitHead *unit;       // [nunit] gradient units (j < i; j <= i with images); the
+                                   // first nunit_near hold a pair within cutoff + skin of the
+                                   // pair-order hint, the others none
+    int nunit_near;
+    const double *hint;            // [3*nat] hint geometry (nullptr: no hint)
+    double hint_r2;                // squared displacement below which the far units stay dead
     const int *plan;               // their plans (16-byte words)
     int nfunit;
     const TbxUnitHead *funit;      // [nfunit] forward units (j <= i, on-site pairs included)

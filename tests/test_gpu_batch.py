@@ -285,10 +285,14 @@ def test_pair_order_hint_keeps_results(batch_case):
         torch.cuda.synchronize()
         return f + a + g
 
+    # hints displaced by less and by more than half the skin (0.25 bohr):
+    # the far units are skipped in the first case and walked in the second
+    d = rng.normal(size=b.xyz0.shape)
+    d /= np.linalg.norm(d, axis=1)[:, None]
     try:
         b.order_pairs(None)
         ref = run()
-        for hint in (b.xyz0, 3.0 * b.xyz0):
+        for hint in (b.xyz0, 3.0 * b.xyz0, b.xyz0 + 0.2 * d, b.xyz0 + 0.3 * d):
             b.order_pairs(hint, skin=0.5)
             for k, (r, g) in enumerate(zip(ref, run())):
                 assert torch.abs(r - g).max().item() < 1e-10, k
