@@ -345,6 +345,7 @@ struct HostMols {
     std::vector<double> hint_xyz;
     double near2 = 0.0, skin = 0.0;
     int nunit_near = 0;  // gradient units holding a pair within cutoff + skin at the hint
+    int nfunit_far = 0;  // forward units holding none
 };
 
 // Work units (see TbxSeg): the half list of every molecule, cut into units of
@@ -500,7 +501,11 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
         pad4();
         plan.insert(plan.end(), pext.begin(), pext.end());
         pad4();
-        plan.insert(plan.end(), sinfo.begin(), sinfo.end());
+        // the kernels read the segments' first AO of atom i where the host
+        // keeps the first pair
+        for (int k = 0; k < nseg; ++k)
+            plan.insert(plan.end(), {hm.at_ao[seg[s0 + k].i], sinfo[4 * k + 1], sinfo[4 * k + 2],
+                                     sinfo[4 * k + 3]});
         plan.insert(plan.end(), spl.begin(), spl.end());
         pad4();
         plan.insert(plan.end(), toff, toff + 10);
@@ -552,10 +557,11 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
 
 static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
 {
-    std::vector<char> far;
+    std::vector<char> far, ffar;
     build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg, hm.unit_seg,
                      &far);
-    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg, hm.funit_seg);
+    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg, hm.funit_seg,
+                     &ffar);
     if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, TBX_GRAD_UNIT_TASKS,
                      TBX_GRAD_UNIT_PAIRS, hm.uhead, hm.uplan) ||
         !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, TBX_UNIT_TASKS, TBX_UNIT_PAIRS,
@@ -568,6 +574,14 @@ static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
     hm.nunit_near = (int)near.size();
     near.insert(near.end(), farh.begin(), farh.end());
     hm.uhead.swap(near);
+    // forward units of such pairs stay in place (the XCD-contiguous unit
+    // order), flagged: while the hint holds they only write zero blocks
+    hm.nfunit_far = 0;
+    for (size_t u = 0; u < hm.fhead.size(); ++u)
+        if (ffar[u]) {
+            hm.fhead[u].nel |= TBX_UNIT_FAR;
+            ++hm.nfunit_far;
+        }
     return TBX_OK;
 }
 
@@ -798,6 +812,7 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
     // (a little less than) half the skin away from the hint
     const bool hinted = hm.hint_xyz.size() == 3 * (size_t)hm.nat && hm.nat > 0;
     mt.nunit_near = hinted ? hm.nunit_near : mt.nunit;
+    mt.nfunit_far = hinted ? hm.nfunit_far : 0;
     mt.hint = hinted ? reinterpret_cast<const double *>(at(o_hint)) : nullptr;
     mt.hint_r2 = 0.25 * 0.999 * hm.skin * hm.skin;
     return hipSuccess;
@@ -1499,7 +1514,7 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
             (fwd ? s.npair_fwd : s.npair_grad) += v.npair;
             (fwd ? s.ntask_fwd : s.ntask_grad) += ntask;
             if (ntask > cap_tasks || v.npair > cap_pairs || v.nseg > TBX_UNIT_SEGS ||
-                v.nspair > ntask || (fwd && h.nel > 2 * t.fwd_unit_tasks))
+                v.nspair > ntask || (fwd && (h.nel & TBX_NEL_MASK) > 2 * t.fwd_unit_tasks))
                 ++s.bad_units;
             bool live = false;
             for (int p = 0; p < v.npair; ++p) {
@@ -1528,7 +1543,7 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
             const int m = h.mol;
             const long long nao = hm.mol_nao[m];
             const bool block_ok = h.dense0 == hm.mol_dense[m] && h.nao == nao;
-            for (int e = 0; e < h.nel; ++e) {
+            for (int e = 0; e < (h.nel & TBX_NEL_MASK); ++e) {
                 ++s.nelem;
                 const int dsc = ed[e];
                 const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31,

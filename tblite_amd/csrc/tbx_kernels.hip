@@ -562,6 +562,7 @@ struct FwdArgs {
     double *S, *D, *Q, *H;
     int accumulate;      // 1: add into zeroed outputs (lattice images), 0: plain stores
     int chunk;           // primitive pairs per work item
+    const int *stale;    // nonzero: the pair-order hint no longer holds (far units run whole)
 };
 
 // 1D factors V[m] = int (y+pa)^(a+m) (y+pb)^b, m = 0..2 (moments about A)
@@ -648,6 +649,62 @@ __device__ __forceinline__ void zero_elements(const FwdArgs &A, const FwdStage &
     }
 }
 
+// A forward unit none of whose pairs is within the cutoff, known from the
+// pair-order hint (TBX_UNIT_FAR) while every atom stays within skin/2 of it:
+// its blocks are zero.  They are the segments' rectangles, block A (rows of
+// the atoms j, columns of atom i) and the mirror (rows of atom i, columns of
+// the atoms j < i), so only the segment words of the plan are read.  off:
+// LDS of 2 * TBX_UNIT_SEGS + 1 ints.
+__device__ __forceinline__ void far_zero_unit(const FwdArgs &A, const TbxUnitHead &H, FwdStage &U,
+                                              int *off)
+{
+    const int tid = threadIdx.x;
+    const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255;
+    const int4 *src = reinterpret_cast<const int4 *>(A.mol.fplan) + H.plan +
+                      ((10 * npair + 3) >> 2) + ((npair + 3) >> 2);
+    if (tid < nseg) U.sinfo[tid] = src[tid];
+    __syncthreads();
+    if (tid == 0) {  // element offsets of block A and mirror of each segment
+        int n = 0;
+        for (int k = 0; k < nseg; ++k) {
+            const int4 g = U.sinfo[k];
+            off[2 * k] = n;
+            n += g.w * (g.z & 0xffff);
+            off[2 * k + 1] = n;
+            n += g.w * (g.z >> 16);
+        }
+        off[2 * nseg] = n;
+    }
+    __syncthreads();
+    const int total = off[2 * nseg];
+    const long long dense0 = H.dense0;
+    const int nao = H.nao;
+    const double2 z = make_double2(0.0, 0.0);
+    for (int e = tid; e < total; e += NTHREADS) {
+        const int k2 = find_last_le(off, 2 * nseg, e);
+        const int4 g = U.sinfo[k2 >> 1];  // aoi, aoj0, rows | mrows << 16, naoi
+        const int local = e - off[k2];
+        long long idx;
+        if ((k2 & 1) == 0) {  // block A, column-major: consecutive rows
+            const int rows = g.z & 0xffff;
+            const int c = local / rows;
+            idx = dense0 + (long long)(g.x + c) * nao + g.y + (local - c * rows);
+        } else {  // mirror: rows of atom i, runs of naoi
+            const int c = local / g.w;
+            idx = dense0 + (long long)(g.y + c) * nao + g.x + (local - c * g.w);
+        }
+        A.S[idx] = 0.0;
+        A.H[idx] = 0.0;
+        const bool odd = (idx & 1) != 0;
+        *reinterpret_cast<double2 *>(A.D + 3 * idx + (odd ? 1 : 0)) = z;
+        A.D[3 * idx + (odd ? 0 : 2)] = 0.0;
+        double2 *q2 = reinterpret_cast<double2 *>(A.Q + 6 * idx);
+        q2[0] = z;
+        q2[1] = z;
+        q2[2] = z;
+    }
+}
+
 // One workgroup per forward unit.  (1) stage pairs and shell pairs; (2) one
 // lane per Cartesian-pair task counts its kept primitive pairs, the first
 // lane of each shell pair also forms its H0 factor; (3) chunks of kept pairs
@@ -668,16 +725,21 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
     const TbxUnitHead H = mt.funit[xcd_unit(blockIdx.x, gridDim.x)];
+    const bool accum = A.accumulate != 0;
+    if ((H.nel & TBX_UNIT_FAR) && *A.stale == 0) {
+        // outputs with lattice images are zero-filled before the launch
+        if (!accum) far_zero_unit(A, H, U, coff);
+        return;
+    }
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
     const int nao = H.nao;
     const long long dense0 = H.dense0;
-    const bool accum = A.accumulate != 0;
     const int chunk = A.chunk;
     for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
     // (1) the unit's plan; this lane's first element descriptor (used last)
     load_unit(H, mt.fplan, U);
     const int *edesc = mt.fplan + 4 * ((long long)H.plan + plan_words(H));
-    const int nel = H.nel;
+    const int nel = H.nel & TBX_NEL_MASK;
     // nel <= 2 * TBX_UNIT_TASKS: at most two elements per lane
     const int d0 = tid < nel ? edesc[tid] : 0;
     const int d1 = tid + NTHREADS < nel ? edesc[tid + NTHREADS] : 0;
@@ -1584,8 +1646,15 @@ extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mo
                                          int *flag, int chunk, hipStream_t stream)
 {
     const int accumulate = mol->ntrans > 1;
-    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate, chunk > 0 ? chunk : FWD_CHUNK_DEFAULT};
+    FwdArgs a{*sp, *mol, xyz, se, S, D, Q, H, accumulate, chunk > 0 ? chunk : FWD_CHUNK_DEFAULT,
+              flag};
     (void)hipGetLastError();  // launch errors are read below; drop stale ones
+    if (mol->nfunit_far > 0) {  // does the pair-order hint still hold for the far units?
+        if (nnl) return hipErrorInvalidValue;  // the flag serves one purpose per launch
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, flag, 0);
+        hipLaunchKernelGGL(k_hint_check, dim3((mol->nat + 255) / 256), dim3(256), 0, stream,
+                           *mol, xyz, flag);
+    }
     if (accumulate) {  // lattice images add into the same elements
         double *outs[4] = {S, H, D, Q};
         const long long n[4] = {ndense, ndense, 3 * ndense, 6 * ndense};

@@ -85,8 +85,9 @@ struct TbxPair {
 // order, each section padded to whole words:
 //   pairs      npair TbxPair (40 bytes each)
 //   pext       npair ints: lattice vector << 5 | segment
-//   segments   nseg int4: first pair, first row AO, rows | mirrored rows << 16,
-//              AOs of the row atom i
+//   segments   nseg int4: first AO of the row atom i (the block's column), first AO
+//              of the segment's atoms j (its rows), rows | mirrored rows << 16,
+//              AOs of atom i (all molecule-local)
 //   shell pairs nspair ints: pair | jsh << 8 | ish << 12 | type << 16, type-major
 //              (type = 3 l_i + l_j)
 //   offsets    toff[10] then tsp[10]: first task / shell pair of each type
@@ -102,8 +103,12 @@ struct TbxUnitHead {
     int counts;        // npair | nseg << 8 | nspair << 16
     int nao, ao_m;     // AOs of the molecule, its first AO (global)
     int mol;
-    int nel;           // dense elements written (forward units)
+    int nel;           // dense elements written (forward units) | TBX_UNIT_FAR
 };
+// TbxUnitHead::nel flag of a forward unit none of whose pairs lies within
+// cutoff + skin at the pair-order hint; TBX_NEL_MASK gives the count
+#define TBX_UNIT_FAR (1 << 30)
+#define TBX_NEL_MASK 0xffff
 
 // Molecule / atom tables of a batch (device pointers).  "local" indices
 // count from the start of the atom's molecule.
@@ -134,6 +139,7 @@ struct TbxMols {
     const int *plan;               // their plans (16-byte words)
     int nfunit;
     const TbxUnitHead *funit;      // [nfunit] forward units (j <= i, on-site pairs included)
+    int nfunit_far;                // forward units flagged TBX_UNIT_FAR
     const int *fplan;
     int nsh_tot, nao_tot;
     const int *sh_aog;             // [nsh_tot+1] first AO (global) of every shell
