@@ -79,6 +79,15 @@ typedef struct {
 /* Create a context on HIP device `device`.  Returns NULL on failure. */
 tbx_context *tbx_context_new(int device);
 void tbx_context_delete(tbx_context *ctx);
+/* Limit on the device memory the library allocates for this context and
+ * the batches created on it (bytes; < 0: no limit, the default).  A call that
+ * would need more fails with TBX_ERR_ALLOC before it changes any output or
+ * buffer, and the context stays usable; no counterpart in the reference,
+ * whose routines cannot fail (its errors go through the context,
+ * context/type.F90). */
+int tbx_context_set_memory_limit(tbx_context *ctx, long long bytes);
+/* Device memory currently held by the context and its batches (bytes). */
+long long tbx_context_memory_used(const tbx_context *ctx);
 /* Message of the last failing call on this thread ("" if none). */
 const char *tbx_last_error(void);
 /* Library version, major*10000 + minor*100 + patch. */
@@ -301,6 +310,34 @@ int tbx_batch_mulliken_atomic_multipoles(tbx_batch *b, int nspin, int mpdim,
 /* Milliseconds of the last add_vao_to_h1, add_vmp_to_h1, shell-charge and
  * atomic-multipole launches (ms[4]; -1 if not run), no reference counterpart. */
 int tbx_batch_last_scf_timing(const tbx_batch *b, double *ms);
+
+/* --- density matrices of the SCF (batch layout) ---------------------------
+ * Per molecule, coefficients and densities are dense (nao_m x nao_m,
+ * column-major, at dense_off[m]; column k of C is orbital k), orbital
+ * energies and occupations [nao] at ao_off[m].  One spin channel per call
+ * (the reference loops over spins, scf/diag.f90:62-108). */
+/* sygvd (lapack/sygvd.f90, scf/diag.f90:84): H C = S C diag(eval) for every
+ * molecule through rocSOLVER's dsygvd; d_H is overwritten with the
+ * S-orthonormal eigenvectors C, d_eval [nao] with the ascending eigenvalues;
+ * d_S is left unchanged (solved on a copy the batch holds).  Synchronises
+ * the stream; TBX_ERR_ARG names a molecule whose S is not positive
+ * definite.  rocSOLVER's own workspace is allocated by rocBLAS and is not
+ * counted against tbx_context_set_memory_limit. */
+int tbx_batch_sygvd(tbx_batch *b, double *d_H, const double *d_S, double *d_eval, void *stream);
+/* get_fermi_filling (wavefunction/fermi.f90:31-117) for every molecule:
+ * d_nel [nmol] electrons of this spin channel, kt (hartree), d_emo [nao]
+ * ascending orbital energies; d_focc [nao] and d_efermi [nmol] OVERWRITTEN
+ * (the reference's e_fermi: half the converged Fermi level). */
+int tbx_batch_fermi_filling(tbx_batch *b, const double *d_nel, double kt, const double *d_emo,
+                            double *d_focc, double *d_efermi, void *stream);
+/* get_density_matrix (scf/diag.f90:150-166): d_P = C diag(focc) C^T, and the
+ * energy-weighted form of get_wdensity (scf/diag.f90:110-143):
+ * d_W = C diag(focc eval) C^T; FP64 matrix-core (MFMA) products, d_P / d_W
+ * OVERWRITTEN. */
+int tbx_batch_get_density_matrix(tbx_batch *b, const double *d_coeff, const double *d_focc,
+                                 double *d_P, void *stream);
+int tbx_batch_get_wdensity(tbx_batch *b, const double *d_coeff, const double *d_focc,
+                           const double *d_eval, double *d_W, void *stream);
 
 /* Kernel timing of the last batched call on the context's internal events:
  * milliseconds of the forward / pair-list / gradient / adjoint kernels. */

@@ -9,6 +9,11 @@ Plain numpy restatements, loop structure kept from the reference (tblite
   get_mulliken_shell_charges      wavefunction/mulliken.f90:37-62
   get_mulliken_atomic_multipoles  wavefunction/mulliken.f90:65-98
   updown_to_magnet                wavefunction/spin.f90 (updown_to_magnet_2/3)
+  get_density_matrix              scf/diag.f90:150-166
+  get_wdensity (one spin)         scf/diag.f90:110-143
+  get_fermi_filling               wavefunction/fermi.f90:31-117
+  sygvd                           lapack/sygvd.f90 (LAPACK dsygvd, here through
+                                  scipy.linalg.eigh, which calls the same routine)
 
 Arrays are Fortran-shaped numpy arrays: sint (nao, nao), mpint (nmp, nao,
 nao), vao (nao, nspin), vmp (nmp, nat, nspin), h1 / pmat (nao, nao, nspin).
@@ -84,3 +89,61 @@ def get_mulliken_atomic_multipoles(mpmat, pmat, ao_first, nao_at):
             mpat[:, iat, spin] = -pat
     updown_to_magnet(mpat)
     return mpat
+
+
+def get_density_matrix(focc, coeff):
+    """scratch(jao, iao) = coeff(jao, iao) * focc(iao); pmat = scratch coeff^T
+    (scf/diag.f90:150-166)."""
+    scratch = coeff * focc[None, :]
+    return scratch @ coeff.T
+
+
+def get_wdensity(focc, eval_, coeff):
+    """One spin channel of get_wdensity (scf/diag.f90:110-143): the density
+    matrix of the occupations focc * eval."""
+    return get_density_matrix(focc * eval_, coeff)
+
+
+def sygvd(hmat, smat):
+    """H C = S C diag(e), C S-orthonormal, e ascending (lapack/sygvd.f90)."""
+    import scipy.linalg
+    e, c = scipy.linalg.eigh(hmat, smat, driver="gvd")
+    return e, c
+
+
+def get_fermi_filling(nel, kt, emo):
+    """Aufbau filling, then the Newton iteration of the Fermi level
+    (wavefunction/fermi.f90:31-117); returns (homo, focc, e_fermi)."""
+    n = len(emo)
+    occ = np.zeros(n)
+    homo = int(np.floor(nel))
+    occ[: min(homo, n)] = 1.0
+    if homo < n:
+        occ[homo] = np.fmod(nel, 1.0)
+    homo = homo + 1 if np.fmod(nel, 1.0) > 0.5 else homo
+    e_fermi = 0.0
+    if homo > 0:
+        ef = 0.5 * (emo[max(homo, 1) - 1] + emo[min(homo + 1, n) - 1])
+        occt = float(homo)
+        thr = np.sqrt(np.finfo(float).eps)
+        sqrttiny = np.sqrt(np.finfo(float).tiny)
+        for _ in range(200):
+            total_number = 0.0
+            total_dfermi = 0.0
+            for iao in range(n):
+                x = (emo[iao] - ef) / kt
+                fermifunct = 0.0
+                if x < 50:
+                    fermifunct = 1.0 / (np.exp(x) + 1.0)
+                    dfermifunct = np.exp(x) / (kt * (np.exp(x) + 1.0) ** 2)
+                else:
+                    dfermifunct = 0.0
+                occ[iao] = fermifunct
+                total_number += fermifunct
+                total_dfermi += dfermifunct
+            change = (occt - total_number) / total_dfermi if total_dfermi > sqrttiny else 0.0
+            ef += change
+            if abs(occt - total_number) <= thr:
+                break
+        e_fermi = 0.5 * ef
+    return homo, occ, e_fermi

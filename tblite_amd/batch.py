@@ -255,6 +255,44 @@ class Batch:
                                                   _ptr(sigma), self._stream()))
         return grad, sigma
 
+    # -- density matrices (scf/diag.f90:62-166, wavefunction/fermi.f90) -----
+    def sygvd(self, H, S, eval_out=None):
+        """Generalized eigenproblem H C = S C diag(eval) of every molecule:
+        ``H`` (dense, packed) is overwritten with C; returns ``eval`` [nao]."""
+        if eval_out is None:
+            eval_out = torch.empty(self.nao, dtype=torch.float64, device=self.torch_device)
+        L.check(L.lib().tbx_batch_sygvd(self.handle, _ptr(H), _ptr(S), _ptr(eval_out),
+                                         self._stream()))
+        return eval_out
+
+    def fermi_filling(self, nel, kt, emo, focc=None, efermi=None):
+        """``get_fermi_filling`` of one spin channel: ``nel`` [nmol] electrons,
+        ``emo`` [nao] ascending; returns (focc [nao], efermi [nmol])."""
+        dev = self.torch_device
+        if focc is None:
+            focc = torch.empty(self.nao, dtype=torch.float64, device=dev)
+        if efermi is None:
+            efermi = torch.empty(self.nmol, dtype=torch.float64, device=dev)
+        L.check(L.lib().tbx_batch_fermi_filling(self.handle, _ptr(nel), float(kt), _ptr(emo),
+                                                _ptr(focc), _ptr(efermi), self._stream()))
+        return focc, efermi
+
+    def density_matrix(self, coeff, focc, out=None):
+        """``get_density_matrix``: P = C diag(focc) C^T per molecule (packed)."""
+        if out is None:
+            out = torch.empty(self.ndense, dtype=torch.float64, device=self.torch_device)
+        L.check(L.lib().tbx_batch_get_density_matrix(self.handle, _ptr(coeff), _ptr(focc),
+                                                     _ptr(out), self._stream()))
+        return out
+
+    def wdensity(self, coeff, focc, eval_, out=None):
+        """Energy-weighted density of ``get_wdensity``: W = C diag(focc eval) C^T."""
+        if out is None:
+            out = torch.empty(self.ndense, dtype=torch.float64, device=self.torch_device)
+        L.check(L.lib().tbx_batch_get_wdensity(self.handle, _ptr(coeff), _ptr(focc), _ptr(eval_),
+                                               _ptr(out), self._stream()))
+        return out
+
     # -- SCF-loop consumers of the resident integrals ------------------------
     # (scf/potential.f90:192-255, wavefunction/mulliken.f90:37-98).  Spin
     # arrays are packed spin-outermost: P and h1 [nspin*ndense], vao

@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/tbx.h"
 #include "tbx_tables.h"
 
@@ -46,6 +48,11 @@ extern "C" hipError_t tbx_launch_scf(int what, const TbxMols *mol, int nspin, lo
                                      hipStream_t stream);
 extern "C" hipError_t tbx_launch_copy(const double *src, double *dst, long long n,
                                       hipStream_t stream);
+extern "C" hipError_t tbx_launch_density(const TbxMols *mol, const double *C, const double *focc,
+                                         const double *eval, double *P, hipStream_t stream);
+extern "C" hipError_t tbx_launch_fermi(const TbxMols *mol, const double *nel, double kt,
+                                       const double *emo, double *occ, double *efermi,
+                                       hipStream_t stream);
 extern "C" hipError_t tbx_launch_ncoord(const TbxMols *mol, const double *rcov, double ka,
                                         double kb, double rshift, double cutoff,
                                         const double *xyz, double *cn, hipStream_t stream);
@@ -77,24 +84,43 @@ int fail(int code, const char *fmt, ...)
                         "%s: %s", #expr, hipGetErrorString(e_));                        \
     } while (0)
 
-// Device buffer that grows on demand and is freed with its owner.
+// Device memory the library holds for one context and its batches, and the
+// limit a caller may set on it (tbx_context_set_memory_limit).
+struct Budget {
+    size_t limit = SIZE_MAX, used = 0;
+};
+
+// Device buffer that grows on demand and is freed with its owner.  A request
+// beyond the owner's budget fails with hipErrorOutOfMemory and leaves the
+// buffer as it was.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    Budget *budget = nullptr;
     hipError_t reserve(size_t bytes)
     {
         if (bytes <= cap) return hipSuccess;
+        const size_t want = std::max<size_t>(bytes, 256);
+        if (budget && (budget->used - cap > budget->limit || want > budget->limit - (budget->used - cap)))
+            return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
+        if (budget) budget->used -= cap;
         p = nullptr;
         cap = 0;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
-        if (e == hipSuccess) cap = bytes;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) {
+            cap = want;
+            if (budget) budget->used += cap;
+        } else {
+            p = nullptr;
+        }
         return e;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
     void release()
     {
         if (p) (void)hipFree(p);
+        if (budget) budget->used -= cap;
         p = nullptr;
         cap = 0;
     }
@@ -823,6 +849,7 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
 struct tbx_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    Budget budget;  // device memory of the context and its batches
     DevTables tab;
     DevBuf xyz, se, dsedcn, S, D, Q, H, P, W, vao, vdp, vqp, grad, dse, sigma, dEdcn, nnl, flag;
     std::vector<double> host_tmp;
@@ -880,6 +907,9 @@ struct tbx_batch {
     DevBuf flag, dse_tmp;
     hipEvent_t ev[16] = {};  // pairs (start, end): forward, -, gradient, adjoint, 4 SCF kernels
     bool timed = false;
+    // generalized eigensolver (rocSOLVER): handle, copy of S, workspace, info
+    rocblas_handle solver = nullptr;
+    DevBuf sy_b, sy_e, sy_info;
     // coordination number: covalent radii per species (device) and constants
     DevBuf rcov;
     bool cn_set = false;
@@ -906,6 +936,10 @@ tbx_context *tbx_context_new(int device)
     if (scratch_init(device) != TBX_OK) return nullptr;
     tbx_context *ctx = new tbx_context;
     ctx->device = device;
+    for (DevBuf *b : {&ctx->tab.species, &ctx->tab.mols, &ctx->xyz, &ctx->se, &ctx->dsedcn, &ctx->S,
+                      &ctx->D, &ctx->Q, &ctx->H, &ctx->P, &ctx->W, &ctx->vao, &ctx->vdp, &ctx->vqp,
+                      &ctx->grad, &ctx->dse, &ctx->sigma, &ctx->dEdcn, &ctx->nnl, &ctx->flag})
+        b->budget = &ctx->budget;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         fail(TBX_ERR_DEVICE, "stream creation failed");
         delete ctx;
@@ -926,6 +960,18 @@ void tbx_context_delete(tbx_context *ctx)
     for (DevBuf *b : all) b->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+int tbx_context_set_memory_limit(tbx_context *ctx, long long bytes)
+{
+    if (!ctx) return fail(TBX_ERR_ARG, "null context");
+    ctx->budget.limit = bytes < 0 ? SIZE_MAX : (size_t)bytes;
+    return TBX_OK;
+}
+
+long long tbx_context_memory_used(const tbx_context *ctx)
+{
+    return ctx ? (long long)ctx->budget.used : -1;
 }
 
 int tbx_get_selfenergy(int nat, const int *id, const int *ish_at, const int *nsh_id, int msh,
@@ -1318,6 +1364,9 @@ static int batch_create(tbx_context *ctx, const tbx_batch_desc *desc, const int 
     tbx_batch *b = new tbx_batch;
     b->ctx = ctx;
     b->cutoff = desc->cutoff;
+    for (DevBuf *d : {&b->tab.species, &b->tab.mols, &b->flag, &b->dse_tmp, &b->rcov, &b->sy_b,
+                      &b->sy_e, &b->sy_info})
+        d->budget = &ctx->budget;
     int st = build_species(desc->bas, desc->h0, b->hs);
     if (!st) st = build_mols(b->hs, desc->nmol, desc->mol_nat, desc->id, b->hm);
     if (!st && mol_ntrans) {
@@ -1605,6 +1654,10 @@ extern "C" void tbx_batch_delete(tbx_batch *b)
     b->flag.release();
     b->dse_tmp.release();
     b->rcov.release();
+    b->sy_b.release();
+    b->sy_e.release();
+    b->sy_info.release();
+    if (b->solver) (void)rocblas_destroy_handle(b->solver);
     for (auto &ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete b;
@@ -1785,6 +1838,75 @@ extern "C" int tbx_batch_ncoord_gradient(tbx_batch *b, const double *d_xyz,
     HIP_TRY(tbx_launch_ncoord_grad(&b->tab.mt, b->rcov.as<double>(), b->cn_ka, b->cn_kb,
                                    b->cn_rshift, b->cn_cutoff, d_xyz, d_dEdcn, d_grad, d_sigma,
                                    pick_stream(b, stream)));
+    return TBX_OK;
+}
+
+// Density matrices of the SCF (scf/diag.f90:62-166, wavefunction/fermi.f90):
+// the generalized eigenproblem per molecule (rocSOLVER dsygvd, as the
+// reference's lapack/sygvd.f90 calls LAPACK's), the Fermi filling and the
+// density products on the device.
+extern "C" int tbx_batch_sygvd(tbx_batch *b, double *d_H, const double *d_S, double *d_eval,
+                               void *stream)
+{
+    if (!b || !d_H || !d_S || !d_eval) return fail(TBX_ERR_ARG, "null argument");
+    hipStream_t s = pick_stream(b, stream);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (!b->solver && rocblas_create_handle(&b->solver) != rocblas_status_success)
+        return fail(TBX_ERR_DEVICE, "rocblas_create_handle failed");
+    if (rocblas_set_stream(b->solver, s) != rocblas_status_success)
+        return fail(TBX_ERR_DEVICE, "rocblas_set_stream failed");
+    // S is overwritten by its Cholesky factor: solve on a copy
+    HIP_TRY(b->sy_b.reserve(sizeof(double) * (size_t)b->hm.ndense));
+    HIP_TRY(b->sy_e.reserve(sizeof(double) * (size_t)std::max(1, b->hm.nao)));
+    HIP_TRY(b->sy_info.reserve(sizeof(rocblas_int) * (size_t)b->hm.nmol));
+    HIP_TRY(hipMemcpyAsync(b->sy_b.p, d_S, sizeof(double) * (size_t)b->hm.ndense,
+                           hipMemcpyDeviceToDevice, s));
+    for (int m = 0; m < b->hm.nmol; ++m) {
+        const int n = b->hm.mol_nao[m];
+        const long long d0 = b->hm.mol_dense[m];
+        const int a0 = b->hm.mol_ao[m];
+        const rocblas_status st = rocsolver_dsygvd(
+            b->solver, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n, d_H + d0,
+            n, b->sy_b.as<double>() + d0, n, d_eval + a0, b->sy_e.as<double>() + a0,
+            b->sy_info.as<rocblas_int>() + m);
+        if (st != rocblas_status_success)
+            return fail(TBX_ERR_DEVICE, "rocsolver_dsygvd (molecule %d): status %d", m, (int)st);
+    }
+    std::vector<rocblas_int> info(b->hm.nmol);
+    HIP_TRY(hipMemcpyAsync(info.data(), b->sy_info.p, sizeof(rocblas_int) * info.size(),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int m = 0; m < b->hm.nmol; ++m)
+        if (info[m] != 0)
+            return fail(TBX_ERR_ARG, "sygvd failed for molecule %d (info %d): S not positive "
+                                     "definite or no convergence", m, (int)info[m]);
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_fermi_filling(tbx_batch *b, const double *d_nel, double kt,
+                                       const double *d_emo, double *d_focc, double *d_efermi,
+                                       void *stream)
+{
+    if (!b || !d_nel || !d_emo || !d_focc || !d_efermi) return fail(TBX_ERR_ARG, "null argument");
+    if (!(kt > 0.0) || !std::isfinite(kt)) return fail(TBX_ERR_ARG, "kt %g", kt);
+    HIP_TRY(tbx_launch_fermi(&b->tab.mt, d_nel, kt, d_emo, d_focc, d_efermi,
+                             pick_stream(b, stream)));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_get_density_matrix(tbx_batch *b, const double *d_coeff,
+                                            const double *d_focc, double *d_P, void *stream)
+{
+    if (!b || !d_coeff || !d_focc || !d_P) return fail(TBX_ERR_ARG, "null argument");
+    HIP_TRY(tbx_launch_density(&b->tab.mt, d_coeff, d_focc, nullptr, d_P, pick_stream(b, stream)));
+    return TBX_OK;
+}
+
+extern "C" int tbx_batch_get_wdensity(tbx_batch *b, const double *d_coeff, const double *d_focc,
+                                      const double *d_eval, double *d_W, void *stream)
+{
+    if (!b || !d_coeff || !d_focc || !d_eval || !d_W) return fail(TBX_ERR_ARG, "null argument");
+    HIP_TRY(tbx_launch_density(&b->tab.mt, d_coeff, d_focc, d_eval, d_W, pick_stream(b, stream)));
     return TBX_OK;
 }
 

@@ -1,0 +1,73 @@
+"""Run the batched kernels alone on the 2048-molecule workload, for profilers.
+
+The workload and inputs are bench.py's (2048 molecules of 8-256 atoms,
+seed 1004, near-first pair order, default tiles); each kernel is launched
+``--reps`` times after one warm-up, with nothing else on the GPU in between,
+so that ``rocprofv3 --kernel-include-regex`` can pick them out:
+
+    python -m tblite_amd.kernel_driver [--reps 3] [--nmol 2048] [--kernels forward,adjoint,gradient]
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+import torch
+
+from . import synthetic
+from .batch import Batch
+
+
+def workload(nmol: int, seed: int = 1004):
+    mols = synthetic.molecule_batch(nmol, seed=seed, workers=8)
+    b = Batch(mols)
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    xyz = b.tensor_xyz()
+    cn = torch.rand(b.nat, generator=g, dtype=torch.float64, device=dev) * 3.0 + 0.5
+    se, dsedcn = b.selfenergy(cn)
+    se, dsedcn = se.contiguous(), dsedcn.contiguous()
+
+    def rnd(n, scale=1.0):
+        return torch.randn(n, generator=g, dtype=torch.float64, device=dev) * scale
+
+    P, W = rnd(b.ndense), rnd(b.ndense, 0.05)
+    plain = (dsedcn, P, W, rnd(b.nao, 1e-2), rnd(3 * b.nat, 1e-3), rnd(6 * b.nat, 1e-3))
+    adjoints = (rnd(b.ndense), rnd(3 * b.ndense), rnd(6 * b.ndense), rnd(b.ndense))
+    return b, xyz, se, plain, adjoints
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--nmol", type=int, default=2048)
+    ap.add_argument("--kernels", default="forward,adjoint,gradient")
+    a = ap.parse_args(argv)
+    b, xyz, se, plain, adjoints = workload(a.nmol)
+    outs = (torch.empty(b.ndense, dtype=torch.float64, device="cuda"),
+            torch.empty(3 * b.ndense, dtype=torch.float64, device="cuda"),
+            torch.empty(6 * b.ndense, dtype=torch.float64, device="cuda"),
+            torch.empty(b.ndense, dtype=torch.float64, device="cuda"))
+    run = {"forward": lambda: b.forward(xyz, se, out=outs),
+           "adjoint": lambda: b.adjoint(xyz, se, *adjoints),
+           "gradient": lambda: b.gradient(xyz, se, *plain)}
+    names = [k for k in a.kernels.split(",") if k]
+    for k in names:
+        run[k]()  # warm-up
+    torch.cuda.synchronize()
+    for k in names:
+        for _ in range(a.reps):
+            run[k]()
+            torch.cuda.synchronize()
+            t = b.last_timing()
+            ms = {"forward": t["forward_ms"], "adjoint": t["adjoint_ms"],
+                  "gradient": t["gradient_ms"]}[k]
+            print(f"{k} {ms:.3f} ms", flush=True)
+    time.sleep(0.1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -77,6 +77,8 @@ _SIGS = {
     "tbx_last_error": (ct.c_char_p, []),
     "tbx_context_new": (_P, [ct.c_int]),
     "tbx_context_delete": (None, [_P]),
+    "tbx_context_set_memory_limit": (ct.c_int, [_P, ct.c_longlong]),
+    "tbx_context_memory_used": (ct.c_longlong, [_P]),
     "tbx_get_selfenergy": (ct.c_int, [ct.c_int, _P, _P, _P, ct.c_int, _P, _P, _P, _P, _P]),
     "tbx_get_lattice_points": (ct.c_int, [_P, _P, ct.c_double, ct.POINTER(ct.c_int), _P]),
     "tbx_batch_create_periodic": (ct.c_int, [_P, ct.POINTER(BatchDesc), _P, _P, ct.POINTER(_P)]),
@@ -118,6 +120,10 @@ _SIGS = {
     "tbx_batch_plan_stats": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(Tiles), _P,
                                         ct.c_double, ct.POINTER(PlanStats)]),
     "tbx_batch_order_pairs": (ct.c_int, [_P, _P, ct.c_double]),
+    "tbx_batch_sygvd": (ct.c_int, [_P, _P, _P, _P, _P]),
+    "tbx_batch_fermi_filling": (ct.c_int, [_P, _P, ct.c_double, _P, _P, _P, _P]),
+    "tbx_batch_get_density_matrix": (ct.c_int, [_P, _P, _P, _P, _P]),
+    "tbx_batch_get_wdensity": (ct.c_int, [_P, _P, _P, _P, _P, _P]),
     "tbx_batch_set_ncoord": (ct.c_int, [_P, _P, ct.c_double, ct.c_double, ct.c_double,
                                         ct.c_double]),
     "tbx_batch_ncoord": (ct.c_int, [_P, _P, _P, _P]),

@@ -131,3 +131,53 @@ def test_identity_detects_planted_mistake():
     first, nao_at = _atom_ranges(bas)
     mpat = OS.get_mulliken_atomic_multipoles(D, P[:, :, None], first, nao_at)
     assert abs(np.sum(P * h1[:, :, 0]) - np.sum(vmp[:, :, 0] * mpat[:, :, 0])) > 1e-6
+
+
+# -- density matrices ----------------------------------------------------------
+
+def _eigen_case(n=12, seed=3):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=(n, n))
+    s = a @ a.T / n + np.eye(n)          # positive definite overlap
+    h = rng.normal(size=(n, n))
+    return 0.5 * (h + h.T), s
+
+
+def test_oracle_density_matrix_identities():
+    """For S-orthonormal C: tr(P S) = sum focc, P S C = C diag(focc), P
+    symmetric; W is the density of focc * eval; a transposed coefficient
+    matrix breaks them."""
+    h, s = _eigen_case()
+    e, c = OS.sygvd(h, s)
+    assert np.abs(c.T @ s @ c - np.eye(len(e))).max() < 1e-12
+    assert np.abs(h @ c - s @ c * e[None, :]).max() < 1e-11
+    focc = np.random.default_rng(1).uniform(0.0, 2.0, len(e))
+    p = OS.get_density_matrix(focc, c)
+    assert abs(np.trace(p @ s) - focc.sum()) < 1e-12
+    assert np.abs(p @ s @ c - c * focc[None, :]).max() < 1e-12
+    assert np.abs(p - p.T).max() < 1e-14
+    w = OS.get_wdensity(focc, e, c)
+    assert np.abs(w @ s @ c - c * (focc * e)[None, :]).max() < 1e-11
+    assert abs(np.trace(p @ h) - np.trace(w @ s)) < 1e-11  # both are sum f e
+    bad = OS.get_density_matrix(focc, c.T)
+    assert abs(np.trace(bad @ s) - focc.sum()) > 1e-3
+
+
+def test_oracle_fermi_filling():
+    """The occupations sum to the electron count; they are the Fermi function
+    of the returned level (2 e_fermi); small kt gives the aufbau filling."""
+    emo = np.sort(np.random.default_rng(4).normal(size=20))
+    kt = 300.0 * 3.166808578545117e-6
+    for nel in (7.0, 7.3, 7.6, 0.4):
+        homo, occ, ef = OS.get_fermi_filling(nel, kt, emo)
+        if homo > 0:
+            occt = float(homo)
+            assert abs(occ.sum() - occt) < np.sqrt(np.finfo(float).eps)
+            # occupations of the level before the last Newton step: within one step
+            with np.errstate(over="ignore"):  # exp overflows to inf far above the level: f = 0
+                f = 1.0 / (np.exp((emo - 2 * ef) / kt) + 1.0)
+            assert abs(f.sum() - occt) < 1e-6
+    homo, occ, _ = OS.get_fermi_filling(8.0, 1e-6, emo)
+    assert homo == 8 and np.allclose(occ[:8], 1.0) and np.allclose(occ[8:], 0.0)
+    homo, occ, ef = OS.get_fermi_filling(0.4, kt, emo)  # homo = 0: aufbau only
+    assert homo == 0 and ef == 0.0 and occ[0] == 0.4 and occ[1:].sum() == 0.0
