@@ -53,6 +53,16 @@ typedef struct {
     const double *hscale; /* [msh*msh*nid*nid] hscale(jsh, ish, jsp, isp) */
     const double *shpoly; /* [msh*nid]         shpoly(ish, isp) */
     const double *rad;    /* [nid]             atomic radii (bohr) */
+    /* Diatomic-frame scaling of the overlap in H0 (do_diat_scale with
+     * ksig, kpi, kdel, xtb/h0.f90:64-65, 345-390; integral/diat_trafo.f90):
+     * [nid*nid] each, ksig(izp, jzp) at izp + nid*jzp, or all three NULL
+     * (off, as for GFN2).  With them, off-site H0 blocks use the overlap
+     * scaled in the frame of the bond (sigma, pi, delta by ksig, kpi,
+     * kdel); every species' shells must then be s, p, d in this order
+     * (TBX_ERR_BASIS otherwise), and the gradient entry points return
+     * TBX_ERR_ARG (the reference computes no gradient on its one scaled
+     * path, the CEH guess, ceh/singlepoint.f90:166). */
+    const double *ksig, *kpi, *kdel;
 } tbx_h0par;
 
 /* Structure and lattice images (structure_type + trans, cutoff.f90:97-121). */

@@ -174,7 +174,9 @@ def _common_args(setup):
 
 
 def get_hamiltonian(setup, selfenergy):
-    """Dense Fortran-order S (nao,nao), D (3,nao,nao), Q (6,nao,nao), H (nao,nao)."""
+    """Dense Fortran-order S (nao,nao), D (3,nao,nao), Q (6,nao,nao), H (nao,nao).
+    With ``setup.spec.ksig`` set, H0 is built from the diatomic-frame scaled
+    overlap (``do_diat_scale``, xtb/h0.f90:345-390)."""
     nao = setup.bas.nao
     S = np.zeros((nao, nao), order="F")
     D = np.zeros((3, nao, nao), order="F")
@@ -182,10 +184,61 @@ def get_hamiltonian(setup, selfenergy):
     H = np.zeros((nao, nao), order="F")
     se = _f64(selfenergy)
     args, keep = _common_args(setup)
-    st = lib().orc_get_hamiltonian(*args, _p(se), _p(S), _p(D), _p(Q), _p(H))
+    spec = setup.spec
+    if spec.ksig is None:
+        st = lib().orc_get_hamiltonian(*args, _p(se), _p(S), _p(D), _p(Q), _p(H))
+    else:
+        ks = [_f64(np.asarray(k, dtype=np.float64).reshape(-1, order="F"))
+              for k in (spec.ksig, spec.kpi, spec.kdel)]
+        st = lib().orc_get_hamiltonian_diat(*args, *[_p(k) for k in ks], _p(se), _p(S), _p(D),
+                                            _p(Q), _p(H))
+    if st == -2:
+        raise ValueError("oracle: diatomic-frame scaling needs shells s, p, d in this order")
     if st != 0:
         raise ValueError("oracle: unsupported basis")
     return S, D, Q, H
+
+
+def harmtr(maxl, vec):
+    """Diatomic-frame rotation (integral/diat_trafo.f90:221-335) as a (9, 9)
+    array: rows the molecular-frame functions (s; p y, z, x; d m = -2..2),
+    columns those of the frame whose z axis is ``vec``."""
+    R = np.zeros((9, 9), order="F")
+    lib().orc_harmtr(ct.c_int(maxl), _p(_f64(vec)), _p(R))
+    return R
+
+
+def diat_trafo(vec, maxlj, maxli, ksig, kpi, kdel, block):
+    """diat_trafo (integral/diat_trafo.f90:99-156, no derivative) of a (9, 9)
+    atom-pair block (rows atom j, columns atom i); returns the scaled block."""
+    B = np.array(block, dtype=np.float64, order="F").reshape(9, 9, order="F").copy(order="F")
+    lib().orc_diat_trafo(_p(_f64(vec)), ct.c_int(maxlj), ct.c_int(maxli), ct.c_double(ksig),
+                         ct.c_double(kpi), ct.c_double(kdel), _p(B))
+    return B
+
+
+def get_overlap_diat_lat(setup, ksig, kpi, kdel, cutoff=None):
+    """get_overlap_diat_lat (integral/native/integrals.f90:473-582): S and
+    the diatomic-frame scaled S, dense, Fortran order; ksig/kpi/kdel are
+    (nid, nid) tables indexed (izp, jzp)."""
+    mol, bas, nao = setup.mol, setup.bas, setup.bas.nao
+    cut = bas.get_cutoff() if cutoff is None else cutoff
+    S = np.zeros((nao, nao), order="F")
+    Sd = np.zeros((nao, nao), order="F")
+    keep = [_i32(mol.id), _f64(mol.xyz), _f64(setup.trans), _i32(setup.spec.nsh_id),
+            _i32(bas.ish_at), _i32(bas.iao_sh), _cgto_array(setup)]
+    ks = [_f64(np.broadcast_to(np.asarray(k, dtype=np.float64), (mol.nid, mol.nid))
+               .reshape(-1, order="F")) for k in (ksig, kpi, kdel)]
+    st = lib().orc_get_overlap_diat_lat(
+        ct.c_int(mol.nat), ct.c_int(mol.nid), ct.c_int(setup.msh), ct.c_int(nao), _p(keep[0]),
+        _p(keep[1]), ct.c_int(len(setup.trans)), _p(keep[2]), ct.c_double(cut), _p(keep[3]),
+        _p(keep[4]), _p(keep[5]), _p(keep[6]), ct.c_double(bas.intcut), *[_p(k) for k in ks],
+        _p(S), _p(Sd))
+    if st == -2:
+        raise ValueError("oracle: diatomic-frame scaling needs shells s, p, d in this order")
+    if st != 0:
+        raise ValueError("oracle: unsupported basis")
+    return S, Sd
 
 
 def get_multipole_integrals_lat(setup, cutoff=None, which="multipole"):

@@ -23,6 +23,11 @@
  *   orc_get_selfenergy        xtb/h0.f90:113-217 (CN branch; GFN2 has no cn_en / q terms)
  *   orc_new_adjacency_list    adjlist.f90:96-143
  *   orc_get_hamiltonian       xtb/h0.f90:220-429 (do_diat_scale = .false. for GFN2)
+ *   orc_get_hamiltonian_diat  the same with do_diat_scale = .true. (the CEH
+ *                             parametrisation, ceh/ceh.f90:1207-1242)
+ *   orc_harmtr, orc_diat_trafo  integral/diat_trafo.f90:221-335 and :99-156
+ *                             with scale_diatomic_frame :642-689 (no gradient)
+ *   orc_get_overlap_diat_lat  integral/native/integrals.f90:473-582
  *   orc_get_hamiltonian_gradient  xtb/h0.f90:432-666
  *   orc_get_hamiltonian_adjoint   xtb/h0.f90:220-429 differentiated as in :432-666
  *                             (the adjoint of get_hamiltonian for arbitrary
@@ -602,18 +607,183 @@ int orc_new_adjacency_list(int nat, const double *xyz, int ntr, const double *tr
     return img;
 }
 
+/* --- diatomic-frame scaling (integral/diat_trafo.f90) --------------------
+ * Slots of an atom's block: 0 s; 1-3 p (y, z, x); 4-8 d (m = -2..2), as
+ * smap/sdim place the shells s, p, d of an atom (integrals.f90:56-58).
+ * Matrices are 9 x 9, column-major. */
+
+/* harmtr (diat_trafo.f90:221-335): R[row + 9 col] takes the real spherical
+ * functions of the molecular frame (rows) to those of the diatomic frame
+ * whose z axis is vec (columns), up to angular momentum maxl. */
+void orc_harmtr(int maxl, const double *vec, double *R)
+{
+#define T_(r, c) R[((r) - 1) + 9 * ((c) - 1)]
+    for (int k = 0; k < 81; ++k) R[k] = 0.0;
+    T_(1, 1) = 1.0;
+    if (maxl == 0) return;
+    const double len = sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+    const double n[3] = {vec[0] / len, vec[1] / len, vec[2] / len};
+    const double cost = n[2];
+    double sint, cosp, sinp;
+    if (fabs(cost) == 1.0) {  /* phi arbitrary on the z axis: x chosen */
+        sint = 0.0;
+        cosp = 1.0;
+        sinp = 0.0;
+    } else if (fabs(cost) == 0.0) {
+        sint = 1.0;
+        cosp = n[0];
+        sinp = n[1];
+    } else {
+        sint = sqrt(n[0] * n[0] + n[1] * n[1]);
+        cosp = n[0] / sint;
+        sinp = n[1] / sint;
+    }
+    T_(2, 2) = cosp;        T_(3, 2) = 0.0;   T_(4, 2) = -sinp;
+    T_(2, 3) = sint * sinp; T_(3, 3) = cost;  T_(4, 3) = sint * cosp;
+    T_(2, 4) = cost * sinp; T_(3, 4) = -sint; T_(4, 4) = cost * cosp;
+    if (maxl <= 1) return;
+    const double cos2t = cost * cost - sint * sint, sin2t = 2.0 * sint * cost;
+    const double cos2p = cosp * cosp - sinp * sinp, sin2p = 2.0 * sinp * cosp;
+    const double sqrt3 = sqrt(3.0);
+    T_(5, 5) = cost * cos2p;
+    T_(6, 5) = -sint * cosp;
+    T_(7, 5) = 0.0;
+    T_(8, 5) = sint * sinp;
+    T_(9, 5) = -cost * sin2p;
+    T_(5, 6) = sint * cos2p;
+    T_(6, 6) = cost * cosp;
+    T_(7, 6) = 0.0;
+    T_(8, 6) = -cost * sinp;
+    T_(9, 6) = -sint * sin2p;
+    T_(5, 7) = sqrt3 * sint * sint * sin2p * 0.5;
+    T_(6, 7) = sqrt3 * sin2t * sinp * 0.5;
+    T_(7, 7) = (3.0 * cost * cost - 1.0) * 0.5;
+    T_(8, 7) = sqrt3 * sin2t * cosp * 0.5;
+    T_(9, 7) = sqrt3 * sint * sint * cos2p * 0.5;
+    T_(5, 8) = sin2t * sin2p * 0.5;
+    T_(6, 8) = cos2t * sinp;
+    T_(7, 8) = -sqrt3 * sin2t * 0.5;
+    T_(8, 8) = cos2t * cosp;
+    T_(9, 8) = sin2t * cos2p * 0.5;
+    T_(5, 9) = (1.0 + cost * cost) * sin2p * 0.5;
+    T_(6, 9) = -sin2t * sinp * 0.5;
+    T_(7, 9) = sqrt3 * sint * sint * 0.5;
+    T_(8, 9) = -sin2t * cosp * 0.5;
+    T_(9, 9) = (1.0 + cost * cost) * cos2p * 0.5;
+#undef T_
+}
+
+/* scale_diatomic_frame (diat_trafo.f90:642-689): sigma, pi and delta
+ * elements of a diatomic-frame block M (rows: atom j up to maxlj, columns:
+ * atom i up to maxli). */
+static void scale_diat(double *M, double ksig, double kpi, double kdel, int maxlj, int maxli)
+{
+#define M_(r, c) M[((r) - 1) + 9 * ((c) - 1)]
+    M_(1, 1) *= ksig;
+    if (maxlj > 0) M_(3, 1) *= ksig;
+    if (maxli > 0) M_(1, 3) *= ksig;
+    if (maxlj > 0 && maxli > 0) {
+        M_(3, 3) *= ksig;
+        M_(4, 4) *= kpi;
+        M_(2, 2) *= kpi;
+        if (maxlj > 1) {
+            M_(7, 3) *= ksig;
+            M_(8, 4) *= kpi;
+            M_(6, 2) *= kpi;
+        }
+        if (maxli > 1) {
+            M_(3, 7) *= ksig;
+            M_(4, 8) *= kpi;
+            M_(2, 6) *= kpi;
+        }
+    }
+    if (maxlj > 1) M_(7, 1) *= ksig;
+    if (maxli > 1) M_(1, 7) *= ksig;
+    if (maxlj > 1 && maxli > 1) {
+        M_(7, 7) *= ksig;
+        M_(8, 8) *= kpi;
+        M_(6, 6) *= kpi;
+        M_(9, 9) *= kdel;
+        M_(5, 5) *= kdel;
+    }
+#undef M_
+}
+
+/* setup_diat_trafo + diat_trafo without derivatives (diat_trafo.f90:61-156):
+ * the atom-pair overlap block B (9 x 9, rows atom j, columns atom i) is
+ * taken to the diatomic frame, S' = R_j^T B R_i, scaled, and taken back,
+ * B = R_j S'' R_i^T.  maxlj / maxli: highest angular momentum of the atoms
+ * (their shells are s, p, d in this order). */
+void orc_diat_trafo(const double *vec, int maxlj, int maxli, double ksig, double kpi,
+                    double kdel, double *B)
+{
+    if (ksig == 1.0 && kpi == 1.0 && kdel == 1.0) return;
+    static const int sdim[3] = {1, 4, 9};
+    maxlj = maxlj < 0 ? 0 : (maxlj > 2 ? 2 : maxlj);
+    maxli = maxli < 0 ? 0 : (maxli > 2 ? 2 : maxli);
+    const int dj = sdim[maxlj], di = sdim[maxli];
+    if (dj == 1 && di == 1) {
+        B[0] *= ksig;
+        return;
+    }
+    double R[81], tmp[81], dm[81];
+    orc_harmtr(maxlj > maxli ? maxlj : maxli, vec, R);
+    for (int b = 0; b < di; ++b)  /* tmp = R^T B (dj x di) */
+        for (int a = 0; a < dj; ++a) {
+            double t = 0.0;
+            for (int c = 0; c < dj; ++c) t += R[c + 9 * a] * B[c + 9 * b];
+            tmp[a + 9 * b] = t;
+        }
+    for (int b = 0; b < 81; ++b) dm[b] = 0.0;
+    for (int b = 0; b < di; ++b)  /* S' = tmp R */
+        for (int a = 0; a < dj; ++a) {
+            double t = 0.0;
+            for (int d = 0; d < di; ++d) t += tmp[a + 9 * d] * R[d + 9 * b];
+            dm[a + 9 * b] = t;
+        }
+    scale_diat(dm, ksig, kpi, kdel, maxlj, maxli);
+    for (int b = 0; b < di; ++b)  /* tmp = R S'' */
+        for (int a = 0; a < dj; ++a) {
+            double t = 0.0;
+            for (int c = 0; c < dj; ++c) t += R[a + 9 * c] * dm[c + 9 * b];
+            tmp[a + 9 * b] = t;
+        }
+    for (int b = 0; b < di; ++b)  /* B = tmp R^T */
+        for (int a = 0; a < dj; ++a) {
+            double t = 0.0;
+            for (int d = 0; d < di; ++d) t += tmp[a + 9 * d] * R[b + 9 * d];
+            B[a + 9 * b] = t;
+        }
+}
+
+/* The shells of every species are s, p, d in this order (shell k has l = k),
+ * as the diatomic-frame slots assume (smap(ish - 1), h0.f90:288-294). */
+static int diat_shells_ok(int nid, int msh, const int *nsh_id, const orc_cgto *cgto)
+{
+    for (int isp = 0; isp < nid; ++isp)
+        for (int ish = 0; ish < nsh_id[isp]; ++ish)
+            if (cgto[ish + msh * isp].ang != ish) return 0;
+    return 1;
+}
+
 #define HSCALE(sys, jsh, ish, jsp, isp) \
     (sys)->hscale[(jsh) + (sys)->msh * ((ish) + (sys)->msh * ((jsp) + (sys)->nid * (isp)))]
 
-/* get_hamiltonian: xtb/h0.f90:220-429 */
-int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *id,
-                        const double *xyz, int ntr, const double *trans, const int *inl,
-                        const int *nnl, const int *nlat, const int *nltr, const int *nsh_id,
-                        const int *ish_at, const int *iao_sh, const orc_cgto *cgto, double intcut,
-                        const double *hscale, const double *shpoly, const double *rad,
-                        const double *selfenergy, double *overlap, double *dpint, double *qpint,
-                        double *hamiltonian)
+/* get_hamiltonian: xtb/h0.f90:220-429.  ksig, kpi, kdel [nid*nid] (izp +
+ * nid*jzp) switch the diatomic-frame scaling of the overlap in H0 on
+ * (do_diat_scale, :257-259, :345-390), or are NULL. */
+static int ham_impl(int nat, int nid, int msh, int nsh, int nao, const int *id,
+                    const double *xyz, int ntr, const double *trans, const int *inl,
+                    const int *nnl, const int *nlat, const int *nltr, const int *nsh_id,
+                    const int *ish_at, const int *iao_sh, const orc_cgto *cgto, double intcut,
+                    const double *hscale, const double *shpoly, const double *rad,
+                    const double *ksig, const double *kpi, const double *kdel,
+                    const double *selfenergy, double *overlap, double *dpint, double *qpint,
+                    double *hamiltonian)
 {
+    const int diat = ksig != NULL;
+    /* with the scaling, H0 is made of the scaled overlap alone */
+    const double mod_h0_fraction = diat ? 1.0 : 0.0;
     orc_system sys = {nat, nid, msh, nsh, nao, id, xyz, nsh_id, ish_at, iao_sh, cgto,
                       intcut, hscale, shpoly, rad};
     size_t n2 = (size_t)nao * nao;
@@ -624,6 +794,7 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
     for (int isp = 0; isp < nid; ++isp)
         for (int ish = 0; ish < nsh_id[isp]; ++ish)
             if (cgto[ish + msh * isp].ang > 2) return -1;
+    if (diat && !diat_shells_ok(nid, msh, nsh_id, cgto)) return -2;
 
 #define S_(r, c) overlap[(size_t)(r) + (size_t)nao * (c)]
 #define H_(r, c) hamiltonian[(size_t)(r) + (size_t)nao * (c)]
@@ -632,7 +803,7 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
 
 #pragma omp parallel for schedule(dynamic)
     for (int iat = 0; iat < nat; ++iat) {
-        double stmp[25], dtmpi[75], qtmpi[150];
+        double stmp[25], dtmpi[75], qtmpi[150], block[81];
         int izp = id[iat], is = ish_at[iat], nsi = nsh_id[izp];
         for (int img = 0; img < nnl[iat]; ++img) {
             int jat = nlat[inl[iat] + img], itr = nltr[inl[iat] + img];
@@ -643,6 +814,8 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
             }
             r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
             double rr = sqrt(sqrt(r2) / (rad[jzp] + rad[izp]));
+            if (diat)
+                for (int k = 0; k < 81; ++k) block[k] = 0.0;
             for (int ish = 0; ish < nsi; ++ish) {
                 int ii = iao_sh[is + ish];
                 const orc_cgto *ci = &cgto[ish + msh * izp];
@@ -652,7 +825,8 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
                     const orc_cgto *cj = &cgto[jsh + msh * jzp];
                     orc_multipole_cgto(cj, ci, r2, vec, intcut, stmp, dtmpi, qtmpi);
                     double shp = shpolyi * (1.0 + shpoly[jsh + msh * jzp] * rr);
-                    double hij = 0.5 * (selfenergy[is + ish] + selfenergy[js + jsh]) *
+                    double hij = 0.5 * (1.0 - mod_h0_fraction) *
+                                 (selfenergy[is + ish] + selfenergy[js + jsh]) *
                                  HSCALE(&sys, jsh, ish, jzp, izp) * shp;
                     int nj = nsph(cj->ang), ni = nsph(ci->ang);
                     for (int iao = 0; iao < ni; ++iao)
@@ -661,6 +835,8 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
                             double dtmpj[3], qtmpj[6];
                             orc_shift_operator(vec, stmp[ij], dtmpi + 3 * ij, qtmpi + 6 * ij, dtmpj,
                                                qtmpj);
+                            /* smap: shell k of the atom starts at slot k * k */
+                            if (diat) block[jsh * jsh + jao + 9 * (ish * ish + iao)] = stmp[ij];
                             S_(jj + jao, ii + iao) += stmp[ij];
                             for (int k = 0; k < 3; ++k) D_(k, jj + jao, ii + iao) += dtmpi[3 * ij + k];
                             for (int k = 0; k < 6; ++k) Q_(k, jj + jao, ii + iao) += qtmpi[6 * ij + k];
@@ -671,6 +847,27 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
                                 for (int k = 0; k < 6; ++k) Q_(k, ii + iao, jj + jao) += qtmpj[k];
                                 H_(ii + iao, jj + jao) += stmp[ij] * hij;
                             }
+                        }
+                }
+            }
+            if (!diat) continue;
+            orc_diat_trafo(vec, nsj - 1, nsi - 1, ksig[izp + nid * jzp], kpi[izp + nid * jzp],
+                           kdel[izp + nid * jzp], block);
+            for (int ish = 0; ish < nsi; ++ish) {
+                int ii = iao_sh[is + ish];
+                double shpolyi = 1.0 + shpoly[ish + msh * izp] * rr;
+                for (int jsh = 0; jsh < nsj; ++jsh) {
+                    int jj = iao_sh[js + jsh];
+                    double shp = shpolyi * (1.0 + shpoly[jsh + msh * jzp] * rr);
+                    double hij = 0.5 * mod_h0_fraction *
+                                 (selfenergy[is + ish] + selfenergy[js + jsh]) *
+                                 HSCALE(&sys, jsh, ish, jzp, izp) * shp;
+                    int nj = nsph(cgto[jsh + msh * jzp].ang), ni = nsph(cgto[ish + msh * izp].ang);
+                    for (int iao = 0; iao < ni; ++iao)
+                        for (int jao = 0; jao < nj; ++jao) {
+                            double b = block[jsh * jsh + jao + 9 * (ish * ish + iao)];
+                            H_(jj + jao, ii + iao) += b * hij;
+                            if (iat != jat) H_(ii + iao, jj + jao) += b * hij;
                         }
                 }
             }
@@ -701,6 +898,102 @@ int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *
 #undef H_
 #undef D_
 #undef Q_
+    return 0;
+}
+
+int orc_get_hamiltonian(int nat, int nid, int msh, int nsh, int nao, const int *id,
+                        const double *xyz, int ntr, const double *trans, const int *inl,
+                        const int *nnl, const int *nlat, const int *nltr, const int *nsh_id,
+                        const int *ish_at, const int *iao_sh, const orc_cgto *cgto, double intcut,
+                        const double *hscale, const double *shpoly, const double *rad,
+                        const double *selfenergy, double *overlap, double *dpint, double *qpint,
+                        double *hamiltonian)
+{
+    return ham_impl(nat, nid, msh, nsh, nao, id, xyz, ntr, trans, inl, nnl, nlat, nltr, nsh_id,
+                    ish_at, iao_sh, cgto, intcut, hscale, shpoly, rad, NULL, NULL, NULL,
+                    selfenergy, overlap, dpint, qpint, hamiltonian);
+}
+
+/* Returns -2 if a species' shells are not s, p, d in this order. */
+int orc_get_hamiltonian_diat(int nat, int nid, int msh, int nsh, int nao, const int *id,
+                             const double *xyz, int ntr, const double *trans, const int *inl,
+                             const int *nnl, const int *nlat, const int *nltr, const int *nsh_id,
+                             const int *ish_at, const int *iao_sh, const orc_cgto *cgto,
+                             double intcut, const double *hscale, const double *shpoly,
+                             const double *rad, const double *ksig, const double *kpi,
+                             const double *kdel, const double *selfenergy, double *overlap,
+                             double *dpint, double *qpint, double *hamiltonian)
+{
+    if (!ksig || !kpi || !kdel) return -3;
+    return ham_impl(nat, nid, msh, nsh, nao, id, xyz, ntr, trans, inl, nnl, nlat, nltr, nsh_id,
+                    ish_at, iao_sh, cgto, intcut, hscale, shpoly, rad, ksig, kpi, kdel,
+                    selfenergy, overlap, dpint, qpint, hamiltonian);
+}
+
+/* get_overlap_diat_lat (integrals.f90:473-582): the overlap and the overlap
+ * with every atom-pair block (r2 > tiny) scaled in its diatomic frame, full
+ * double loop over (iat, jat, itr) within the cutoff. */
+int orc_get_overlap_diat_lat(int nat, int nid, int msh, int nao, const int *id,
+                             const double *xyz, int ntr, const double *trans, double cutoff,
+                             const int *nsh_id, const int *ish_at, const int *iao_sh,
+                             const orc_cgto *cgto, double intcut, const double *ksig,
+                             const double *kpi, const double *kdel, double *overlap,
+                             double *overlap_diat)
+{
+    size_t n2 = (size_t)nao * nao;
+    memset(overlap, 0, sizeof(double) * n2);
+    memset(overlap_diat, 0, sizeof(double) * n2);
+    for (int isp = 0; isp < nid; ++isp)
+        for (int ish = 0; ish < nsh_id[isp]; ++ish)
+            if (cgto[ish + msh * isp].ang > 2) return -1;
+    if (!diat_shells_ok(nid, msh, nsh_id, cgto)) return -2;
+    const double cutoff2 = cutoff * cutoff;
+    const double tiny = 2.2250738585072014e-308;
+#pragma omp parallel for schedule(dynamic)
+    for (int iat = 0; iat < nat; ++iat) {
+        double stmp[25], dtmp[75], qtmp[150], block[81];
+        int izp = id[iat], is = ish_at[iat], nsi = nsh_id[izp];
+        for (int jat = 0; jat < nat; ++jat) {
+            int jzp = id[jat], js = ish_at[jat], nsj = nsh_id[jzp];
+            for (int itr = 0; itr < ntr; ++itr) {
+                double vec[3];
+                for (int k = 0; k < 3; ++k)
+                    vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k] - trans[3 * itr + k];
+                double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+                if (r2 > cutoff2) continue;
+                for (int k = 0; k < 81; ++k) block[k] = 0.0;
+                for (int ish = 0; ish < nsi; ++ish) {
+                    int ii = iao_sh[is + ish];
+                    const orc_cgto *ci = &cgto[ish + msh * izp];
+                    for (int jsh = 0; jsh < nsj; ++jsh) {
+                        int jj = iao_sh[js + jsh];
+                        const orc_cgto *cj = &cgto[jsh + msh * jzp];
+                        orc_multipole_cgto(cj, ci, r2, vec, intcut, stmp, dtmp, qtmp);
+                        int nj = nsph(cj->ang), ni = nsph(ci->ang);
+                        for (int iao = 0; iao < ni; ++iao)
+                            for (int jao = 0; jao < nj; ++jao) {
+                                int ij = jao + nj * iao;
+                                block[jsh * jsh + jao + 9 * (ish * ish + iao)] = stmp[ij];
+                                overlap[(size_t)(jj + jao) + (size_t)nao * (ii + iao)] += stmp[ij];
+                            }
+                    }
+                }
+                if (r2 > tiny)
+                    orc_diat_trafo(vec, nsj - 1, nsi - 1, ksig[izp + nid * jzp],
+                                   kpi[izp + nid * jzp], kdel[izp + nid * jzp], block);
+                for (int ish = 0; ish < nsi; ++ish) {
+                    int ii = iao_sh[is + ish], ni = nsph(cgto[ish + msh * izp].ang);
+                    for (int jsh = 0; jsh < nsj; ++jsh) {
+                        int jj = iao_sh[js + jsh], nj = nsph(cgto[jsh + msh * jzp].ang);
+                        for (int iao = 0; iao < ni; ++iao)
+                            for (int jao = 0; jao < nj; ++jao)
+                                overlap_diat[(size_t)(jj + jao) + (size_t)nao * (ii + iao)] +=
+                                    block[jsh * jsh + jao + 9 * (ish * ish + iao)];
+                    }
+                }
+            }
+        }
+    }
     return 0;
 }
 

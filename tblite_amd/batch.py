@@ -32,11 +32,18 @@ class _Description:
     """GFN2 species tables and molecule sizes of a batch, as the C-ABI's
     ``tbx_batch_desc`` (the arrays it points to are kept alive here)."""
 
-    def __init__(self, molecules):
+    def __init__(self, molecules, diat_scale=None):
         numbers = [np.asarray(z, dtype=np.int32) for z, _ in molecules]
         allz = np.concatenate(numbers)
         self.nums, self.ids = params.species_of(allz)
         self.spec = params.gfn2_species(self.nums)
+        if diat_scale is not None:
+            # get_diat_scale of an H0 spec (xtb/spec.f90:191-211): (ksig, kpi,
+            # kdel), each (nid, nid) over the batch's species (izp, jzp)
+            nid = len(self.nums)
+            ks = [np.asfortranarray(np.asarray(k, dtype=np.float64).reshape(nid, nid))
+                  for k in diat_scale(self.nums)]
+            self.spec.ksig, self.spec.kpi, self.spec.kdel = ks
         self.msh = int(self.spec.nsh_id.max())
         bas = params.Basis.new(self.ids, self.spec.nsh_id, self.spec.basis_cgto)
         self.cutoff = bas.get_cutoff()
@@ -49,9 +56,15 @@ class _Description:
             self.spec.rad, self.mol_nat, self.ids.astype(np.int32))]
         k = self.keep
         p = lambda a: a.ctypes.data_as(ct.c_void_p)  # noqa: E731
+        kd = [None] * 3
+        if self.spec.ksig is not None:
+            kflat = [np.ascontiguousarray(x.reshape(-1, order="F"))
+                     for x in (self.spec.ksig, self.spec.kpi, self.spec.kdel)]
+            self.keep += kflat
+            kd = [p(x) for x in kflat]
         self.basis = L.Basis(len(self.nums), self.msh, p(k[0]), p(k[1]), p(k[2]), p(k[3]),
                              p(k[4]), self.intcut)
-        self.h0 = L.H0Par(p(k[5]), p(k[6]), p(k[7]))
+        self.h0 = L.H0Par(p(k[5]), p(k[6]), p(k[7]), *kd)
         self.desc = L.BatchDesc(len(molecules), p(k[8]), p(k[9]), ct.pointer(self.basis),
                                 ct.pointer(self.h0), self.cutoff)
 
@@ -88,12 +101,18 @@ class Batch:
     or ``(periodic[3], lattice)`` with the lattice vectors as the rows of
     ``lattice`` (bohr); each periodic structure then sums over its lattice
     points within the larger of the integral and coordination-number cutoffs
-    (``structure.get_lattice_points``, cutoff.f90:97-121)."""
+    (``structure.get_lattice_points``, cutoff.f90:97-121).
+
+    ``diat_scale``: optional ``f(nums) -> (ksig, kpi, kdel)``, each (nid, nid)
+    over the batch's species ``nums`` (``get_diat_scale`` of an H0 spec,
+    xtb/spec.f90:191-211, e.g. the CEH one, ceh/ceh.f90:1207-1242): the
+    forward then builds H0 from the diatomic-frame scaled overlap
+    (xtb/h0.f90:345-390); the gradient kernels refuse such a batch."""
 
     def __init__(self, molecules, device: int = 0, order_pairs: bool = True, skin: float = 1.0,
-                 lattices=None):
+                 lattices=None, diat_scale=None):
         self.device = device
-        d = _Description(molecules)
+        d = _Description(molecules, diat_scale)
         self.nums, ids, self.spec, self.msh = d.nums, d.ids, d.spec, d.msh
         self.cutoff, self.intcut, self.mol_nat, self.ids = d.cutoff, d.intcut, d.mol_nat, d.ids
         self.xyz0 = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)

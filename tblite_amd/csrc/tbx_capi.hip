@@ -180,8 +180,15 @@ struct HostSpecies {
     std::vector<int> nsh, ang, nprim, ncart, nao, cart_sh, cart_exp, ao_sh, ao_m, sh_cart, sh_ao,
         prim_off, pp_off, ao_info, ao_desc;
     std::vector<double> alpha, coeff, shpoly, rad, hscale, prim, pp, pp_mu, onsite;
+    std::vector<double> kdiat;  // (ksig, kpi, kdel) at 3 * (isp + nsp * jsp); empty: off
+    bool diat = false;
     double intcut = 25.0;
 };
+
+// The gradient kernels do not differentiate the diatomic-frame scaling (the
+// reference's one scaled path, the CEH guess, ceh/singlepoint.f90:166, takes
+// no gradient).
+static const char kNoDiatGradient[] = "diatomic-frame scaling: no gradient of the scaled H0";
 
 static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies &hs)
 {
@@ -205,6 +212,31 @@ static int build_species(const tbx_basis *bas, const tbx_h0par *h0, HostSpecies 
     hs.shpoly.assign(h0->shpoly, h0->shpoly + nsp * msh);
     hs.rad.assign(h0->rad, h0->rad + nsp);
     hs.hscale.assign(h0->hscale, h0->hscale + (size_t)msh * msh * nsp * nsp);
+    // diatomic-frame scaling (xtb/h0.f90:64-65, 107-108): all three tables or none
+    const int nk = (h0->ksig != nullptr) + (h0->kpi != nullptr) + (h0->kdel != nullptr);
+    if (nk != 0 && nk != 3) return fail(TBX_ERR_ARG, "ksig, kpi and kdel must be given together");
+    hs.diat = nk == 3;
+    hs.kdiat.clear();
+    if (hs.diat) {
+        hs.kdiat.resize((size_t)3 * nsp * nsp);
+        for (size_t k = 0; k < (size_t)nsp * nsp; ++k) {
+            const double v[3] = {h0->ksig[k], h0->kpi[k], h0->kdel[k]};
+            for (int c = 0; c < 3; ++c) {
+                if (!std::isfinite(v[c]))
+                    return fail(TBX_ERR_ARG, "non-finite diatomic scaling factor (pair %zu)", k);
+                hs.kdiat[3 * k + c] = v[c];
+            }
+        }
+        // the frame slots of an atom are its shells s, p, d in this order
+        // (smap(ish - 1), xtb/h0.f90:288-294)
+        for (int isp = 0; isp < nsp; ++isp)
+            for (int ish = 0; ish < std::min(bas->nsh_id[isp], msh); ++ish)
+                if (bas->ang[ish + msh * isp] != ish)
+                    return fail(TBX_ERR_BASIS,
+                                "diatomic-frame scaling needs the shells of every species in the "
+                                "order s, p, d (species %d shell %d has l=%d)",
+                                isp, ish, bas->ang[ish + msh * isp]);
+    }
     hs.ncart.assign(nsp, 0);
     hs.nao.assign(nsp, 0);
     hs.cart_sh.assign((size_t)nsp * TBX_MAXCART, 0);
@@ -747,7 +779,8 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
                  o_sh_cart = a.add(hs.sh_cart), o_sh_ao = a.add(hs.sh_ao), o_prim = a.add(hs.prim),
                  o_prim_off = a.add(hs.prim_off), o_pp = a.add(hs.pp), o_pp_off = a.add(hs.pp_off),
                  o_pp_mu = a.add(hs.pp_mu), o_ao_info = a.add(hs.ao_info),
-                 o_ao_desc = a.add(hs.ao_desc), o_onsite = a.add(hs.onsite);
+                 o_ao_desc = a.add(hs.ao_desc), o_onsite = a.add(hs.onsite),
+                 o_kdiat = hs.diat ? a.add(hs.kdiat) : 0;
     const hipError_t e = upload_arena(d.species, a, s);
     if (e != hipSuccess) return e;
     const char *base = static_cast<const char *>(d.species.p);
@@ -781,6 +814,7 @@ static hipError_t upload_species(DevTables &d, const HostSpecies &hs, double cut
     sp.onsite = reinterpret_cast<const double *>(at(o_onsite));
     sp.ao_info = reinterpret_cast<const int *>(at(o_ao_info));
     sp.ao_desc = reinterpret_cast<const int *>(at(o_ao_desc));
+    sp.kdiat = hs.diat ? reinterpret_cast<const double *>(at(o_kdiat)) : nullptr;
     sp.intcut = hs.intcut;
     sp.cutoff2 = cutoff * cutoff;
     return hipSuccess;
@@ -894,6 +928,14 @@ static std::vector<char> species_bytes(const tbx_basis *bas, const tbx_h0par *h0
     put(h0->hscale, sizeof(double) * msh * msh * nsp * nsp);
     put(h0->shpoly, sizeof(double) * msh * nsp);
     put(h0->rad, sizeof(double) * nsp);
+    const int diat = (h0->ksig && h0->kpi && h0->kdel) ? 1 : 0;
+    if (!diat && (h0->ksig || h0->kpi || h0->kdel)) return std::vector<char>();
+    put(&diat, sizeof(int));
+    if (diat) {
+        put(h0->ksig, sizeof(double) * nsp * nsp);
+        put(h0->kpi, sizeof(double) * nsp * nsp);
+        put(h0->kdel, sizeof(double) * nsp * nsp);
+    }
     return r;
 }
 
@@ -1245,6 +1287,7 @@ extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *ge
 {
     int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao);
     if (st) return st;
+    if (ctx->hs.diat) return fail(TBX_ERR_ARG, kNoDiatGradient);
     const HostMols &hm = ctx->hm;
     if (nspin < 1 || nspin > 2) return fail(TBX_ERR_ARG, "nspin=%d", nspin);
     if (!selfenergy || !dsedcn || !vao || !vdp || !vqp || !pmat || !xmat || !dEdcn || !gradient ||
@@ -1309,6 +1352,7 @@ extern "C" int tbx_get_hamiltonian_adjoint(tbx_context *ctx, const tbx_geom *geo
         return fail(TBX_ERR_ARG, "null argument");
     int st = prepare_single(ctx, geom, list, bas, h0, ish_at, iao_sh, nao);
     if (st) return st;
+    if (ctx->hs.diat) return fail(TBX_ERR_ARG, kNoDiatGradient);
     const HostMols &hm = ctx->hm;
     hipStream_t s = ctx->stream;
     const int nat = geom->nat;
@@ -1730,6 +1774,7 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
     if (!b || !d_xyz || !d_se || !d_dsedcn || !d_P || !d_W || !d_vao || !d_vdp || !d_vqp ||
         !d_dEdcn || !d_grad || !d_sigma)
         return fail(TBX_ERR_ARG, "null argument");
+    if (b->hs.diat) return fail(TBX_ERR_ARG, kNoDiatGradient);
     hipStream_t s = pick_stream(b, stream);
     HIP_TRY(mark(b, 4, s));
     HIP_TRY(tbx_launch_fill(d_grad, 3LL * b->hm.nat, 0.0, s));
@@ -1758,6 +1803,7 @@ extern "C" int tbx_batch_gradient_noalloc(tbx_batch *b, const double *d_xyz, con
     if (!b || !d_xyz || !d_se || !d_dsedcn || !d_P || !d_W || !d_vao || !d_vdp || !d_vqp ||
         !d_dEdcn || !d_grad || !d_sigma)
         return fail(TBX_ERR_ARG, "null argument");
+    if (b->hs.diat) return fail(TBX_ERR_ARG, kNoDiatGradient);
     if (b->hm.nsh > TBX_NOALLOC_MAX_SHELLS)
         return fail(TBX_ERR_ARG,
                     "tbx_batch_gradient_noalloc: %d shells exceed the static scratch of %d; "
@@ -1784,6 +1830,7 @@ extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double
 {
     if (!b || !d_xyz || !d_se || !d_Sbar || !d_Dbar || !d_Qbar || !d_Hbar || !d_dxyz || !d_dse)
         return fail(TBX_ERR_ARG, "null argument");
+    if (b->hs.diat) return fail(TBX_ERR_ARG, kNoDiatGradient);
     hipStream_t s = pick_stream(b, stream);
     HIP_TRY(mark(b, 6, s));
     HIP_TRY(tbx_launch_fill(d_dxyz, 3LL * b->hm.nat, 0.0, s));

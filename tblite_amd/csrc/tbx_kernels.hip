@@ -705,6 +705,121 @@ __device__ __forceinline__ void far_zero_unit(const FwdArgs &A, const TbxUnitHea
     }
 }
 
+// --- diatomic-frame scaling of the overlap in H0 (integral/diat_trafo.f90)
+// The bond vector's polar angles with the reference's conventions (harmtr,
+// :241-262): on the z axis phi is taken as 0, in the xy plane sin(theta) = 1.
+struct DiatFrame {
+    double ct, st, cp, sp;
+};
+__device__ __forceinline__ DiatFrame diat_frame(const double v[3])
+{
+    const double len = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double n0 = v[0] / len, n1 = v[1] / len, ct = v[2] / len;
+    DiatFrame F;
+    F.ct = ct;
+    if (fabs(ct) == 1.0) {
+        F.st = 0.0;
+        F.cp = 1.0;
+        F.sp = 0.0;
+    } else if (fabs(ct) == 0.0) {
+        F.st = 1.0;
+        F.cp = n0;
+        F.sp = n1;
+    } else {
+        F.st = sqrt(n0 * n0 + n1 * n1);
+        F.cp = n0 / F.st;
+        F.sp = n1 / F.st;
+    }
+    return F;
+}
+
+// Rotation of shell l to the diatomic frame (harmtr, :264-335): entry (r, c),
+// r a function of the molecular frame, c one of the bond frame (both 0..2l,
+// p in y, z, x order, d in m = -2..2 order); the bond axis is c = l.
+__device__ double diat_rot(const DiatFrame &F, int l, int r, int c)
+{
+    if (l == 0) return 1.0;
+    const double ct = F.ct, st = F.st, cp = F.cp, sp = F.sp;
+    if (l == 1) {
+        switch (r + 3 * c) {
+        case 0: return cp;
+        case 1: return 0.0;
+        case 2: return -sp;
+        case 3: return st * sp;
+        case 4: return ct;
+        case 5: return st * cp;
+        case 6: return ct * sp;
+        case 7: return -st;
+        default: return ct * cp;
+        }
+    }
+    const double c2t = ct * ct - st * st, s2t = 2.0 * st * ct;
+    const double c2p = cp * cp - sp * sp, s2p = 2.0 * sp * cp;
+    switch (r + 5 * c) {
+    case 0: return ct * c2p;
+    case 1: return -st * cp;
+    case 2: return 0.0;
+    case 3: return st * sp;
+    case 4: return -ct * s2p;
+    case 5: return st * c2p;
+    case 6: return ct * cp;
+    case 7: return 0.0;
+    case 8: return -ct * sp;
+    case 9: return -st * s2p;
+    case 10: return kS3 * st * st * s2p * 0.5;
+    case 11: return kS3 * s2t * sp * 0.5;
+    case 12: return (3.0 * ct * ct - 1.0) * 0.5;
+    case 13: return kS3 * s2t * cp * 0.5;
+    case 14: return kS3 * st * st * c2p * 0.5;
+    case 15: return s2t * s2p * 0.5;
+    case 16: return c2t * sp;
+    case 17: return -kS3 * s2t * 0.5;
+    case 18: return c2t * cp;
+    case 19: return s2t * c2p * 0.5;
+    case 20: return (1.0 + ct * ct) * s2p * 0.5;
+    case 21: return -s2t * sp * 0.5;
+    case 22: return kS3 * st * st * 0.5;
+    case 23: return -s2t * cp * 0.5;
+    default: return (1.0 + ct * ct) * c2p * 0.5;
+    }
+}
+
+// Element (mj, mi) of the scaled overlap of a shell pair (lj on atom j, li on
+// atom i) at bond vector v: in the bond frame T = R_j^T S R_i, the elements
+// of equal m (m = -L..L, L = min(lj, li), at T[lj + m][li + m]) are scaled by
+// ksig, kpi or kdel for |m| = 0, 1, 2 (scale_diatomic_frame, :642-689), and
+// the block is taken back, R_j T' R_i^T (diat_trafo, :99-156).  The others
+// are unchanged and R is orthogonal, so the element is sv plus the rotated
+// changes.  S(c, d) of the shell pair comes from its Cartesian tasks
+// (task0 first, row-major by the Cartesian function of atom i).
+__device__ double diat_element(const double (*acc)[10], int task0, int ncj, int lj, int li,
+                               int mj, int mi, const double v[3], const double *k3, double sv)
+{
+    if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0.0)) return sv;
+    const DiatFrame F = diat_frame(v);
+    const int L = min(lj, li);
+    double T[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // T[m + 2]
+    for (int c = 0; c < 2 * lj + 1; ++c)
+        for (int d = 0; d < 2 * li + 1; ++d) {
+            double sc = 0.0;
+            for (int a = 0; a < sph_n(lj, c); ++a)
+                for (int b = 0; b < sph_n(li, d); ++b)
+                    sc += sph_w(lj, c, a) * sph_w(li, d, b) *
+                          acc[task0 + sph_c(li, d, b) * ncj + sph_c(lj, c, a)][0];
+#pragma unroll
+            for (int m = -2; m <= 2; ++m)
+                if (m >= -L && m <= L)
+                    T[m + 2] += diat_rot(F, lj, c, lj + m) * sc * diat_rot(F, li, d, li + m);
+        }
+    double out = sv;
+#pragma unroll
+    for (int m = -2; m <= 2; ++m)
+        if (m >= -L && m <= L)
+            out += (k3[m < 0 ? -m : m] - 1.0) * T[m + 2] * diat_rot(F, lj, mj, lj + m) *
+                   diat_rot(F, li, mi, li + m);
+    return out;
+}
+
 // One workgroup per forward unit.  (1) stage pairs and shell pairs; (2) one
 // lane per Cartesian-pair task counts its kept primitive pairs, the first
 // lane of each shell pair also forms its H0 factor; (3) chunks of kept pairs
@@ -712,7 +827,9 @@ __device__ __forceinline__ void far_zero_unit(const FwdArgs &A, const TbxUnitHea
 // about i in LDS; (5) one lane per dense element applies the spherical
 // transform and writes the block (rows j, column i: operator on i) and its
 // mirror (rows i, column j: operator shifted to j, xtb/h0.f90:704-747) --
-// every element of the molecule's matrices is written exactly once.
+// every element of the molecule's matrices is written exactly once.  DIAT:
+// off-site H0 blocks from the diatomic-frame scaled overlap (sp.kdiat).
+template <bool DIAT>
 __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 {
     __shared__ FwdStage U;
@@ -892,16 +1009,22 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
                 qv[5] += 1.5 * sq[5] - ts;
                 for (int c = 0; c < 3; ++c) d[c] += vv[c] * sv;
             }
+            double hv = sv;  // overlap entering H0
+            if (DIAT && spk[spx] >= 0) {
+                const double vv[3] = {U.pv[p][0], U.pv[p][1], U.pv[p][2]};
+                hv = diat_element(acc, task0, ncj, lj, li, mj, mi, vv,
+                                  sp.kdiat + 3 * (pr.spi + sp.nsp * pr.spj), sv);
+            }
             if (accum) {
                 put(A.S + idx, sv, true);
-                put(A.H + idx, hfac[spx] * sv, true);
+                put(A.H + idx, hfac[spx] * hv, true);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) put(A.D + 3 * idx + c, d[c], true);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) put(A.Q + 6 * idx + c, qv[c], true);
             } else {
                 A.S[idx] = sv;
-                A.H[idx] = hfac[spx] * sv;
+                A.H[idx] = hfac[spx] * hv;
                 // the 3 dipole components: one 16-byte store (aligned for either
                 // parity of idx) and one 8-byte store
                 const bool odd = (idx & 1) != 0;
@@ -1663,8 +1786,12 @@ extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mo
             if (e != hipSuccess) return e;
         }
     }
-    if (mol->nfunit > 0)
-        hipLaunchKernelGGL(k_forward, dim3(mol->nfunit), dim3(NTHREADS), 0, stream, a);
+    if (mol->nfunit > 0) {
+        if (sp->kdiat)
+            hipLaunchKernelGGL(k_forward<true>, dim3(mol->nfunit), dim3(NTHREADS), 0, stream, a);
+        else
+            hipLaunchKernelGGL(k_forward<false>, dim3(mol->nfunit), dim3(NTHREADS), 0, stream, a);
+    }
     if (nnl)
         hipLaunchKernelGGL(k_listcheck, dim3((mol->nat + 255) / 256), dim3(256), 0, stream, *sp,
                            *mol, xyz, nnl, flag);

@@ -53,6 +53,10 @@ struct TbxSpecies {
     const double *onsite; // [nsp*TBX_MAXCART*TBX_MAXCART*10]
     const int *ao_info;   // [nsp*TBX_MAXAO] shell | m << 4 | l << 8 of each spherical function
     const int *ao_desc;   // [nsp] shells (bits 0-2) | l of shell s << (3 + 2 s)
+    // diatomic-frame scaling of the overlap in H0: (ksig, kpi, kdel) of the
+    // species pair (isp of atom i, jsp of atom j) at 3 * (isp + nsp * jsp),
+    // or null (off)
+    const double *kdiat;
     double intcut;
     double cutoff2;
 };

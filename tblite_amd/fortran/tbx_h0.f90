@@ -41,6 +41,9 @@ module tbx_h0
       real(c_double), allocatable :: hscale(:, :, :, :) !< (msh, msh, nid, nid)
       real(c_double), allocatable :: shpoly(:, :)       !< (msh, nid)
       real(c_double), allocatable :: rad(:)             !< (nid), bohr
+      !> diatomic-frame scaling h0%ksig, kpi, kdel (nid, nid) (xtb/h0.f90:64-65);
+      !> not allocated: off (GFN2)
+      real(c_double), allocatable :: ksig(:, :), kpi(:, :), kdel(:, :)
       real(c_double) :: intcut = 25.0_c_double
    end type tbx_h0_tables
 
@@ -54,7 +57,7 @@ module tbx_h0
    end type basis_c
 
    type, bind(C) :: h0par_c
-      type(c_ptr) :: hscale, shpoly, rad
+      type(c_ptr) :: hscale, shpoly, rad, ksig, kpi, kdel
    end type h0par_c
 
    type, bind(C) :: geom_c
@@ -193,14 +196,16 @@ contains
    !> coeff(1:nprim) (bas%cgto(ish, isp), basis/type.f90:43-56), bas%nsh_id,
    !> bas%intcut, and h0%hscale(msh, msh, nid, nid), h0%shpoly(msh, nid),
    !> h0%rad(nid) (xtb/h0.f90:39-67).  Slots beyond nsh_id(isp) are ignored.
+   !> With h0%do_diat_scale, pass h0%ksig, h0%kpi, h0%kdel (nid, nid) too.
    subroutine tbx_h0_pack_tables(nsh_id, ang, nprim, alpha, coeff, intcut, hscale, shpoly, &
-         & rad, tab, stat)
+         & rad, tab, stat, ksig, kpi, kdel)
       integer, intent(in) :: nsh_id(:), ang(:, :), nprim(:, :)
       real(c_double), intent(in) :: alpha(:, :, :), coeff(:, :, :)
       real(c_double), intent(in) :: intcut
       real(c_double), intent(in) :: hscale(:, :, :, :), shpoly(:, :), rad(:)
       type(tbx_h0_tables), intent(out) :: tab
       integer, intent(out) :: stat
+      real(c_double), intent(in), optional :: ksig(:, :), kpi(:, :), kdel(:, :)
       integer :: nid, msh, isp, ish, np
       nid = size(nsh_id)
       msh = size(ang, 1)
@@ -234,8 +239,29 @@ contains
       tab%hscale = hscale
       tab%shpoly = shpoly
       tab%rad = rad
+      if (present(ksig) .or. present(kpi) .or. present(kdel)) then
+         if (.not. (present(ksig) .and. present(kpi) .and. present(kdel))) return
+         if (any(shape(ksig) /= [nid, nid]) .or. any(shape(kpi) /= [nid, nid]) .or. &
+            & any(shape(kdel) /= [nid, nid])) return
+         tab%ksig = ksig
+         tab%kpi = kpi
+         tab%kdel = kdel
+      end if
       stat = 0
    end subroutine tbx_h0_pack_tables
+
+   !> The C-ABI's H0 parameter record of tab (the scaling tables when allocated)
+   function h0par_of(tab) result(p)
+      type(tbx_h0_tables), intent(in), target :: tab
+      type(h0par_c) :: p
+      p = h0par_c(c_loc(tab%hscale), c_loc(tab%shpoly), c_loc(tab%rad), c_null_ptr, &
+         & c_null_ptr, c_null_ptr)
+      if (allocated(tab%ksig)) then
+         p%ksig = c_loc(tab%ksig)
+         p%kpi = c_loc(tab%kpi)
+         p%kdel = c_loc(tab%kdel)
+      end if
+   end function h0par_of
 
    !> Message of the last failing call ("" if none)
    function tbx_h0_last_error() result(msg)
@@ -318,7 +344,7 @@ contains
          type(h0par_c), intent(out) :: p
          b = basis_c(int(t%nid, c_int), int(t%msh, c_int), c_loc(t%nsh_id), c_loc(t%ang), &
             & c_loc(t%nprim), c_loc(t%alpha), c_loc(t%coeff), t%intcut)
-         p = h0par_c(c_loc(t%hscale), c_loc(t%shpoly), c_loc(t%rad))
+         p = h0par_of(t)
       end subroutine pack_tables
    end subroutine tbx_h0_get_hamiltonian
 
@@ -385,7 +411,7 @@ contains
       id0 = int(id - 1, c_int)
       bas = basis_c(int(tab%nid, c_int), int(tab%msh, c_int), c_loc(tab%nsh_id), c_loc(tab%ang), &
          & c_loc(tab%nprim), c_loc(tab%alpha), c_loc(tab%coeff), tab%intcut)
-      h0 = h0par_c(c_loc(tab%hscale), c_loc(tab%shpoly), c_loc(tab%rad))
+      h0 = h0par_of(tab)
       geom = geom_c(int(size(id), c_int), c_loc(id0), c_loc(xyz), int(size(trans, 2), c_int), &
          & c_loc(trans), cutoff)
       plist = c_null_ptr
@@ -433,7 +459,7 @@ contains
       id0 = int(id - 1, c_int)
       bas = basis_c(int(tab%nid, c_int), int(tab%msh, c_int), c_loc(tab%nsh_id), c_loc(tab%ang), &
          & c_loc(tab%nprim), c_loc(tab%alpha), c_loc(tab%coeff), tab%intcut)
-      h0 = h0par_c(c_loc(tab%hscale), c_loc(tab%shpoly), c_loc(tab%rad))
+      h0 = h0par_of(tab)
       geom = geom_c(int(size(id), c_int), c_loc(id0), c_loc(xyz), int(size(trans, 2), c_int), &
          & c_loc(trans), cutoff)
       plist = c_null_ptr

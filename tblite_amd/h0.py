@@ -42,8 +42,12 @@ class _Args:
         self.basis = L.Basis(mol.nid, msh, _p(i32(spec.nsh_id)), _p(i32(setup.ang)),
                              _p(i32(setup.nprim)), _p(f64(setup.alpha)), _p(f64(setup.coeff)),
                              bas.intcut)
+        kd = [None] * 3
+        if spec.ksig is not None:  # diatomic-frame scaling tables (nid, nid)
+            kd = [_p(f64(np.asarray(k, dtype=np.float64).reshape(-1, order="F")))
+                  for k in (spec.ksig, spec.kpi, spec.kdel)]
         self.h0 = L.H0Par(_p(f64(spec.hscale.reshape(-1, order="F"))),
-                          _p(f64(spec.shpoly.reshape(-1, order="F"))), _p(f64(spec.rad)))
+                          _p(f64(spec.shpoly.reshape(-1, order="F"))), _p(f64(spec.rad)), *kd)
         trans = f64(setup.trans)
         self.geom = L.Geom(mol.nat, _p(i32(mol.id)), _p(f64(mol.xyz)), len(setup.trans),
                            _p(trans), setup.cutoff)

@@ -33,7 +33,8 @@ class Basis(ct.Structure):
 
 
 class H0Par(ct.Structure):
-    _fields_ = [("hscale", ct.c_void_p), ("shpoly", ct.c_void_p), ("rad", ct.c_void_p)]
+    _fields_ = [("hscale", ct.c_void_p), ("shpoly", ct.c_void_p), ("rad", ct.c_void_p),
+                ("ksig", ct.c_void_p), ("kpi", ct.c_void_p), ("kdel", ct.c_void_p)]
 
 
 class Geom(ct.Structure):

@@ -231,6 +231,11 @@ class GFN2Species:
     hscale: np.ndarray       # (msh, msh, nid, nid) Fortran order (jsh, ish, jsp, isp)
     rad: np.ndarray          # (nid,), bohr
     slater: np.ndarray       # (msh, nid)
+    # diatomic-frame scaling of the overlap in H0 (h0%ksig/kpi/kdel(izp, jzp),
+    # xtb/h0.f90:64-65, 107-108): (nid, nid) each, or None (GFN2: off)
+    ksig: np.ndarray | None = None
+    kpi: np.ndarray | None = None
+    kdel: np.ndarray | None = None
 
 
 def gfn2_species(nums) -> GFN2Species:

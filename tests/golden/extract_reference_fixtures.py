@@ -4,8 +4,10 @@ unit tests for this path, as plain numbers.
 Sources (tblite 0.7.0, read as text from the reference tree):
 
 - ``test/unit/test_integral_overlap.f90``: the test basis tables of
-  ``make_basis`` (:90-210) and the literal STO-6G overlap matrices of
-  ``test_overlap_{alh3,bh3,beh2,ch4,cl2,f2,h2,lih}`` (:273-823);
+  ``make_basis`` (:90-210), the literal STO-6G overlap matrices of
+  ``test_overlap_{alh3,bh3,beh2,ch4,cl2,f2,h2,lih}`` (:273-823) and the
+  diatomic-frame scaled overlaps of ``test_overlap_diat_{alh3,cl2}``
+  (:337-399, :622-744; ksig = kpi = kdel = 1.2, :231-233);
 - ``test/unit/test_hamiltonian.f90``: the test basis tables (:60-110) and the
   literal GFN2 H0 matrices of ``test_hamiltonian_{h2,lih,s2,sih4}`` (:510-745).
 
@@ -25,6 +27,8 @@ OVERLAP_CASES = {"alh3": [13, 1, 1, 1], "bh3": [5, 1, 1, 1], "beh2": [4, 1, 1],
                  "ch4": [6, 1, 1, 1, 1], "cl2": [17, 17], "f2": [9, 9], "h2": [1, 1],
                  "lih": [3, 1]}
 HAMILTONIAN_CASES = {"h2": [1, 1], "lih": [3, 1], "s2": [16, 16], "sih4": [14, 1, 1, 1, 1]}
+# test_overlap_diat_<name>: literal variable name (CeCl3 has f functions: l > 2, left out)
+OVERLAP_DIAT_CASES = {"alh3": "overlap_diat", "cl2": "overlap"}
 
 NUM = r"[-+]?\d+\.\d+E[-+]?\d+"
 
@@ -74,6 +78,12 @@ def main(root):
         nao, vals = _literal(ov, f"test_overlap_{name}", "overlap")
         out["overlap"][name] = {"numbers": nums, "nao": nao, "fortran_order": vals,
                                 "thr": "1e5*epsilon (test_integral_overlap.f90:37)"}
+    out["overlap_diat"] = {}
+    for name, var in OVERLAP_DIAT_CASES.items():
+        nao, vals = _literal(ov, f"test_overlap_diat_{name}", var)
+        out["overlap_diat"][name] = {"numbers": OVERLAP_CASES[name], "nao": nao,
+                                     "fortran_order": vals, "ksig": 1.2, "kpi": 1.2, "kdel": 1.2,
+                                     "thr": "1e6*epsilon (thr1*10, test_integral_overlap.f90:251)"}
     for name, nums in HAMILTONIAN_CASES.items():
         nao, vals = _literal(ham, f"test_hamiltonian_{name}", "hamiltonian")
         out["hamiltonian"][name] = {"numbers": nums, "nao": nao, "fortran_order": vals,

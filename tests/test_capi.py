@@ -268,3 +268,31 @@ def test_pair_order_hint_validates_arguments():
                    hint=True)
     assert e.value.code == TBX_ERR_ARG
     assert L.lib().tbx_batch_order_pairs(None, None, 1.0) == TBX_ERR_ARG
+
+
+def test_diat_scale_tables_are_checked_on_the_host():
+    """The diatomic-frame scaling needs all three tables and species whose
+    shells are s, p, d in this order (GFN2's transition metals are d, s, p)."""
+    import ctypes as ct
+    from tblite_amd.batch import _Description
+    mols = [(np.array([8, 1, 1]), np.array([[0.0, 0, 0], [1.8, 0, 0], [0, 1.8, 0]])),
+            (np.array([17, 6]), np.array([[0.0, 0, 0], [0, 0, 3.3]]))]
+
+    def ones(nums):
+        n = len(nums)
+        return [np.ones((n, n)), np.full((n, n), 1.1), np.full((n, n), 0.9)]
+
+    def stats(desc):
+        out = L.PlanStats()
+        return L.lib().tbx_batch_plan_stats(ct.byref(desc), None, None, ct.c_double(1.0),
+                                            ct.byref(out))
+
+    d = _Description(mols, ones)
+    assert stats(d.desc) == 0
+    d.h0.kdel = None  # one table missing
+    assert stats(d.desc) == 1
+    assert b"together" in L.lib().tbx_last_error()
+    tm = mols + [(np.array([26, 1]), np.array([[0.0, 0, 0], [0, 0, 3.0]]))]
+    assert stats(_Description(tm, ones).desc) == 2
+    assert b"s, p, d" in L.lib().tbx_last_error()
+    assert stats(_Description(tm).desc) == 0  # without the scaling the order is free
