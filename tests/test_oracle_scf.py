@@ -10,7 +10,10 @@ The checks are identities the physics fixes, computed by a different route
   derivatives of the energy with respect to those potentials);
 - the charge sum sum(qsh) = sum(n0sh) - tr(P S);
 - the spin representation: equal up and down densities give the closed-shell
-  charges with zero magnetisation (updown_to_magnet, wavefunction/spin.f90).
+  charges with zero magnetisation (updown_to_magnet, wavefunction/spin.f90);
+- the molecular dipole: sum_a (mu_a + q_a R_a) of the Mulliken atomic dipoles
+  and charges equals -tr(P r) with the origin dipole integrals of the full
+  double-loop builder (an independent route to the integrals and the sum).
 """
 import numpy as np
 import pytest
@@ -181,3 +184,33 @@ def test_oracle_fermi_filling():
     assert homo == 8 and np.allclose(occ[:8], 1.0) and np.allclose(occ[8:], 0.0)
     homo, occ, ef = OS.get_fermi_filling(0.4, kt, emo)  # homo = 0: aufbau only
     assert homo == 0 and ef == 0.0 and occ[0] == 0.4 and occ[1:].sum() == 0.0
+
+
+@pytest.mark.parametrize("name", ["octane", "sulfur"])
+def test_molecular_dipole_closed_form(name):
+    """sum_a (mu_a + q_a R_a) = -tr(P r) about the origin: the atomic dipoles
+    (about each atom) and the electronic Mulliken charges of the restatement
+    against the origin dipole integrals of the full double-loop builder,
+    <j|r|i> = <j|r - R_i|i> + R_i S_ji (no half list, no shift_operator).
+    A dipole of the wrong sign, or charges placed on the wrong atoms, misses it."""
+    st, S, D, _ = _setup(name)
+    bas, nao, nat = st.bas, st.bas.nao, st.mol.nat
+    rng = np.random.default_rng(8)
+    P = _density(nao, rng)
+    S_lat, D_lat = O.get_multipole_integrals_lat(st, which="dipole")
+    assert np.abs(S_lat - S).max() < 1e-13
+    R_ao = st.mol.xyz[bas.ao2at]  # centre of the column function
+    r_origin = D_lat + R_ao.T[:, None, :] * S_lat[None, :, :]
+    mu_ref = -np.einsum("ji,kji->k", P, r_origin)
+    first, nao_at = _atom_ranges(bas)
+    mpat = OS.get_mulliken_atomic_multipoles(D, P[:, :, None], first, nao_at)[:, :, 0]
+    n0 = np.zeros(bas.nsh)
+    qsh = OS.get_mulliken_shell_charges(S, P[:, :, None], n0, bas.iao_sh, bas.nao_sh)[:, 0]
+    q_at = np.bincount(np.repeat(np.arange(nat), st.spec.nsh_id[st.mol.id]), weights=qsh,
+                       minlength=nat)
+    mu = mpat.sum(axis=1) + st.mol.xyz.T @ q_at
+    assert np.abs(mu - mu_ref).max() < 1e-11
+    assert np.abs(mu_ref).max() > 1e-2
+    # planted mistakes
+    assert np.abs(-mpat.sum(axis=1) + st.mol.xyz.T @ q_at - mu_ref).max() > 1e-3
+    assert np.abs(mpat.sum(axis=1) + st.mol.xyz.T @ np.roll(q_at, 1) - mu_ref).max() > 1e-3
