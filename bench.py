@@ -15,9 +15,12 @@ replayed as a captured HIP graph, the batched plain-gradient step, the plain
 per-molecule GPU path (one molecule per launch, inputs resident in HBM, on a
 sample of the molecules) and the drop-in host-pointer entry points (PCIe
 copies included), the roofline of the dominant kernel (HIP events on the
-launch stream) and the CPU baseline (the oracle, a C restatement of the
+launch stream), the CPU baseline (the oracle, a C restatement of the
 reference loop nest with OpenMP on the physical cores, timed in a child
-process on a bounded sample).
+process on a bounded sample), the SCF-loop consumers of the resident S / Q,
+and the path's neighbours on the same batch ("next_rows": coordination
+number and its derivative, Fermi filling, the density products, and the
+generalized eigensolver on a sample of molecules).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -407,6 +410,66 @@ def main():
                                       for k, v in scf.items()))
     del h1
 
+    # the path's neighbours on the same batch, timed with events on the
+    # launch stream (median of 5): coordination number and its derivative,
+    # Fermi filling, the density products (FP64 MFMA; algorithmic FLOP =
+    # 2 nao^3 per molecule and matrix), and the generalized eigensolver on a
+    # sample (one rocSOLVER dsygvd call per molecule)
+    def ev_ms(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t.append(e0.elapsed_time(e1))
+        return float(np.median(t))
+
+    nao_m = np.diff(b.ao_off).astype(np.float64)
+    nat_m = np.diff(b.atom_off).astype(np.float64)
+    rows = {}
+    cn_out = torch.empty(b.nat, dtype=torch.float64, device=dev)
+    t_cn = ev_ms(lambda: b.ncoord(xyz, cn_out))
+    dEdcn = torch.randn(b.nat, generator=g, dtype=torch.float64, device=dev)
+    gtmp = torch.zeros(3 * b.nat, dtype=torch.float64, device=dev)
+    stmp = torch.zeros(9 * b.nmol, dtype=torch.float64, device=dev)
+    t_dcn = ev_ms(lambda: b.ncoord_gradient(xyz, dEdcn, gtmp, stmp))
+    rows["ncoord"] = {"ms": t_cn, "atom_pairs": float((nat_m ** 2).sum()),
+                      "atom_pairs_per_s": float((nat_m ** 2).sum()) / (t_cn * 1e-3)}
+    rows["ncoord_gradient"] = {"ms": t_dcn}
+    emo, _ = torch.sort(torch.randn(b.nao, generator=g, dtype=torch.float64, device=dev))
+    for m in range(b.nmol):  # ascending within each molecule
+        a0, a1 = int(b.ao_off[m]), int(b.ao_off[m + 1])
+        emo[a0:a1] = torch.sort(emo[a0:a1])[0]
+    nel = torch.as_tensor(0.5 * nao_m, dtype=torch.float64, device=dev)
+    focc, _ = b.fermi_filling(nel, 300.0 * 3.166808578545117e-6, emo)
+    rows["fermi_filling"] = {"ms": ev_ms(lambda: b.fermi_filling(nel, 9.5e-4, emo))}
+    coeff = outs[3]  # any dense matrix serves as coefficients for the product's timing
+    Pd = torch.empty(b.ndense, dtype=torch.float64, device=dev)
+    t_p = ev_ms(lambda: b.density_matrix(coeff, focc, Pd))
+    t_w = ev_ms(lambda: b.wdensity(coeff, focc, emo, Pd))
+    gemm_flop = 2.0 * float((nao_m ** 3).sum())
+    rows["density_matrix"] = {"ms": t_p, "algorithmic_flop": gemm_flop,
+                              "TFLOP/s": gemm_flop / (t_p * 1e-3) / 1e12,
+                              "frac_fp64_peak": gemm_flop / (t_p * 1e-3) / 1e12
+                              / roofline.PEAK_FP64_TFLOPS}
+    rows["wdensity"] = {"ms": t_w, "TFLOP/s": gemm_flop / (t_w * 1e-3) / 1e12}
+    del Pd
+    nsy = min(32, b.nmol)
+    bs = Batch(mols[:nsy], order_pairs=False)
+    Ss, _, _, Hs = bs.forward(bs.tensor_xyz(), se[: int(b.sh_off[nsy])].contiguous())
+    Cs = Hs.clone()
+    t_sy = ev_ms(lambda: (Cs.copy_(Hs), bs.sygvd(Cs, Ss)), reps=3)
+    rows["sygvd_sample"] = {"ms": t_sy, "molecules": nsy,
+                            "ms_per_molecule": t_sy / nsy,
+                            "nao_range": [int(np.diff(bs.ao_off).min()),
+                                          int(np.diff(bs.ao_off).max())]}
+    del bs, Ss, Hs, Cs
+    log("rows: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in rows.items()))
+
     ms_step = el_a / args.steps * 1e3
     value = b.nmol / (el_a / args.steps)
     # dominant kernel of the autograd step.  The adjoint's derivative work is
@@ -469,6 +532,7 @@ def main():
         "graph_vs_autograd_max_abs_diff": agree_graph,
         "tiles": b.tiles(),
         "scf_consumers": scf,
+        "next_rows": rows,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

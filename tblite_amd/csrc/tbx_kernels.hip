@@ -1,25 +1,44 @@
 // tbx_kernels.hip -- gfx950 kernels of the GFN2-xTB integral / H0 path.
 //
-// Forward (get_hamiltonian, xtb/h0.f90:220-429): one workgroup owns the dense
-// column panel of one atom i (its <= 9 spherical columns over all rows of its
-// molecule).  Rows are walked in windows of consecutive atoms j.  Phase A: one
-// lane per Cartesian function pair (c_j, c_i) accumulates S, D, Q (operator on
-// i, the column atom) and the H0-weighted overlap over primitive pairs and
-// lattice images into LDS.  Phase B: one lane per spherical element (mu, nu)
-// applies the Cartesian->spherical transform (integral/trafo.f90:34-44, p in
-// y,z,x order), removes the quadrupole trace and writes S, D, Q, H0 columns
-// straight to HBM -- including the zero rows of atoms beyond the cutoff, so
-// every dense element is written exactly once with column-contiguous stores
-// and no memset pass is needed.
+// Work units.  The host cuts every molecule's half list (atom pairs i, j <= i
+// and lattice images) into work units of consecutive pairs, at most one
+// workgroup's worth each (tbx_tables.h: TbxUnitHead and its plan), and
+// packs into each unit's plan everything that does not depend on the
+// geometry: pair records, per-segment rows, the unit's shell pairs (type-
+// major) and the per-type task offsets.  One workgroup runs one unit; it
+// copies the plan to LDS with one 16-byte load per lane.  With a geometry
+// hint the pairs of a molecule closer than cutoff + skin come first and the
+// others fill units of their own (TBX_UNIT_FAR); k_hint_check tells every
+// launch whether any atom moved more than skin/2 from the hint, and while
+// none did those units are known to hold no pair inside the cutoff.
 //
-// Gradient / adjoint (get_hamiltonian_gradient, xtb/h0.f90:432-666): one
-// workgroup per atom i walks the atoms j < i (the reference's half list) in
-// windows.  One lane per Cartesian pair first folds the spherical weights of
-// its block (from P, W and the potentials, or from the adjoints of S, H0, D,
-// Q) into Cartesian weights through the transpose of the transform, then
-// recomputes the primitive integrals and their R_i derivatives and contracts
-// them on the fly -- no derivative tensor is ever stored.  Per-pair results go
-// to atom j / shells of j through LDS and then global FP64 atomics.
+// Forward, k_forward (get_hamiltonian, xtb/h0.f90:220-429): a unit's lanes
+// form the pair vectors, then one lane per shell pair counts its kept
+// primitive pairs (the sorted primitive-pair tables make them a prefix) and
+// its H0 factor; one lane per Cartesian-pair task schedules its chunks of
+// kept primitive pairs, a block scan packs the chunks into work items, and
+// each item accumulates the 10 Cartesian moments (S, D, Q about centre i) of
+// its task in LDS (on-site blocks copy the species' same-centre moments).
+// Then one lane per dense element applies the Cartesian -> spherical map
+// (integral/trafo.f90), removes the quadrupole trace and stores the block
+// (rows j, column i, operator on i) or its mirror (rows i, column j,
+// operator shifted to j, h0.f90:704-747): every element of a molecule is
+// written exactly once, so molecules need no zero-fill.  Far units only
+// store their zero blocks.  k_forward<true> builds H0 from the overlap
+// scaled in the diatomic frame of each pair (integral/diat_trafo.f90).
+//
+// Gradient and adjoint, k_gradient<MODE> (get_hamiltonian_gradient,
+// h0.f90:432-666, and the backward of the forward): one lane per task gathers
+// the spherical weights of its block (P, W and the potentials, or the
+// adjoints of S, H0, D, Q) into a Cartesian task record, with the j-centred
+// moment weights shifted to centre i; work items recompute the primitive
+// integrals and their R_i derivatives (Obara-Saika tables shared by the
+// Cartesian pairs of s/p shell pairs) and contract them with the record at
+// once -- no derivative tensor is stored.  Item results are summed across
+// the wavefront with DPP segmented scans, then per pair in LDS, and leave as
+// FP64 atomics into dE/dR, dE/dse and sigma.  Far units are not dispatched
+// while the hint holds; k_gradient_far runs them (persistent) when it does
+// not.  k_onsite adds the on-site dE/dse of the adjoint.
 //
 // All arithmetic is FP64.
 #include <hip/hip_runtime.h>
@@ -29,12 +48,9 @@
 
 #include "tbx_tables.h"
 
-#define MAXG 12
 // Workgroup sizes: one lane per Cartesian-pair task of a work unit
 #define NTHREADS TBX_UNIT_TASKS        // forward
 #define GTHREADS TBX_GRAD_UNIT_TASKS   // gradient
-#define TW_FWD 512   // Cartesian-pair tasks per forward window (LDS: 11 doubles each)
-#define TW_GRAD 256  // Cartesian-pair tasks per gradient window
 #define MAXSH 4
 
 static constexpr double kS3 = 1.7320508075688772935;
