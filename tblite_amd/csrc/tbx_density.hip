@@ -9,8 +9,10 @@
 // (v_mfma_f64_16x16x4_f64): one wavefront computes a 32 x 32 tile of one
 // molecule's P (the upper tile pairs bi <= bj of the batch's tile list) as
 // 2 x 2 blocks of 16 x 16, four k per instruction, and writes the tile and
-// its mirror.  Operands come straight from HBM (C is read column-wise, 16
-// consecutive rows per k); zero padding covers rows and k beyond nao.
+// its mirror.  Operands come straight from memory (C is read column-wise, 16
+// consecutive rows per k; the tiles of a molecule run on one XCD, so C is
+// fetched from HBM about once per molecule and re-read from L2); zero
+// padding covers rows and k beyond nao.
 #include <hip/hip_runtime.h>
 
 #include "tbx_tables.h"
@@ -27,7 +29,9 @@ __global__ void __launch_bounds__(256) k_density(TbxMols mt, const double *__res
                                                  const double *__restrict__ eval,
                                                  double *__restrict__ P)
 {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // four consecutive tiles per workgroup, workgroups XCD-contiguous: the tiles
+    // of a molecule read its C through one L2
+    const int w = tbx_xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w >= mt.ntile) return;  // whole wavefronts leave together
     const int m = mt.tiles[4 * w], bi = mt.tiles[4 * w + 1], bj = mt.tiles[4 * w + 2];

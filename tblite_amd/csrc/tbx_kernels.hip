@@ -409,17 +409,10 @@ __device__ __forceinline__ int find_task(const int *off, int c)
     return 16 * b + t - 1;
 }
 
-// Unit of workgroup b of n: the hardware deals consecutive workgroups to
-// the 8 XCDs in turn; this permutation gives each XCD a contiguous range of
-// units instead, so that neighbouring units of a molecule (the same dense
-// columns and cache lines) share one L2.
-#define TBX_NXCD 8
-__device__ __forceinline__ int xcd_unit(int b, int n)
-{
-    const int x = b % TBX_NXCD, k = b / TBX_NXCD;
-    const int per = n / TBX_NXCD, rem = n % TBX_NXCD;
-    return x * per + min(x, rem) + k;
-}
+// Unit of workgroup b of n: XCD-contiguous ranges (tbx_xcd_block), so that
+// neighbouring units of a molecule (the same dense columns and cache lines)
+// share one L2.
+__device__ __forceinline__ int xcd_unit(int b, int n) { return tbx_xcd_block(b, n); }
 
 // Unit staging shared by the forward and gradient kernels (LDS): the unit's
 // plan (TbxUnitHead) and the pair vectors, for units of at most NT tasks

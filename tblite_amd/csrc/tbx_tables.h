@@ -151,3 +151,16 @@ struct TbxMols {
     const int *tiles;              // [4*ntile] (mol, bi, bj, 0): 32 x 32 tile pairs bi <= bj
                                    // of the dense blocks
 };
+
+// Work item of workgroup b of n: the hardware deals consecutive workgroups to
+// the 8 XCDs of the MI355X in turn; this permutation gives each XCD a
+// contiguous range of items instead (XCD x takes items [x per + min(x, rem),
+// ...) with per = n / 8, rem = n % 8), so that items of one molecule share
+// one L2.  A bijection of [0, n).
+#define TBX_NXCD 8
+__device__ __forceinline__ int tbx_xcd_block(int b, int n)
+{
+    const int x = b % TBX_NXCD, k = b / TBX_NXCD;
+    const int per = n / TBX_NXCD, rem = n % TBX_NXCD;
+    return x * per + min(x, rem) + k;
+}
