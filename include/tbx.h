@@ -391,13 +391,14 @@ int tbx_batch_order_pairs(tbx_batch *b, const double *xyz, double skin);
  * entry for tests and tooling).  Checked: every atom pair (i, j <= i) of every
  * molecule is in exactly one forward unit and every (i, j < i) in exactly one
  * gradient unit; every dense element of every molecule is written by exactly
- * one forward element descriptor, inside its molecule's block, and reads only
- * its unit's own Cartesian-pair tasks; no unit exceeds the tiles' caps. */
+ * one forward element as the kernel locates it from the unit's segment words,
+ * inside its molecule's block, and reads only its unit's own Cartesian-pair
+ * tasks; no unit exceeds the tiles' caps. */
 typedef struct {
     long long nunit_fwd, nunit_grad;     /* work units */
     long long npair_fwd, npair_grad;     /* atom pairs over all units */
     long long ntask_fwd, ntask_grad;     /* Cartesian-pair tasks over all units */
-    long long nelem;                     /* forward element descriptors */
+    long long nelem;                     /* forward elements located */
     long long ndense;                    /* dense elements of the batch, sum of nao^2 */
     long long plan_bytes_fwd, plan_bytes_grad;
     long long bad_pairs;  /* pairs missing, repeated, or not of one molecule with j <= i */

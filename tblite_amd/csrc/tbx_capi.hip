@@ -576,37 +576,12 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
             return false;
         h.nel = 0;
         if (with_elems) {
-            // element descriptors: block A of every segment, then the mirrors
-            auto elem = [&](int p, int nul, int muj, bool mirror) {
-                const TbxPair &pr = pairs[p];
-                const int ish = hs.ao_sh[nul + TBX_MAXAO * pr.spi];
-                const int jsh = hs.ao_sh[muj + TBX_MAXAO * pr.spj];
-                const int li = hs.ang[ish + hs.msh * pr.spi], lj = hs.ang[jsh + hs.msh * pr.spj];
-                const int t = 3 * li + lj;
-                const int spx = spidx[(size_t)16 * p + 4 * ish + jsh];
-                const int task0 = toff[t] + (spx - tsp[t]) * ((li + 1) * (li + 2) / 2) *
-                                                ((lj + 1) * (lj + 2) / 2);
-                plan.push_back(p | (nul << 7) | (muj << 12) | (task0 << 17) |
-                               ((mirror ? 1 : 0) << 25));
-            };
-            for (int mirror = 0; mirror < 2; ++mirror)
-                for (int k = 0; k < nseg; ++k) {
-                    const int p0 = sinfo[4 * k], p1 = k + 1 < nseg ? sinfo[4 * (k + 1)] : npair;
-                    const int naoi = sinfo[4 * k + 3];
-                    if (!mirror) {
-                        for (int nul = 0; nul < naoi; ++nul)
-                            for (int p = p0; p < p1; ++p)
-                                for (int muj = 0; muj < hs.nao[pairs[p].spj]; ++muj)
-                                    elem(p, nul, muj, false);
-                    } else {
-                        for (int p = p0; p < p1 && pairs[p].j < pairs[p].i; ++p)
-                            for (int muj = 0; muj < hs.nao[pairs[p].spj]; ++muj)
-                                for (int nul = 0; nul < naoi; ++nul) elem(p, nul, muj, true);
-                    }
-                }
-            h.nel = (int)(plan.size() - 4 * ((size_t)h.plan + words));
+            // dense elements: block A and mirror of every segment (tbx_elem_offsets)
+            for (int k = 0; k < nseg; ++k) {
+                const int rows = sinfo[4 * k + 2] & 0xffff, mrows = sinfo[4 * k + 2] >> 16;
+                h.nel += sinfo[4 * k + 3] * (rows + mrows);
+            }
             if (h.nel > 2 * max_tasks) return false;  // two elements per lane
-            pad4();
         }
         if (plan.size() / 4 > (size_t)INT_MAX) return false;
     }
@@ -1632,23 +1607,37 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
             }
             if (!fwd && live) ++s.nunit_grad_live;
             if (!fwd) continue;
-            const int *ed = plan.data() + 4 * ((size_t)h.plan + v.words);
             const int m = h.mol;
             const long long nao = hm.mol_nao[m];
             const bool block_ok = h.dense0 == hm.mol_dense[m] && h.nao == nao;
+            // the kernel's element phase: offsets and the locator from the
+            // segment words, first task of each shell pair from the shell-pair list
+            const int4 *seg = reinterpret_cast<const int4 *>(v.sinfo);
+            const TbxPair *pinf = reinterpret_cast<const TbxPair *>(v.pairs);
+            std::vector<int> off(2 * v.nseg + 1);
+            tbx_elem_offsets(seg, v.nseg, off.data());
+            std::vector<int> spmap((size_t)16 * v.npair, -1);
+            for (int q = 0; q < v.nspair; ++q) {
+                const int e = v.spl[q], ty = e >> 16, li = ty / 3, lj = ty % 3;
+                const int nc = (li + 1) * (li + 2) / 2 * ((lj + 1) * (lj + 2) / 2);
+                const int pq = e & 255, jsh = (e >> 8) & 15, ish = (e >> 12) & 15;
+                if (pq < v.npair && ish < 4 && jsh < 4)
+                    spmap[16 * pq + 4 * ish + jsh] = v.toff[ty] + (q - v.tsp[ty]) * nc;
+            }
+            if (off[2 * v.nseg] != (h.nel & TBX_NEL_MASK)) ++s.bad_units;
             for (int e = 0; e < (h.nel & TBX_NEL_MASK); ++e) {
                 ++s.nelem;
-                const int dsc = ed[e];
-                const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31,
-                          task0 = (dsc >> 17) & 255;
-                const bool mirror = (dsc >> 25) & 1;
-                if (!block_ok || p >= v.npair) {
+                const TbxElem el = tbx_elem_locate(off.data(), v.nseg, seg, pinf, v.pext, v.npair, e);
+                const int p = el.p, nul = el.nul, muj = el.muj;
+                const bool mirror = el.mirror != 0;
+                if (!block_ok || p < 0 || p >= v.npair) {
                     ++s.bad_elems;
                     continue;
                 }
                 const int *pr = v.pairs + 10 * p;  // i, j, spi, spj, aoi, aoj, ...
                 const int spi = pr[2], spj = pr[3];
-                if (nul >= hs.nao[spi] || muj >= hs.nao[spj] || (mirror && pr[1] >= pr[0])) {
+                if (nul < 0 || muj < 0 || nul >= hs.nao[spi] || muj >= hs.nao[spj] ||
+                    (mirror && pr[1] >= pr[0])) {
                     ++s.bad_elems;
                     continue;
                 }
@@ -1657,8 +1646,9 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
                 const int jsh = hs.ao_sh[muj + TBX_MAXAO * spj];
                 const int li = hs.ang[ish + hs.msh * spi], lj = hs.ang[jsh + hs.msh * spj];
                 const int ty = 3 * li + lj, nc = (li + 1) * (li + 2) / 2 * ((lj + 1) * (lj + 2) / 2);
+                const int task0 = spmap[16 * p + 4 * ish + jsh];
                 const int rel = task0 - v.toff[ty], spx = v.tsp[ty] + rel / nc;
-                if (rel < 0 || rel % nc != 0 || task0 + nc > v.toff[ty + 1] ||
+                if (task0 < 0 || rel < 0 || rel % nc != 0 || task0 + nc > v.toff[ty + 1] ||
                     spx >= v.tsp[ty + 1] ||
                     v.spl[spx] != (p | (jsh << 8) | (ish << 12) | (ty << 16))) {
                     ++s.bad_elems;

@@ -631,59 +631,25 @@ __device__ __forceinline__ void put(double *p, double v, bool acc)
     if (acc) unsafeAtomicAdd(p, v); else *p = v;
 }
 
-// The elements of a forward unit whose blocks are all zero (no pair inside
-// the cutoff): the addresses and the store pattern of the element phase.
-__device__ __forceinline__ void zero_elements(const FwdArgs &A, const FwdStage &U,
-                                              long long dense0, int nao, int nel, int d0, int d1)
-{
-    for (int base = 0; base < nel; base += NTHREADS) {
-        const int e = base + threadIdx.x;
-        if (e >= nel) continue;
-        const int dsc = base == 0 ? d0 : d1;
-        const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31;
-        const bool mirror = (dsc >> 25) & 1;
-        const int nu = U.pinf[p].aoi + nul, mu = U.pinf[p].aoj + muj;
-        const long long idx = mirror ? dense0 + (long long)mu * nao + nu
-                                     : dense0 + (long long)nu * nao + mu;
-        const double2 z = make_double2(0.0, 0.0);
-        A.S[idx] = 0.0;
-        A.H[idx] = 0.0;
-        const bool odd = (idx & 1) != 0;
-        *reinterpret_cast<double2 *>(A.D + 3 * idx + (odd ? 1 : 0)) = z;
-        A.D[3 * idx + (odd ? 0 : 2)] = 0.0;
-        double2 *q2 = reinterpret_cast<double2 *>(A.Q + 6 * idx);
-        q2[0] = z;
-        q2[1] = z;
-        q2[2] = z;
-    }
-}
-
-// A forward unit none of whose pairs is within the cutoff, known from the
-// pair-order hint (TBX_UNIT_FAR) while every atom stays within skin/2 of it:
-// its blocks are zero.  They are the segments' rectangles, block A (rows of
-// the atoms j, columns of atom i) and the mirror (rows of atom i, columns of
-// the atoms j < i), so only the segment words of the plan are read.  off:
-// LDS of 2 * TBX_UNIT_SEGS + 1 ints.
+// A forward unit none of whose pairs is within the cutoff: its blocks are
+// zero.  They are the segments' rectangles, block A (rows of the atoms j,
+// columns of atom i) and the mirror (rows of atom i, columns of the atoms
+// j < i), so only the segment words of the plan are needed (read here unless
+// `staged`: a far unit, TBX_UNIT_FAR, known from the pair-order hint while
+// every atom stays within skin/2 of it, stages nothing else).  off: LDS of
+// 2 * TBX_UNIT_SEGS + 1 ints.
 __device__ __forceinline__ void far_zero_unit(const FwdArgs &A, const TbxUnitHead &H, FwdStage &U,
-                                              int *off)
+                                              int *off, bool staged)
 {
     const int tid = threadIdx.x;
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255;
-    const int4 *src = reinterpret_cast<const int4 *>(A.mol.fplan) + H.plan +
-                      ((10 * npair + 3) >> 2) + ((npair + 3) >> 2);
-    if (tid < nseg) U.sinfo[tid] = src[tid];
-    __syncthreads();
-    if (tid == 0) {  // element offsets of block A and mirror of each segment
-        int n = 0;
-        for (int k = 0; k < nseg; ++k) {
-            const int4 g = U.sinfo[k];
-            off[2 * k] = n;
-            n += g.w * (g.z & 0xffff);
-            off[2 * k + 1] = n;
-            n += g.w * (g.z >> 16);
-        }
-        off[2 * nseg] = n;
+    if (!staged) {
+        const int4 *src = reinterpret_cast<const int4 *>(A.mol.fplan) + H.plan +
+                          ((10 * npair + 3) >> 2) + ((npair + 3) >> 2);
+        if (tid < nseg) U.sinfo[tid] = src[tid];
     }
+    __syncthreads();
+    if (tid == 0) tbx_elem_offsets(U.sinfo, nseg, off);  // block A and mirror of each segment
     __syncthreads();
     const int total = off[2 * nseg];
     const long long dense0 = H.dense0;
@@ -846,7 +812,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ double hfac[TBX_UNIT_TASKS];  // H0 / S per shell pair
     __shared__ int spk[TBX_UNIT_TASKS], spp[TBX_UNIT_TASKS];  // kept pairs, first entry
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12 | pair << 20
-    __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];
+    __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];  // chunk offsets, then element offsets
+    __shared__ unsigned char spmap[TBX_UNIT_PAIRS * 16];     // first task of (pair, ish, jsh)
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -854,7 +821,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const bool accum = A.accumulate != 0;
     if ((H.nel & TBX_UNIT_FAR) && *A.stale == 0) {
         // outputs with lattice images are zero-filled before the launch
-        if (!accum) far_zero_unit(A, H, U, coff);
+        if (!accum) far_zero_unit(A, H, U, coff, false);
         return;
     }
     const int npair = H.counts & 255, nseg = (H.counts >> 8) & 255, nspair = H.counts >> 16;
@@ -862,13 +829,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     const long long dense0 = H.dense0;
     const int chunk = A.chunk;
     for (int k = tid; k < 10 * TBX_UNIT_TASKS; k += NTHREADS) (&acc[0][0])[k] = 0.0;
-    // (1) the unit's plan; this lane's first element descriptor (used last)
+    // (1) the unit's plan
     load_unit(H, mt.fplan, U);
-    const int *edesc = mt.fplan + 4 * ((long long)H.plan + plan_words(H));
-    const int nel = H.nel & TBX_NEL_MASK;
-    // nel <= 2 * TBX_UNIT_TASKS: at most two elements per lane
-    const int d0 = tid < nel ? edesc[tid] : 0;
-    const int d1 = tid + NTHREADS < nel ? edesc[tid + NTHREADS] : 0;
+    const int nel = H.nel & TBX_NEL_MASK;  // <= 2 * TBX_UNIT_TASKS: two elements per lane
     __syncthreads();
 
     // (2) pair vectors.  A unit with no pair inside the cutoff (on-site
@@ -884,7 +847,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         live = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] <= sp.cutoff2;
     }
     if (!__syncthreads_or(live)) {
-        if (!accum) zero_elements(A, U, dense0, nao, nel, d0, d1);
+        if (!accum) far_zero_unit(A, H, U, coff, true);
         return;
     }
     // shell pairs: screening counts and H0 factors (one lane per shell pair)
@@ -904,6 +867,10 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         int spx;
         const int4 d = decode_task(U, tid, spx);
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (spx << 12) | ((U.spl[spx] & 255) << 20);
+        if (d.w == 0) {  // the shell pair's first task, for the element phase
+            const int4 e = unpack_spl(U.spl[spx]);  // pair, jsh, ish, type
+            spmap[16 * e.x + 4 * e.z + e.y] = (unsigned char)tid;
+        }
         const int kept = spk[spx];
         if (kept >= 0) {
             nch = (kept + chunk - 1) / chunk;
@@ -951,23 +918,25 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
         for (int k = 0; k < 10; ++k) atomicAdd(&acc[tk][k], o[k]);
     }
 
-    // (5) one lane per dense element (descriptors from the plan): block A
-    // (rows of atoms j, the columns of atom i) and the mirror (rows of atom
-    // i, columns of the j < i); every element of the molecule's matrices is
-    // written by exactly one lane of one unit
+    // (5) one lane per dense element, located from the segment words
+    // (tbx_elem_locate): block A (rows of atoms j, the columns of atom i) and
+    // the mirror (rows of atom i, columns of the j < i); every element of the
+    // molecule's matrices is written by exactly one lane of one unit
+    __syncthreads();  // the items are done with coff
+    if (tid == 0) tbx_elem_offsets(U.sinfo, nseg, coff);
     __syncthreads();
     for (int base = 0; base < nel; base += NTHREADS) {
         const int e = base + tid;
         if (e < nel) {
-            const int dsc = base == 0 ? d0 : d1;
-            const int p = dsc & 127, nul = (dsc >> 7) & 31, muj = (dsc >> 12) & 31;
-            const int task0 = (dsc >> 17) & 255;
-            const bool mirror = (dsc >> 25) & 1;
+            const TbxElem el = tbx_elem_locate(coff, nseg, U.sinfo, U.pinf, U.pext, npair, e);
+            const int p = el.p, nul = el.nul, muj = el.muj;
+            const bool mirror = el.mirror != 0;
             const PairInfo pr = U.pinf[p];
-            const int spx = (tinfo[task0] >> 12) & 255;
             int ish, mi, li, jsh, mj, lj;
             ao_decode(pr.di, nul, ish, mi, li);
             ao_decode(pr.dj, muj, jsh, mj, lj);
+            const int task0 = spmap[16 * p + 4 * ish + jsh];
+            const int spx = (tinfo[task0] >> 12) & 255;
             const int ncj = ncart_of(lj);
             const int mu = pr.aoj + muj;
             double v[10];

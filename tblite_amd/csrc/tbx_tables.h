@@ -95,12 +95,8 @@ struct TbxPair {
 //   shell pairs nspair ints: pair | jsh << 8 | ish << 12 | type << 16, type-major
 //              (type = 3 l_i + l_j)
 //   offsets    toff[10] then tsp[10]: first task / shell pair of each type
-// Forward units add, after these sections (not staged in LDS), one int per
-// dense element the unit writes: pair | nul << 7 | muj << 12 | task0 << 17 |
-// mirror << 25, with nul / muj the AO of the element within atom i / j,
-// task0 the first Cartesian task of its shell pair and mirror set for the
-// transposed block (rows of atom i).  Block A elements come first (segment
-// by segment, column-major), then the mirrored ones (row-major).
+// A forward unit writes, segment by segment, the segment's block A and its
+// mirror (tbx_elem_locate): nel dense elements in all.
 struct TbxUnitHead {
     long long dense0;  // first dense element of the unit's molecule
     int plan;          // first 16-byte word of the plan
@@ -113,6 +109,68 @@ struct TbxUnitHead {
 // cutoff + skin at the pair-order hint; TBX_NEL_MASK gives the count
 #define TBX_UNIT_FAR (1 << 30)
 #define TBX_NEL_MASK 0xffff
+
+// The dense elements of a forward unit, segment k after segment k - 1:
+// block A (rows: the AOs of the segment's atoms j, first AO seg.y, `rows` of
+// them; columns: the seg.w AOs of atom i from seg.x; column-major) at
+// off[2k], then the mirror (rows: the AOs of atom i; columns: the first
+// `mrows` AOs of the segment, those of the atoms j < i; column-major) at
+// off[2k + 1]; off[2 nseg] is the unit's element count.
+__host__ __device__ inline void tbx_elem_offsets(const int4 *seg, int nseg, int *off)
+{
+    int n = 0;
+    for (int k = 0; k < nseg; ++k) {
+        off[2 * k] = n;
+        n += seg[k].w * (seg[k].z & 0xffff);
+        off[2 * k + 1] = n;
+        n += seg[k].w * (seg[k].z >> 16);
+    }
+    off[2 * nseg] = n;
+}
+
+// Element e of a forward unit (tbx_elem_offsets): its pair p (the unit's
+// pairs run segment by segment, pext & 31, and within a segment by their
+// first AO of atom j, so p is the last pair at or before (segment, AO)), the
+// AO nul within atom i, the AO muj within atom j, and whether it lies in the
+// mirror (rows of atom i, operator on j).
+struct TbxElem {
+    int p, nul, muj, mirror;
+};
+__host__ __device__ inline TbxElem tbx_elem_locate(const int *off, int nseg, const int4 *seg,
+                                                    const TbxPair *pinf, const int *pext,
+                                                    int npair, int e)
+{
+    int lo = 0, hi = 2 * nseg;  // last half-segment whose first element is <= e
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int k = lo >> 1;
+    const int4 g = seg[k];
+    const int local = e - off[lo];
+    TbxElem r;
+    r.mirror = lo & 1;
+    int jao;  // molecule-local AO on the atom-j side
+    if (!r.mirror) {
+        const int rows = g.z & 0xffff;
+        const int c = local / rows;
+        r.nul = c;
+        jao = g.y + (local - c * rows);
+    } else {
+        const int c = local / g.w;
+        r.nul = local - c * g.w;
+        jao = g.y + c;
+    }
+    int plo = 0, phi = npair;
+    while (phi - plo > 1) {
+        const int mid = (plo + phi) >> 1;
+        const int sm = pext[mid] & 31;
+        if (sm < k || (sm == k && pinf[mid].aoj <= jao)) plo = mid; else phi = mid;
+    }
+    r.p = plo;
+    r.muj = jao - pinf[plo].aoj;
+    return r;
+}
 
 // Molecule / atom tables of a batch (device pointers).  "local" indices
 // count from the start of the atom's molecule.
