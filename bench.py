@@ -292,13 +292,13 @@ def main():
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
         b.forward(xyz, se, out=outs)
-        b.adjoint(xyz, se, Sb, Db, Qb, Hb, out=aouts)
+        b.adjoint(xyz, se, Sb, Db, Qb, Hb, out=aouts, S=outs[0], D=outs[1])
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         b.forward(xyz, se, out=outs)
-        b.adjoint(xyz, se, Sb, Db, Qb, Hb, out=aouts)
+        b.adjoint(xyz, se, Sb, Db, Qb, Hb, out=aouts, S=outs[0], D=outs[1])
     for _ in range(args.warmup):
         graph.replay()
     torch.cuda.synchronize()

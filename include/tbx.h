@@ -271,6 +271,22 @@ int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
                       const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
                       const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream);
 
+/* tbx_batch_adjoint given also the forward's own outputs d_S (ndense) and d_D
+ * (3*ndense) at the same d_xyz and d_se, as an autograd backward holds them.
+ * The adjoint's terms that depend on the pair vector through the shift of
+ * the moments to atom j and through the H0 distance polynomial are linear in
+ * S and D, so they are contracted element by element from these instead of
+ * being recomputed per primitive pair, which makes the kernel's work items
+ * smaller.  Same outputs as tbx_batch_adjoint, to rounding.  A batch with
+ * lattice images (more than one translation per structure) takes
+ * tbx_batch_adjoint's route and leaves d_S, d_D unread.  The reference has
+ * no such routine: it serves the backward of the differentiable path. */
+int tbx_batch_adjoint_with_forward(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                   const double *d_S, const double *d_D, const double *d_Sbar,
+                                   const double *d_Dbar, const double *d_Qbar,
+                                   const double *d_Hbar, double *d_dxyz, double *d_dse,
+                                   void *stream);
+
 /* --- coordination number (device) ----------------------------------------
  * The GFN2 coordination number, mctc-lib's double-exponential counting
  * function as tblite creates it (xtb/gfn2.f90:640) and evaluates it

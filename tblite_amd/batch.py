@@ -237,16 +237,27 @@ class Batch:
                                            _ptr(dEdcn), _ptr(grad), _ptr(sigma), self._stream()))
         return dEdcn, grad, sigma
 
-    def adjoint(self, xyz, se, Sb, Db, Qb, Hb, out=None):
-        """Backward of ``forward``: dE/dxyz [3*nat] and dE/dse [nsh]."""
+    def adjoint(self, xyz, se, Sb, Db, Qb, Hb, out=None, S=None, D=None):
+        """Backward of ``forward``: dE/dxyz [3*nat] and dE/dse [nsh].  Given
+        the forward's own S and D at the same xyz and se, the terms linear in
+        them are contracted per element (``tbx_batch_adjoint_with_forward``);
+        the results agree to rounding."""
         dev = self.torch_device
         if out is None:
             out = (torch.empty(3 * self.nat, dtype=torch.float64, device=dev),
                    torch.empty(self.nsh, dtype=torch.float64, device=dev))
         dxyz, dse = out
-        L.check(L.lib().tbx_batch_adjoint(self.handle, _ptr(xyz), _ptr(se), _ptr(Sb), _ptr(Db),
-                                          _ptr(Qb), _ptr(Hb), _ptr(dxyz), _ptr(dse),
-                                          self._stream()))
+        if (S is None) != (D is None):
+            raise ValueError("give both S and D of the forward, or neither")
+        if S is None:
+            L.check(L.lib().tbx_batch_adjoint(self.handle, _ptr(xyz), _ptr(se), _ptr(Sb),
+                                              _ptr(Db), _ptr(Qb), _ptr(Hb), _ptr(dxyz),
+                                              _ptr(dse), self._stream()))
+        else:
+            assert S.numel() == self.ndense and D.numel() == 3 * self.ndense
+            L.check(L.lib().tbx_batch_adjoint_with_forward(
+                self.handle, _ptr(xyz), _ptr(se), _ptr(S), _ptr(D), _ptr(Sb), _ptr(Db),
+                _ptr(Qb), _ptr(Hb), _ptr(dxyz), _ptr(dse), self._stream()))
         return dxyz, dse
 
     # -- coordination number (mctc-lib ncoord "dexp", xtb/singlepoint.f90:224) --
@@ -392,17 +403,20 @@ class Batch:
 class H0Integrals(torch.autograd.Function):
     """(xyz, se) -> (S, D, Q, H0) packed over a ``Batch``; the backward runs
     the adjoint kernel, recomputing the integrals and their derivatives
-    instead of storing derivative tensors."""
+    instead of storing derivative tensors.  The outputs S and D are kept for
+    the backward (``Batch.adjoint`` with S, D); modifying them in place
+    before the backward makes autograd raise."""
 
     @staticmethod
     def forward(ctx, xyz, se, batch):
         ctx.batch = batch
-        ctx.save_for_backward(xyz, se)
-        return batch.forward(xyz.detach(), se.detach())
+        S, D, Q, H = batch.forward(xyz.detach(), se.detach())
+        ctx.save_for_backward(xyz, se, S, D)
+        return S, D, Q, H
 
     @staticmethod
     def backward(ctx, Sb, Db, Qb, Hb):
-        xyz, se = ctx.saved_tensors
+        xyz, se, S, D = ctx.saved_tensors
         b = ctx.batch
         dev = xyz.device
 
@@ -413,7 +427,7 @@ class H0Integrals(torch.autograd.Function):
 
         dxyz, dse = b.adjoint(xyz.detach(), se.detach(), dense(Sb, b.ndense),
                               dense(Db, 3 * b.ndense), dense(Qb, 6 * b.ndense),
-                              dense(Hb, b.ndense))
+                              dense(Hb, b.ndense), S=S.detach(), D=D.detach())
         return dxyz, dse, None
 
 

@@ -34,8 +34,9 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                                           const double *vdp, const double *vqp, int nspin,
                                           long long pstride, long long vstride,
                                           const double *Sb, const double *Db, const double *Qb,
-                                          const double *Hb, double *grad, double *dse,
-                                          double *sigma, const int *nnl, int *flag, int chunk,
+                                          const double *Hb, const double *Sf, const double *Df,
+                                          double *grad, double *dse, double *sigma,
+                                          const int *nnl, int *flag, int chunk,
                                           hipStream_t stream);
 extern "C" hipError_t tbx_launch_dedcn(const TbxSpecies *sp, const TbxMols *mol,
                                        const double *dse, const double *dsedcn, const double *P,
@@ -818,6 +819,11 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
     mt.nmol = hm.nmol;
     mt.nat = hm.nat;
     mt.ntrans = (int)(trans.size() / 3);
+    mt.zero_image = 1;
+    for (const int2 &tr : hm.mol_tr)
+        if (tr.y != 1 || trans[3 * tr.x] != 0.0 || trans[3 * tr.x + 1] != 0.0 ||
+            trans[3 * tr.x + 2] != 0.0)
+            mt.zero_image = 0;
     mt.trans = reinterpret_cast<const double *>(at(o_trans));
     mt.mol_tr = reinterpret_cast<const int2 *>(at(o_mol_tr));
     mt.mol_atom = reinterpret_cast<const int *>(at(o_mol_atom));
@@ -1295,7 +1301,8 @@ extern "C" int tbx_get_hamiltonian_gradient(tbx_context *ctx, const tbx_geom *ge
     HIP_TRY(tbx_launch_gradient(0, &ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
                                 ctx->se.as<double>(), ctx->P.as<double>(), ctx->W.as<double>(), ctx->vao.as<double>(),
                                 ctx->vdp.as<double>(), ctx->vqp.as<double>(), nspin, (long long)n2,
-                                nao, nullptr, nullptr, nullptr, nullptr, ctx->grad.as<double>(),
+                                nao, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                ctx->grad.as<double>(),
                                 ctx->dse.as<double>(), ctx->sigma.as<double>(),
                                 list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), 0, s));
     HIP_TRY(tbx_launch_dedcn(&ctx->tab.sp, &ctx->tab.mt, ctx->dse.as<double>(),
@@ -1352,7 +1359,8 @@ extern "C" int tbx_get_hamiltonian_adjoint(tbx_context *ctx, const tbx_geom *geo
     HIP_TRY(tbx_launch_gradient(1, &ctx->tab.sp, &ctx->tab.mt, ctx->xyz.as<double>(),
                                 ctx->se.as<double>(), nullptr, nullptr, nullptr, nullptr, nullptr,
                                 1, 0, 0, ctx->S.as<double>(), ctx->D.as<double>(),
-                                ctx->Q.as<double>(), ctx->H.as<double>(), ctx->grad.as<double>(),
+                                ctx->Q.as<double>(), ctx->H.as<double>(), nullptr, nullptr,
+                                ctx->grad.as<double>(),
                                 ctx->dse.as<double>(), ctx->sigma.as<double>(),
                                 list ? ctx->nnl.as<int>() : nullptr, ctx->flag.as<int>(), 0, s));
     std::vector<double> g(3 * nat), e(hm.nsh), sg(9);
@@ -1771,7 +1779,7 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
     HIP_TRY(tbx_launch_fill(b->dse_tmp.as<double>(), b->hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
     HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp, d_vqp, 1, b->hm.ndense, b->hm.nao,
-                                nullptr, nullptr, nullptr, nullptr, d_grad,
+                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_grad,
                                 b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(),
                                 b->tiles.grad_chunk, s));
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
@@ -1808,15 +1816,17 @@ extern "C" int tbx_batch_gradient_noalloc(tbx_batch *b, const double *d_xyz, con
     HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
     HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp,
                                 d_vqp, 1, b->hm.ndense, b->hm.nao, nullptr, nullptr, nullptr,
-                                nullptr, d_grad, dse, d_sigma, nullptr, b->flag.as<int>(),
+                                nullptr, nullptr, nullptr, d_grad, dse, d_sigma, nullptr,
+                                b->flag.as<int>(),
                                 b->tiles.grad_chunk, s));
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, dse, d_dsedcn, d_P, d_dEdcn, s));
     return TBX_OK;
 }
 
-extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
-                                 const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
-                                 const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream)
+static int batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
+                         const double *d_S, const double *d_D, const double *d_Sbar,
+                         const double *d_Dbar, const double *d_Qbar, const double *d_Hbar,
+                         double *d_dxyz, double *d_dse, void *stream)
 {
     if (!b || !d_xyz || !d_se || !d_Sbar || !d_Dbar || !d_Qbar || !d_Hbar || !d_dxyz || !d_dse)
         return fail(TBX_ERR_ARG, "null argument");
@@ -1825,11 +1835,32 @@ extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double
     HIP_TRY(mark(b, 6, s));
     HIP_TRY(tbx_launch_fill(d_dxyz, 3LL * b->hm.nat, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_dse, b->hm.nsh, 0.0, s));
-    HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, nullptr, nullptr, nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar,
-                                d_Dbar, d_Qbar, d_Hbar, d_dxyz, d_dse, b->dse_tmp.as<double>(),
+    HIP_TRY(tbx_launch_gradient(1, &b->tab.sp, &b->tab.mt, d_xyz, d_se, nullptr, nullptr,
+                                nullptr, nullptr, nullptr, 1, 0, 0, d_Sbar, d_Dbar, d_Qbar,
+                                d_Hbar, d_S, d_D, d_dxyz, d_dse, b->dse_tmp.as<double>(),
                                 nullptr, b->flag.as<int>(), b->tiles.grad_chunk, s));
     HIP_TRY(mark(b, 7, s));
     return TBX_OK;
+}
+
+extern "C" int tbx_batch_adjoint(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                 const double *d_Sbar, const double *d_Dbar, const double *d_Qbar,
+                                 const double *d_Hbar, double *d_dxyz, double *d_dse, void *stream)
+{
+    return batch_adjoint(b, d_xyz, d_se, nullptr, nullptr, d_Sbar, d_Dbar, d_Qbar, d_Hbar,
+                         d_dxyz, d_dse, stream);
+}
+
+extern "C" int tbx_batch_adjoint_with_forward(tbx_batch *b, const double *d_xyz,
+                                              const double *d_se, const double *d_S,
+                                              const double *d_D, const double *d_Sbar,
+                                              const double *d_Dbar, const double *d_Qbar,
+                                              const double *d_Hbar, double *d_dxyz,
+                                              double *d_dse, void *stream)
+{
+    if (!d_S || !d_D) return fail(TBX_ERR_ARG, "null argument");
+    return batch_adjoint(b, d_xyz, d_se, d_S, d_D, d_Sbar, d_Dbar, d_Qbar, d_Hbar, d_dxyz,
+                         d_dse, stream);
 }
 
 // Coordination number of the batch (mctc-lib ncoord "dexp" as GFN2 uses it,

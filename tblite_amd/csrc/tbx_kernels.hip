@@ -34,7 +34,10 @@
 // moment weights shifted to centre i; work items recompute the primitive
 // integrals and their R_i derivatives (Obara-Saika tables shared by the
 // Cartesian pairs of s/p shell pairs) and contract them with the record at
-// once -- no derivative tensor is stored.  Item results are summed across
+// once -- no derivative tensor is stored.  Given the forward's S and D of a
+// molecule batch, the terms through the moment shift and the H0 distance
+// polynomial are contracted per element from them instead
+// (explicit_element), and the records shrink to the 10 moment weights.  Item results are summed across
 // the wavefront with DPP segmented scans, then per pair in LDS, and leave as
 // FP64 atomics into dE/dR, dE/dse and sigma.  Far units are not dispatched
 // while the hint holds; k_gradient_far runs them (persistent) when it does
@@ -195,6 +198,11 @@ struct GradArgs {
     long long vstride;   // elements between spin planes of vao
     // adjoint: dense adjoints of S, D, Q, H0
     const double *Sb, *Db, *Qb, *Hb;
+    // the forward's S and D (operator on the column atom) at the same
+    // geometry, molecules only: the explicit pair-vector terms are then
+    // contracted once per element (explicit_element) instead of in the work
+    // items; NULL otherwise
+    const double *Sf, *Df;
     // outputs (accumulated with atomics; caller zeroes)
     double *grad;   // [3*nat]
     double *dse;    // [nsh]  dE/d(selfenergy)
@@ -1025,7 +1033,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 // the pair's block into Cartesian ones (transpose of the spherical map),
 // shifts the j-centred multipole weights to centre i and stores the record.
 // Returns the number of primitive pairs kept by the screening.
-template <int MODE>
+template <int MODE, bool EX>
 __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v[3], int ish,
                            int jsh, int li, int lj, int ki, int kj, int nao, long long dense0,
                            int ao_m, double hs, double sesum, double dsr, double *rec, int ld)
@@ -1118,6 +1126,7 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
     rec[(R_W + 3) * ld] = wDi[2] + wDj[2] + 2.0 * Wv[2];
 #pragma unroll
     for (int k = 0; k < 6; ++k) rec[(R_W + 4 + k) * ld] = wQi[k] + wQj[k];
+    if (!EX) return;  // the explicit terms come from explicit_element
     const double hd = hw * dsr;
     rec[(R_EX + 0) * ld] = wDj[0] + 2.0 * Wv[0] + hd * vx;
     rec[(R_EX + 1) * ld] = wDj[1] + 2.0 * Wv[1] + hd * vy;
@@ -1127,11 +1136,60 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
     rec[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
 }
 
+// The explicit pair-vector terms of one spherical element (mu on atom j, nu
+// on atom i; e1 = (mu, nu), e2 = (nu, mu)) from the forward's S and
+// i-centred D there.  Per Cartesian pair the work items would add
+// S_c ex_c + 2 Wj_c D_c and S_c hsw_c (task_setup's R_EX, R_WQ, R_HSW), with
+// weights that are the spherical weights X pulled back through the
+// spherical transform T; since sum_c (T^T X T)_c S_c = sum_e X_e (T S T^T)_e,
+// the same sum runs over the block's elements with the forward's spherical
+// S and D.  Adds the element's share of the pair sum dE/dv to g and of
+// dE/dse_i = dE/dse_j to ds.
+template <int MODE>
+__device__ __forceinline__ void explicit_element(const GradArgs &A, long long e1, long long e2,
+                                                 int i, int j, const double v[3], double hs,
+                                                 double sesum, double dsr, double g[3],
+                                                 double &ds)
+{
+    double hsum, db[3], qb[6];
+    if (MODE == GRAD_PLAIN) {  // weights -P x (atom j's potentials), as in task_setup
+        const double p = A.P[e1];
+        hsum = 2.0 * p;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) db[k] = -p * A.vdp[3 * j + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qb[k] = -p * A.vqp[6 * j + k];
+    } else {
+        hsum = A.Hb[e1] + A.Hb[e2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) db[k] = A.Db[3 * e2 + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qb[k] = A.Qb[6 * e2 + k];
+    }
+    const double tq = 0.5 * (qb[0] + qb[2] + qb[5]);  // the trace-removal map
+    double wq[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wq[k] = 1.5 * qb[k];
+    wq[0] -= tq;
+    wq[2] -= tq;
+    wq[5] -= tq;
+    const double s = A.Sf[e1];
+    const double d[3] = {A.Df[3 * e1], A.Df[3 * e1 + 1], A.Df[3 * e1 + 2]};
+    const double hd = sesum * hs * hsum * dsr;
+    const double X[3] = {db[0] + 2.0 * (wq[0] * v[0] + 0.5 * (wq[1] * v[1] + wq[3] * v[2])) + hd * v[0],
+                         db[1] + 2.0 * (wq[2] * v[1] + 0.5 * (wq[1] * v[0] + wq[4] * v[2])) + hd * v[1],
+                         db[2] + 2.0 * (wq[5] * v[2] + 0.5 * (wq[3] * v[0] + wq[4] * v[1])) + hd * v[2]};
+    g[0] += s * X[0] + 2.0 * (wq[0] * d[0] + 0.5 * (wq[1] * d[1] + wq[3] * d[2]));
+    g[1] += s * X[1] + 2.0 * (wq[2] * d[1] + 0.5 * (wq[1] * d[0] + wq[4] * d[2]));
+    g[2] += s * X[2] + 2.0 * (wq[5] * d[2] + 0.5 * (wq[3] * d[0] + wq[4] * d[1]));
+    ds += (j == i ? 0.5 : 1.0) * 0.5 * hs * hsum * s;
+}
+
 
 // Primitive pairs [p0, p1) of one task of type (LI, LJ): accumulate the
 // contracted derivative a3 = sum cc w.dM/dR_i, the overlap s and the i-centred
 // dipole moments dacc.
-template <int LI, int LJ>
+template <int LI, int LJ, bool EX>
 __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const double *rec, int ld,
                                           const double vec[3], int ki, int kj, int p0, int p1,
                                           double a3[3], double &s, double dacc[3])
@@ -1159,14 +1217,38 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
 #pragma unroll
         for (int k = 0; k < 3; ++k)
             grad1d_t<LI, LJ>(ei[k], ej[k], -vec[k] * ajo, vec[k] * aio, ai, M, V[k], G[k]);
-        s += cc * (V[0][0] * V[1][0] * V[2][0]);
-        dacc[0] += cc * (V[0][1] * V[1][0] * V[2][0]);
-        dacc[1] += cc * (V[0][0] * V[1][1] * V[2][0]);
-        dacc[2] += cc * (V[0][0] * V[1][0] * V[2][1]);
+        if (EX) {
+            s += cc * (V[0][0] * V[1][0] * V[2][0]);
+            dacc[0] += cc * (V[0][1] * V[1][0] * V[2][0]);
+            dacc[1] += cc * (V[0][0] * V[1][1] * V[2][0]);
+            dacc[2] += cc * (V[0][0] * V[1][0] * V[2][1]);
+        }
         a3[0] += cc * contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
         a3[1] += cc * contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
         a3[2] += cc * contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
     }
+}
+
+// The explicit v terms of one Cartesian pair and primitive pair (weight cc)
+// in the s/p items, from its 1D factors V (moments 0..2 of each direction)
+// and task record r: S ex + 2 Wj D_i into ex and cc S hsw added to ds.
+__device__ __forceinline__ void explicit_terms(const double V[3][3], const double *r, int ld,
+                                               double cc, double ex[3], double &ds)
+{
+    const double S = V[0][0] * V[1][0] * V[2][0];
+    const double D0 = V[0][1] * V[1][0] * V[2][0];
+    const double D1 = V[0][0] * V[1][1] * V[2][0];
+    const double D2 = V[0][0] * V[1][0] * V[2][1];
+    const double *wq = r + R_WQ * ld;
+    const double wq0 = wq[0], wq1 = wq[ld], wq2 = wq[2 * ld], wq3 = wq[3 * ld],
+                 wq4 = wq[4 * ld], wq5 = wq[5 * ld];
+    const double WD0 = wq0 * D0 + 0.5 * (wq1 * D1 + wq3 * D2);
+    const double WD1 = wq2 * D1 + 0.5 * (wq1 * D0 + wq4 * D2);
+    const double WD2 = wq5 * D2 + 0.5 * (wq3 * D0 + wq4 * D1);
+    ex[0] = S * r[R_EX * ld] + 2.0 * WD0;
+    ex[1] = S * r[(R_EX + 1) * ld] + 2.0 * WD1;
+    ex[2] = S * r[(R_EX + 2) * ld] + 2.0 * WD2;
+    ds += cc * S * r[R_HSW * ld];
 }
 
 // 1D overlap table of one direction for a primitive pair (normalised
@@ -1193,9 +1275,9 @@ __device__ __forceinline__ void os_table(double pa, double pb, double h,
 // are built once per primitive pair and shared by the Cartesian pairs of one
 // row ki = ROW (ROW < 0: every row), with compile-time exponents; each
 // Cartesian pair reads its task record (rec + cp, rec = the row's first
-// Cartesian pair) and adds its contracted derivative with the explicit v
-// terms to g and its H0 share to ds.
-template <int LI, int LJ, int ROW>
+// Cartesian pair) and adds its contracted derivative to g, with EX the
+// explicit v terms too, and its H0 share to ds.
+template <int LI, int LJ, int ROW, bool EX>
 __device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, const double *rec,
                                               int ld, const double vec[3], int p0, int p1,
                                               double g[3], double &ds)
@@ -1235,23 +1317,20 @@ __device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, con
                 double w[10];
 #pragma unroll
                 for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
-                const double S = V[0][0] * V[1][0] * V[2][0];
-                const double D0 = V[0][1] * V[1][0] * V[2][0];
-                const double D1 = V[0][0] * V[1][1] * V[2][0];
-                const double D2 = V[0][0] * V[1][0] * V[2][1];
-                const double *wq = r + R_WQ * ld;
-                const double wq0 = wq[0], wq1 = wq[ld], wq2 = wq[2 * ld], wq3 = wq[3 * ld],
-                             wq4 = wq[4 * ld], wq5 = wq[5 * ld];
-                const double WD0 = wq0 * D0 + 0.5 * (wq1 * D1 + wq3 * D2);
-                const double WD1 = wq2 * D1 + 0.5 * (wq1 * D0 + wq4 * D2);
-                const double WD2 = wq5 * D2 + 0.5 * (wq3 * D0 + wq4 * D1);
-                g[0] += cc * (contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4) +
-                              S * r[R_EX * ld] + 2.0 * WD0);
-                g[1] += cc * (contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4) +
-                              S * r[(R_EX + 1) * ld] + 2.0 * WD1);
-                g[2] += cc * (contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4) +
-                              S * r[(R_EX + 2) * ld] + 2.0 * WD2);
-                ds += cc * S * r[R_HSW * ld];
+                const double c0 = contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
+                const double c1 = contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
+                const double c2 = contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
+                if (EX) {
+                    double ex[3];
+                    explicit_terms(V, r, ld, cc, ex, ds);
+                    g[0] += cc * (c0 + ex[0]);
+                    g[1] += cc * (c1 + ex[1]);
+                    g[2] += cc * (c2 + ex[2]);
+                } else {
+                    g[0] += cc * c0;
+                    g[1] += cc * c1;
+                    g[2] += cc * c2;
+                }
             }
     }
 }
@@ -1259,6 +1338,7 @@ __device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, con
 // Primitive pairs [p0, p1) of one row ki = row of a p-p shell pair: as
 // shellpair_run, with the row a run-time value so that the rows of a
 // wavefront share one code path (the exponent of direction k is k == row).
+template <bool EX>
 __device__ __forceinline__ void pprow_run(const double *__restrict__ pp, const double *rec,
                                           int ld, const double vec[3], int row, int p0, int p1,
                                           double g[3], double &ds)
@@ -1306,23 +1386,20 @@ __device__ __forceinline__ void pprow_run(const double *__restrict__ pp, const d
             double w[10];
 #pragma unroll
             for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
-            const double S = V[0][0] * V[1][0] * V[2][0];
-            const double D0 = V[0][1] * V[1][0] * V[2][0];
-            const double D1 = V[0][0] * V[1][1] * V[2][0];
-            const double D2 = V[0][0] * V[1][0] * V[2][1];
-            const double *wq = r + R_WQ * ld;
-            const double wq0 = wq[0], wq1 = wq[ld], wq2 = wq[2 * ld], wq3 = wq[3 * ld],
-                         wq4 = wq[4 * ld], wq5 = wq[5 * ld];
-            const double WD0 = wq0 * D0 + 0.5 * (wq1 * D1 + wq3 * D2);
-            const double WD1 = wq2 * D1 + 0.5 * (wq1 * D0 + wq4 * D2);
-            const double WD2 = wq5 * D2 + 0.5 * (wq3 * D0 + wq4 * D1);
-            g[0] += cc * (contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4) +
-                          S * r[R_EX * ld] + 2.0 * WD0);
-            g[1] += cc * (contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4) +
-                          S * r[(R_EX + 1) * ld] + 2.0 * WD1);
-            g[2] += cc * (contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4) +
-                          S * r[(R_EX + 2) * ld] + 2.0 * WD2);
-            ds += cc * S * r[R_HSW * ld];
+            const double c0 = contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
+            const double c1 = contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
+            const double c2 = contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
+            if (EX) {
+                double ex[3];
+                explicit_terms(V, r, ld, cc, ex, ds);
+                g[0] += cc * (c0 + ex[0]);
+                g[1] += cc * (c1 + ex[1]);
+                g[2] += cc * (c2 + ex[2]);
+            } else {
+                g[0] += cc * c0;
+                g[1] += cc * c1;
+                g[2] += cc * c2;
+            }
         }
     }
 }
@@ -1398,18 +1475,23 @@ __device__ __forceinline__ double wave_sum_d(double v)
 // (type-major, so a wavefront mostly runs one specialised code path) and adds
 // the results to its pair's LDS sums; (5) the pair sums go to atoms j, the
 // segment sums to atoms i and the unit sum to sigma, with FP64 atomics.
-template <int MODE>
+template <int MODE, bool EX>
 __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
 {
+    // without EX a task record holds the 10 moment weights only, and the
+    // shell-pair factors live in the records' space until the records are
+    // written (3 * GTHREADS <= 10 * GTHREADS doubles)
+    constexpr int nrec = EX ? NREC : R_EX;
     __shared__ GradStage U;
-    __shared__ double rec[NREC * GTHREADS];
+    __shared__ double rec[nrec * GTHREADS];
     __shared__ double accP[TBX_GRAD_UNIT_PAIRS][3 + 2 * MAXSH];  // G (3), dse_i, dse_j per shell
     __shared__ int tinfo[GTHREADS];  // type | ki << 4 | kj << 8 | pair << 12 | ish << 24 | jsh << 27
     __shared__ int tpp[GTHREADS];    // first primitive pair of the task
     __shared__ int tkept[GTHREADS];
     __shared__ __align__(16) int coff[GTHREADS + 1];  // first chunk of each task
     __shared__ int spk[GTHREADS], spp[GTHREADS];  // kept pairs, first entry
-    __shared__ double sph[3][GTHREADS];  // hs, sesum, dsr per shell pair
+    __shared__ double sph_own[EX ? 3 : 1][GTHREADS];  // hs, sesum, dsr per shell pair
+    double (*sph)[GTHREADS] = EX ? sph_own : reinterpret_cast<double (*)[GTHREADS]>(rec);
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -1450,19 +1532,40 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
         sph[2][k] = dsr;
     }
     __syncthreads();
-    // task records: one Cartesian pair per lane (ntask <= GTHREADS = GTHREADS)
+    // task records: one Cartesian pair per lane (ntask <= GTHREADS)
     const int ntask = U.toff[9];
+    const int lane = tid & 63;
     int nch = 0;
+    int spx = 0, kept = 0;
+    int4 d = make_int4(0, 0, 0, 0), e = make_int4(0, 0, 0, 0);
+    double hs = 0.0, sesum = 0.0, dsr = 0.0;
     if (tid < ntask) {
-        int spx;
-        const int4 d = decode_task(U, tid, spx);
-        const int4 e = unpack_spl(U.spl[spx]);
-        const int kept = spk[spx];
+        d = decode_task(U, tid, spx);
+        e = unpack_spl(U.spl[spx]);
+        kept = spk[spx];
+        hs = sph[0][spx];
+        sesum = sph[1][spx];
+        dsr = sph[2][spx];
+    }
+    if (!EX) __syncthreads();  // the shell-pair factors are read before the records overwrite them
+    int xkey = -1 - lane;  // shell pair of the lane's explicit terms; unique if idle
+    double gx[3] = {0.0, 0.0, 0.0}, dsx = 0.0;
+    if (tid < ntask) {
         if (kept > 0) {
+            const PairInfo pr = U.pinf[e.x];
             const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
-            task_setup<MODE>(A, U.pinf[e.x], v, e.z, e.y, d.x / 3, d.x % 3, d.y, d.z, nao,
-                             dense0, ao_m, sph[0][spx], sph[1][spx], sph[2][spx], rec + tid,
-                             GTHREADS);
+            const int li = d.x / 3, lj = d.x % 3;
+            task_setup<MODE, EX>(A, pr, v, e.z, e.y, li, lj, d.y, d.z, nao, dense0, ao_m, hs,
+                                 sesum, dsr, rec + tid, GTHREADS);
+            // the spherical element (ki, kj) of the shell pair is this lane's
+            if (!EX && d.y < 2 * li + 1 && d.z < 2 * lj + 1) {
+                const int mu = pr.aoj + ao_first(pr.dj, e.y) + d.z;
+                const int nu = pr.aoi + ao_first(pr.di, e.z) + d.y;
+                explicit_element<MODE>(A, dense0 + (long long)nu * nao + mu,
+                                       dense0 + (long long)mu * nao + nu, pr.i, pr.j, v, hs,
+                                       sesum, dsr, gx, dsx);
+            }
+            xkey = e.x | (e.z << 12) | (e.y << 15);
         }
         tinfo[tid] = d.x | (d.y << 4) | (d.z << 8) | (e.x << 12) | (e.z << 24) | (e.y << 27);
         tpp[tid] = spp[spx];
@@ -1473,13 +1576,31 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
         const bool rows = (d.x == 4);
         nch = ((whole && d.w != 0) || (rows && d.z != 0)) ? 0 : (kept + chunk - 1) / chunk;
     }
+    if (!EX) {
+        // the explicit terms of a shell pair's elements (consecutive lanes)
+        // summed across the wavefront, then added to the pair's sums
+        seg_scan_step<0x111, 0xf>(xkey, gx, dsx);
+        seg_scan_step<0x112, 0xf>(xkey, gx, dsx);
+        seg_scan_step<0x114, 0xf>(xkey, gx, dsx);
+        seg_scan_step<0x118, 0xf>(xkey, gx, dsx);
+        seg_scan_step<0x142, 0xa>(xkey, gx, dsx);
+        seg_scan_step<0x143, 0xc>(xkey, gx, dsx);
+        const int knext = __shfl_down(xkey, 1, 64);
+        if (xkey >= 0 && (lane == 63 || knext != xkey)) {
+            const int p = xkey & 4095, ish = (xkey >> 12) & 7, jsh = (xkey >> 15) & 7;
+            atomicAdd(&accP[p][0], gx[0]);
+            atomicAdd(&accP[p][1], gx[1]);
+            atomicAdd(&accP[p][2], gx[2]);
+            atomicAdd(&accP[p][3 + ish], dsx);
+            atomicAdd(&accP[p][3 + MAXSH + jsh], dsx);
+        }
+    }
     // (3) chunk offsets: block-wide exclusive scan
     const int nchunk = block_scan<GTHREADS>(nch, coff, U.wsum);
 
     // (4) work items, one per lane and step; the results of a shell pair's
     // items (consecutive lanes) are summed across the wavefront before the
     // LDS atomics
-    const int lane = tid & 63;
     for (int base = 0; base < nchunk; base += GTHREADS) {
         const int c = base + tid;
         int key = -1 - lane;  // shell pair of the item (p | ish << 12 | jsh << 15); unique if idle
@@ -1498,20 +1619,25 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
         const int L = GTHREADS;
         if (t == 0 || t == 1 || t == 3 || t == 4) {
             switch (t) {
-            case 0: shellpair_run<0, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
-            case 1: shellpair_run<0, 1, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
-            case 3: shellpair_run<1, 0, -1>(sp.pp, r, L, vec, p0, p1, g, ds); break;
-            default: pprow_run(sp.pp, r, L, vec, ki, p0, p1, g, ds); break;  // p-p by rows
+            case 0: shellpair_run<0, 0, -1, EX>(sp.pp, r, L, vec, p0, p1, g, ds); break;
+            case 1: shellpair_run<0, 1, -1, EX>(sp.pp, r, L, vec, p0, p1, g, ds); break;
+            case 3: shellpair_run<1, 0, -1, EX>(sp.pp, r, L, vec, p0, p1, g, ds); break;
+            default: pprow_run<EX>(sp.pp, r, L, vec, ki, p0, p1, g, ds); break;  // p-p by rows
             }
         } else {
             double a3[3] = {0, 0, 0}, dacc[3] = {0, 0, 0}, s = 0.0;
             switch (t) {
-            case 2: chunk_run<0, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-            case 5: chunk_run<1, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-            case 6: chunk_run<2, 0>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-            case 7: chunk_run<2, 1>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
-            default: chunk_run<2, 2>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 2: chunk_run<0, 2, EX>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 5: chunk_run<1, 2, EX>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 6: chunk_run<2, 0, EX>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            case 7: chunk_run<2, 1, EX>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
+            default: chunk_run<2, 2, EX>(sp.pp, r, L, vec, ki, kj, p0, p1, a3, s, dacc); break;
             }
+            if (!EX) {
+                g[0] = a3[0];
+                g[1] = a3[1];
+                g[2] = a3[2];
+            } else {
             // explicit v dependence: S ex + 2 Wj D_i
             const double *wq = r + R_WQ * L;
             const double wq0 = wq[0], wq1 = wq[L], wq2 = wq[2 * L], wq3 = wq[3 * L],
@@ -1523,26 +1649,34 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
             g[1] = a3[1] + s * r[(R_EX + 1) * L] + 2.0 * WD1;
             g[2] = a3[2] + s * r[(R_EX + 2) * L] + 2.0 * WD2;
             ds = s * r[R_HSW * L];
+            }
         }
         }
         // segmented inclusive scan over runs of equal key (a shell pair's
         // items are consecutive lanes) with DPP moves: shifts by 1, 2, 4, 8
         // within rows of 16 lanes, then the row broadcasts of lanes 15 and 31;
         // the last lane of each run ends with the run's total
-        seg_scan_step<0x111, 0xf>(key, g, ds);  // row_shr:1
-        seg_scan_step<0x112, 0xf>(key, g, ds);  // row_shr:2
-        seg_scan_step<0x114, 0xf>(key, g, ds);  // row_shr:4
-        seg_scan_step<0x118, 0xf>(key, g, ds);  // row_shr:8
-        seg_scan_step<0x142, 0xa>(key, g, ds);  // row_bcast:15 into rows 1 and 3
-        seg_scan_step<0x143, 0xc>(key, g, ds);  // row_bcast:31 into rows 2 and 3
+        // (without EX the items have no dE/dse share)
+        if (EX) {
+            seg_scan_step<0x111, 0xf>(key, g, ds);  // row_shr:1
+            seg_scan_step<0x112, 0xf>(key, g, ds);  // row_shr:2
+            seg_scan_step<0x114, 0xf>(key, g, ds);  // row_shr:4
+            seg_scan_step<0x118, 0xf>(key, g, ds);  // row_shr:8
+            seg_scan_step<0x142, 0xa>(key, g, ds);  // row_bcast:15 into rows 1 and 3
+            seg_scan_step<0x143, 0xc>(key, g, ds);  // row_bcast:31 into rows 2 and 3
+        } else {
+            seg_scan_n<3>(key, g);
+        }
         const int knext = __shfl_down(key, 1, 64);
         if (key >= 0 && (lane == 63 || knext != key)) {
             const int p = key & 4095, ish = (key >> 12) & 7, jsh = (key >> 15) & 7;
             atomicAdd(&accP[p][0], g[0]);
             atomicAdd(&accP[p][1], g[1]);
             atomicAdd(&accP[p][2], g[2]);
-            atomicAdd(&accP[p][3 + ish], ds);
-            atomicAdd(&accP[p][3 + MAXSH + jsh], ds);
+            if (EX) {
+                atomicAdd(&accP[p][3 + ish], ds);
+                atomicAdd(&accP[p][3 + MAXSH + jsh], ds);
+            }
         }
     }
     __syncthreads();
@@ -1606,23 +1740,27 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
     }
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(GTHREADS, 2) k_gradient(GradArgs A)
+// EX: the explicit pair-vector terms in the work items (the forward's S and
+// D not given, or lattice images).  Without them the kernel fits three
+// wavefronts per SIMD: 168 VGPRs and under 27 KB of LDS per workgroup.
+template <int MODE, bool EX>
+__global__ void __launch_bounds__(GTHREADS, EX ? 2 : 3) k_gradient(GradArgs A)
 {
-    gradient_unit<MODE>(A, blockIdx.x);
+    gradient_unit<MODE, EX>(A, blockIdx.x);
 }
 
 // The far gradient units (no pair within cutoff + skin at the pair-order
 // hint): nothing to do while every atom is within skin/2 of its hint
 // position (flag 0, set by k_hint_check), since then every such pair is
 // still beyond the cutoff.  Otherwise the workgroups walk them.
-template <int MODE>
-__global__ void __launch_bounds__(GTHREADS, 2) k_gradient_far(GradArgs A, const int *stale)
+template <int MODE, bool EX>
+__global__ void __launch_bounds__(GTHREADS, EX ? 2 : 3) k_gradient_far(GradArgs A,
+                                                                     const int *stale)
 {
     if (*stale == 0) return;
     for (int u = A.mol.nunit_near + blockIdx.x; u < A.mol.nunit; u += gridDim.x) {
         __syncthreads();  // the previous unit's LDS is read until its last phase
-        gradient_unit<MODE>(A, u);
+        gradient_unit<MODE, EX>(A, u);
     }
 }
 
@@ -1782,12 +1920,16 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
                                           const double *vdp, const double *vqp, int nspin,
                                           long long pstride, long long vstride,
                                           const double *Sb, const double *Db, const double *Qb,
-                                          const double *Hb, double *grad, double *dse,
-                                          double *sigma, const int *nnl, int *flag, int chunk,
+                                          const double *Hb, const double *Sf, const double *Df,
+                                          double *grad, double *dse, double *sigma,
+                                          const int *nnl, int *flag, int chunk,
                                           hipStream_t stream)
 {
+    // the per-element explicit terms need the forward's outputs and one
+    // translation (T = 0) per structure: each element then belongs to one pair
+    const bool ex = !(Sf && Df && mol->zero_image);
     GradArgs a{*sp, *mol, xyz, se, P, W, vao, vdp, vqp, nspin, pstride, vstride,
-               Sb, Db, Qb, Hb, grad, dse, sigma, nnl, flag,
+               Sb, Db, Qb, Hb, ex ? nullptr : Sf, ex ? nullptr : Df, grad, dse, sigma, nnl, flag,
                chunk > 0 ? chunk : GRAD_CHUNK_DEFAULT};
     (void)hipGetLastError();
     const int nfar = mol->nunit - mol->nunit_near;
@@ -1797,22 +1939,28 @@ extern "C" hipError_t tbx_launch_gradient(int mode, const TbxSpecies *sp, const 
         hipLaunchKernelGGL(k_hint_check, dim3((mol->nat + 255) / 256), dim3(256), 0, stream,
                            *mol, xyz, flag);
     }
+    const dim3 near(mol->nunit_near), far(nfar < 2048 ? nfar : 2048), blk(GTHREADS);
+    const int *stale = flag;
     if (mol->nunit_near > 0) {
-        if (mode == GRAD_PLAIN)
-            hipLaunchKernelGGL(k_gradient<GRAD_PLAIN>, dim3(mol->nunit_near), dim3(GTHREADS), 0,
-                               stream, a);
+        if (mode == GRAD_PLAIN && ex)
+            hipLaunchKernelGGL((k_gradient<GRAD_PLAIN, true>), near, blk, 0, stream, a);
+        else if (mode == GRAD_PLAIN)
+            hipLaunchKernelGGL((k_gradient<GRAD_PLAIN, false>), near, blk, 0, stream, a);
+        else if (ex)
+            hipLaunchKernelGGL((k_gradient<GRAD_ADJOINT, true>), near, blk, 0, stream, a);
         else
-            hipLaunchKernelGGL(k_gradient<GRAD_ADJOINT>, dim3(mol->nunit_near), dim3(GTHREADS),
-                               0, stream, a);
+            hipLaunchKernelGGL((k_gradient<GRAD_ADJOINT, false>), near, blk, 0, stream, a);
     }
     if (nfar > 0) {
-        const int grid = nfar < 2048 ? nfar : 2048;
-        if (mode == GRAD_PLAIN)
-            hipLaunchKernelGGL(k_gradient_far<GRAD_PLAIN>, dim3(grid), dim3(GTHREADS), 0, stream,
-                               a, (const int *)flag);
+        if (mode == GRAD_PLAIN && ex)
+            hipLaunchKernelGGL((k_gradient_far<GRAD_PLAIN, true>), far, blk, 0, stream, a, stale);
+        else if (mode == GRAD_PLAIN)
+            hipLaunchKernelGGL((k_gradient_far<GRAD_PLAIN, false>), far, blk, 0, stream, a, stale);
+        else if (ex)
+            hipLaunchKernelGGL((k_gradient_far<GRAD_ADJOINT, true>), far, blk, 0, stream, a, stale);
         else
-            hipLaunchKernelGGL(k_gradient_far<GRAD_ADJOINT>, dim3(grid), dim3(GTHREADS), 0, stream,
-                               a, (const int *)flag);
+            hipLaunchKernelGGL((k_gradient_far<GRAD_ADJOINT, false>), far, blk, 0, stream, a,
+                               stale);
     }
     const int per = NTHREADS / 16;
     if (mode == GRAD_ADJOINT)

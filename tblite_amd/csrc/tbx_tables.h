@@ -177,6 +177,7 @@ __host__ __device__ inline TbxElem tbx_elem_locate(const int *off, int nseg, con
 struct TbxMols {
     int nmol, nat;
     int ntrans;
+    int zero_image;                // every structure has the one translation T = 0
     const double *trans;           // [3*ntrans] translation vectors of all molecules
     const int2 *mol_tr;            // [nmol] (first, count): translations of molecule m
     const int *mol_atom;           // [nmol+1] first atom

@@ -5,7 +5,8 @@ seed 1004, near-first pair order, default tiles); each kernel is launched
 ``--reps`` times after one warm-up, with nothing else on the GPU in between,
 so that ``rocprofv3 --kernel-include-regex`` can pick them out:
 
-    python -m tblite_amd.kernel_driver [--reps 3] [--nmol 2048] [--kernels forward,adjoint,gradient]
+    python -m tblite_amd.kernel_driver [--reps 3] [--nmol 2048]
+        [--kernels forward,adjoint,adjoint_fwd,gradient]
 """
 from __future__ import annotations
 
@@ -43,7 +44,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--nmol", type=int, default=2048)
-    ap.add_argument("--kernels", default="forward,adjoint,gradient")
+    ap.add_argument("--kernels", default="forward,adjoint,adjoint_fwd,gradient")
     a = ap.parse_args(argv)
     b, xyz, se, plain, adjoints = workload(a.nmol)
     outs = (torch.empty(b.ndense, dtype=torch.float64, device="cuda"),
@@ -52,6 +53,9 @@ def main(argv=None):
             torch.empty(b.ndense, dtype=torch.float64, device="cuda"))
     run = {"forward": lambda: b.forward(xyz, se, out=outs),
            "adjoint": lambda: b.adjoint(xyz, se, *adjoints),
+           # with the forward's S and D (the autograd backward); outs hold
+           # them once the forward has run
+           "adjoint_fwd": lambda: b.adjoint(xyz, se, *adjoints, S=outs[0], D=outs[1]),
            "gradient": lambda: b.gradient(xyz, se, *plain)}
     names = [k for k in a.kernels.split(",") if k]
     for k in names:
@@ -63,7 +67,7 @@ def main(argv=None):
             torch.cuda.synchronize()
             t = b.last_timing()
             ms = {"forward": t["forward_ms"], "adjoint": t["adjoint_ms"],
-                  "gradient": t["gradient_ms"]}[k]
+                  "adjoint_fwd": t["adjoint_ms"], "gradient": t["gradient_ms"]}[k]
             print(f"{k} {ms:.3f} ms", flush=True)
     time.sleep(0.1)
     return 0

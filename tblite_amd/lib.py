@@ -113,6 +113,8 @@ _SIGS = {
     "tbx_batch_gradient_noalloc": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                               _P]),
     "tbx_batch_adjoint": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "tbx_batch_adjoint_with_forward": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                  _P]),
     "tbx_batch_last_timing": (ct.c_int, [_P, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
                                          ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "tbx_probe_copy": (ct.c_int, [_P, _P, ct.c_longlong, _P]),

@@ -52,8 +52,8 @@ def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, fwd_unit_tasks=FWD_UNI
         b.set_tiles(fwd_unit_tasks=tf, fwd_chunk=chunk, grad_unit_tasks=tg, grad_chunk=chunk)
         f = a = g = 0.0
         for _ in range(reps):
-            b.forward(xyz, se)
-            b.adjoint(xyz, se, *adjoints)
+            S, D, _, _ = b.forward(xyz, se)
+            b.adjoint(xyz, se, *adjoints, S=S, D=D)  # as the autograd backward runs it
             torch.cuda.synchronize()
             t = b.last_timing()
             f += t["forward_ms"] / reps

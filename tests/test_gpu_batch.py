@@ -200,36 +200,43 @@ def test_adjoint_finite_differences(batch_case):
 
 
 def test_graph_replay_matches_eager(batch_case):
-    """Forward + gradient captured in a HIP graph, replayed several times with
-    eager work in between, every replay equal to the eager result."""
+    """Forward + gradient + adjoint (given the forward's S and D) captured in
+    a HIP graph, replayed several times with eager work in between, every
+    replay equal to the eager result."""
     mols, b, cn, se, dse = batch_case
     _say("[gpu] graph replay")
     P, W, vao, vdp, vqp = _random_inputs(b, 5)
     t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
     xyz = b.tensor_xyz()
     P_, W_, va, vd, vq = t(P), t(W), t(vao), t(vdp), t(vqp)
+    rng = np.random.default_rng(6)
+    adj = [t(rng.normal(size=k * b.ndense)) for k in (1, 3, 6, 1)]
     outs_f = b.forward(xyz, se)
     outs_g = b.gradient(xyz, se, dse, P_, W_, va, vd, vq)
+    outs_a = b.adjoint(xyz, se, *adj, S=outs_f[0], D=outs_f[1])
     torch.cuda.synchronize()
-    ref = [o.clone() for o in outs_f + outs_g]
+    outs = outs_f + outs_g + outs_a
+    ref = [o.clone() for o in outs]
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):
         b.forward(xyz, se, out=outs_f)
         b.gradient(xyz, se, dse, P_, W_, va, vd, vq, out=outs_g)
+        b.adjoint(xyz, se, *adj, out=outs_a, S=outs_f[0], D=outs_f[1])
     torch.cuda.current_stream().wait_stream(s)
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
         b.forward(xyz, se, out=outs_f)
         b.gradient(xyz, se, dse, P_, W_, va, vd, vq, out=outs_g)
+        b.adjoint(xyz, se, *adj, out=outs_a, S=outs_f[0], D=outs_f[1])
     for rep in range(3):
-        for o in outs_f + outs_g:
+        for o in outs:
             o.fill_(float("nan"))
         junk = torch.randn(1000, device="cuda").sum()  # eager work between replays
         g.replay()
         torch.cuda.synchronize()
         del junk
-        for o, r in zip(outs_f + outs_g, ref):
+        for o, r in zip(outs, ref):
             assert torch.abs(o - r).max().item() < 1e-12, rep
 
 
@@ -244,6 +251,8 @@ def test_tile_shapes_keep_results(batch_case):
     xyz = b.tensor_xyz()
     ref_f = [t.clone() for t in b.forward(xyz, se)]
     ref_a = [t.clone() for t in b.adjoint(xyz, se, *adj)]
+    for r, g in zip(ref_a, b.adjoint(xyz, se, *adj, S=ref_f[0], D=ref_f[1])):
+        assert torch.abs(r - g).max().item() < 1e-10
     default = b.tiles()
     try:
         for tasks, gtasks, chunk in [(128, 128, 4), (192, 112, 12), (256, 128, 1),
@@ -253,6 +262,8 @@ def test_tile_shapes_keep_results(batch_case):
             for r, g in zip(ref_f, b.forward(xyz, se)):
                 assert torch.abs(r - g).max().item() < 1e-12, (tasks, chunk)
             for r, g in zip(ref_a, b.adjoint(xyz, se, *adj)):
+                assert torch.abs(r - g).max().item() < 1e-10, (tasks, chunk)
+            for r, g in zip(ref_a, b.adjoint(xyz, se, *adj, S=ref_f[0], D=ref_f[1])):
                 assert torch.abs(r - g).max().item() < 1e-10, (tasks, chunk)
         with pytest.raises(L.TbxError):
             b.set_tiles(fwd_unit_tasks=512)
@@ -281,9 +292,12 @@ def test_pair_order_hint_keeps_results(batch_case):
     def run():
         f = [x.clone() for x in b.forward(xyz, se)]
         a = [x.clone() for x in b.adjoint(xyz, se, *adj)]
+        # the per-element explicit terms (far units walked by the
+        # persistent kernel when the hint is stale)
+        a2 = [x.clone() for x in b.adjoint(xyz, se, *adj, S=f[0], D=f[1])]
         g = [x.clone() for x in b.gradient(xyz, se, dse, P, W, vao, vdp, vqp)]
         torch.cuda.synchronize()
-        return f + a + g
+        return f + a + a2 + g
 
     # hints displaced by less and by more than half the skin (0.25 bohr):
     # the far units are skipped in the first case and walked in the second

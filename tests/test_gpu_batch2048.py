@@ -116,27 +116,35 @@ def test_plain_gradient_every_molecule(workload):
 
 
 def test_adjoint_every_molecule(workload):
-    """Backward kernel (the autograd path) with non-symmetric standard-normal
-    adjoints of S, H0, D, Q: dE/dR and dE/dse of every molecule against the
-    oracle's adjoint of get_hamiltonian."""
+    """Backward kernels with non-symmetric standard-normal adjoints of S, H0,
+    D, Q: dE/dR and dE/dse of every molecule against the oracle's adjoint of
+    get_hamiltonian, from tbx_batch_adjoint and from
+    tbx_batch_adjoint_with_forward (the autograd path, explicit pair-vector
+    terms per element from the forward's S and D)."""
     mols, sts, b, se, _ = workload
     g = torch.Generator(device="cuda")
     g.manual_seed(SEED + 2)
     bars = [torch.randn(c * b.ndense, generator=g, dtype=torch.float64, device="cuda")
             for c in (1, 1, 3, 6)]
     Sb, Hb, Db, Qb = bars
-    dxyz, dse_out = b.adjoint(b.tensor_xyz(), se, Sb, Db, Qb, Hb)
+    xyz = b.tensor_xyz()
+    res = [[t.cpu().numpy() for t in b.adjoint(xyz, se, Sb, Db, Qb, Hb)]]
+    S, D, Q, H = b.forward(xyz, se)
+    del Q, H
+    res.append([t.cpu().numpy() for t in b.adjoint(xyz, se, Sb, Db, Qb, Hb, S=S, D=D)])
+    del S, D
     torch.cuda.synchronize()
-    dxyz, dse_out = dxyz.cpu().numpy(), dse_out.cpu().numpy()
     se_h = se.cpu().numpy()
-    worst = 0.0
+    worst = [0.0, 0.0]
     for m, st in enumerate(sts):
         (a0, a1), (s0, s1), (o0, o1), (d0, d1) = _slices(b, m)
         n = o1 - o0
         mb = [_dense(t, c, d0, d1, n) for t, c in ((Sb, 1), (Hb, 1), (Db, 3), (Qb, 6))]
         gr, er, _ = O.get_hamiltonian_adjoint(st, se_h[s0:s1], *mb)
-        errs = (np.abs(dxyz[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
-                np.abs(dse_out[s0:s1] - er).max())
-        worst = max(worst, *errs)
-        assert max(errs) < TOL_GRAD, (m, errs)
-    _say(f"    adjoint: {b.nmol} molecules, max |GPU - oracle| = {worst:.2e}")
+        for k, (dxyz, dse_out) in enumerate(res):
+            errs = (np.abs(dxyz[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
+                    np.abs(dse_out[s0:s1] - er).max())
+            worst[k] = max(worst[k], *errs)
+            assert max(errs) < TOL_GRAD, (m, k, errs)
+    _say(f"    adjoint: {b.nmol} molecules, max |GPU - oracle| = {worst[0]:.2e}; "
+         f"with the forward's S, D: {worst[1]:.2e}")

@@ -125,7 +125,12 @@ def test_periodic_adjoint(pbatch):
     g.manual_seed(9)
     Sb, Hb, Db, Qb = [torch.randn(c * b.ndense, generator=g, dtype=torch.float64, device="cuda")
                       for c in (1, 1, 3, 6)]
-    dxyz, dse_out = b.adjoint(b.tensor_xyz(), se, Sb, Db, Qb, Hb)
+    xyz = b.tensor_xyz()
+    dxyz, dse_out = b.adjoint(xyz, se, Sb, Db, Qb, Hb)
+    # with lattice images the forward's S, D are not used: the same route
+    S, D, _, _ = b.forward(xyz, se)
+    for r, o in zip((dxyz, dse_out), b.adjoint(xyz, se, Sb, Db, Qb, Hb, S=S, D=D)):
+        assert torch.abs(r - o).max().item() < 1e-12
     torch.cuda.synchronize()
     dxyz, dse_out = dxyz.cpu().numpy(), dse_out.cpu().numpy()
     se_h = se.cpu().numpy()
