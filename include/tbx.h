@@ -247,6 +247,21 @@ int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
                        const double *d_vao, const double *d_vdp, const double *d_vqp,
                        double *d_dEdcn, double *d_grad, double *d_sigma, void *stream);
 
+/* tbx_batch_gradient given also the forward's own outputs d_S (ndense) and d_D
+ * (3*ndense) at the same d_xyz and d_se, which an SCF caller holds from its
+ * tbx_batch_forward (tblite keeps them in its integral container between
+ * get_hamiltonian and get_hamiltonian_gradient).  As in
+ * tbx_batch_adjoint_with_forward, the terms linear in S and D are contracted
+ * per element from them; same outputs as tbx_batch_gradient, to rounding.
+ * Structures with lattice images take tbx_batch_gradient's route.  No
+ * counterpart in the reference. */
+int tbx_batch_gradient_with_forward(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                    const double *d_S, const double *d_D,
+                                    const double *d_dsedcn, const double *d_P,
+                                    const double *d_W, const double *d_vao, const double *d_vdp,
+                                    const double *d_vqp, double *d_dEdcn, double *d_grad,
+                                    double *d_sigma, void *stream);
+
 /* tbx_batch_gradient for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc); no counterpart in the reference.  Same arguments and results

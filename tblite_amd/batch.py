@@ -223,18 +223,30 @@ class Batch:
                                           _ptr(Q), _ptr(H), self._stream()))
         return S, D, Q, H
 
-    def gradient(self, xyz, se, dsedcn, P, W, vao, vdp, vqp, out=None):
+    def gradient(self, xyz, se, dsedcn, P, W, vao, vdp, vqp, out=None, S=None, D=None):
         """Plain ``get_hamiltonian_gradient`` contraction for every molecule:
-        returns dEdcn [nat], gradient [3*nat] (xyz order), sigma [9*nmol]."""
+        returns dEdcn [nat], gradient [3*nat] (xyz order), sigma [9*nmol].
+        Given the forward's S and D at the same xyz and se, the terms linear
+        in them are contracted per element (``tbx_batch_gradient_with_forward``)."""
         dev = self.torch_device
         if out is None:
             out = (torch.empty(self.nat, dtype=torch.float64, device=dev),
                    torch.empty(3 * self.nat, dtype=torch.float64, device=dev),
                    torch.empty(9 * self.nmol, dtype=torch.float64, device=dev))
         dEdcn, grad, sigma = out
-        L.check(L.lib().tbx_batch_gradient(self.handle, _ptr(xyz), _ptr(se), _ptr(dsedcn),
-                                           _ptr(P), _ptr(W), _ptr(vao), _ptr(vdp), _ptr(vqp),
-                                           _ptr(dEdcn), _ptr(grad), _ptr(sigma), self._stream()))
+        if (S is None) != (D is None):
+            raise ValueError("give both S and D of the forward, or neither")
+        if S is None:
+            L.check(L.lib().tbx_batch_gradient(self.handle, _ptr(xyz), _ptr(se), _ptr(dsedcn),
+                                               _ptr(P), _ptr(W), _ptr(vao), _ptr(vdp),
+                                               _ptr(vqp), _ptr(dEdcn), _ptr(grad), _ptr(sigma),
+                                               self._stream()))
+        else:
+            assert S.numel() == self.ndense and D.numel() == 3 * self.ndense
+            L.check(L.lib().tbx_batch_gradient_with_forward(
+                self.handle, _ptr(xyz), _ptr(se), _ptr(S), _ptr(D), _ptr(dsedcn), _ptr(P),
+                _ptr(W), _ptr(vao), _ptr(vdp), _ptr(vqp), _ptr(dEdcn), _ptr(grad), _ptr(sigma),
+                self._stream()))
         return dEdcn, grad, sigma
 
     def adjoint(self, xyz, se, Sb, Db, Qb, Hb, out=None, S=None, D=None):

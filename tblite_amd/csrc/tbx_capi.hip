@@ -1764,10 +1764,11 @@ extern "C" int tbx_batch_forward(tbx_batch *b, const double *d_xyz, const double
     return TBX_OK;
 }
 
-extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
-                                  const double *d_dsedcn, const double *d_P, const double *d_W,
-                                  const double *d_vao, const double *d_vdp, const double *d_vqp,
-                                  double *d_dEdcn, double *d_grad, double *d_sigma, void *stream)
+static int batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
+                          const double *d_S, const double *d_D, const double *d_dsedcn,
+                          const double *d_P, const double *d_W, const double *d_vao,
+                          const double *d_vdp, const double *d_vqp, double *d_dEdcn,
+                          double *d_grad, double *d_sigma, void *stream)
 {
     if (!b || !d_xyz || !d_se || !d_dsedcn || !d_P || !d_W || !d_vao || !d_vdp || !d_vqp ||
         !d_dEdcn || !d_grad || !d_sigma)
@@ -1778,14 +1779,36 @@ extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const doubl
     HIP_TRY(tbx_launch_fill(d_grad, 3LL * b->hm.nat, 0.0, s));
     HIP_TRY(tbx_launch_fill(b->dse_tmp.as<double>(), b->hm.nsh, 0.0, s));
     HIP_TRY(tbx_launch_fill(d_sigma, 9LL * b->hm.nmol, 0.0, s));
-    HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp, d_vqp, 1, b->hm.ndense, b->hm.nao,
-                                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_grad,
-                                b->dse_tmp.as<double>(), d_sigma, nullptr, b->flag.as<int>(),
-                                b->tiles.grad_chunk, s));
+    HIP_TRY(tbx_launch_gradient(0, &b->tab.sp, &b->tab.mt, d_xyz, d_se, d_P, d_W, d_vao, d_vdp,
+                                d_vqp, 1, b->hm.ndense, b->hm.nao, nullptr, nullptr, nullptr,
+                                nullptr, d_S, d_D, d_grad, b->dse_tmp.as<double>(), d_sigma,
+                                nullptr, b->flag.as<int>(), b->tiles.grad_chunk, s));
     HIP_TRY(tbx_launch_dedcn(&b->tab.sp, &b->tab.mt, b->dse_tmp.as<double>(), d_dsedcn, d_P,
                              d_dEdcn, s));
     HIP_TRY(mark(b, 5, s));
     return TBX_OK;
+}
+
+extern "C" int tbx_batch_gradient(tbx_batch *b, const double *d_xyz, const double *d_se,
+                                  const double *d_dsedcn, const double *d_P, const double *d_W,
+                                  const double *d_vao, const double *d_vdp, const double *d_vqp,
+                                  double *d_dEdcn, double *d_grad, double *d_sigma, void *stream)
+{
+    return batch_gradient(b, d_xyz, d_se, nullptr, nullptr, d_dsedcn, d_P, d_W, d_vao, d_vdp,
+                          d_vqp, d_dEdcn, d_grad, d_sigma, stream);
+}
+
+extern "C" int tbx_batch_gradient_with_forward(tbx_batch *b, const double *d_xyz,
+                                               const double *d_se, const double *d_S,
+                                               const double *d_D, const double *d_dsedcn,
+                                               const double *d_P, const double *d_W,
+                                               const double *d_vao, const double *d_vdp,
+                                               const double *d_vqp, double *d_dEdcn,
+                                               double *d_grad, double *d_sigma, void *stream)
+{
+    if (!d_S || !d_D) return fail(TBX_ERR_ARG, "null argument");
+    return batch_gradient(b, d_xyz, d_se, d_S, d_D, d_dsedcn, d_P, d_W, d_vao, d_vdp, d_vqp,
+                          d_dEdcn, d_grad, d_sigma, stream);
 }
 
 // tbx_batch_gradient with the dE/dse partial sums in the static `scratch`

@@ -6,7 +6,7 @@ seed 1004, near-first pair order, default tiles); each kernel is launched
 so that ``rocprofv3 --kernel-include-regex`` can pick them out:
 
     python -m tblite_amd.kernel_driver [--reps 3] [--nmol 2048]
-        [--kernels forward,adjoint,adjoint_fwd,gradient]
+        [--kernels forward,adjoint,adjoint_fwd,gradient,gradient_fwd]
 """
 from __future__ import annotations
 
@@ -44,7 +44,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--nmol", type=int, default=2048)
-    ap.add_argument("--kernels", default="forward,adjoint,adjoint_fwd,gradient")
+    ap.add_argument("--kernels", default="forward,adjoint,adjoint_fwd,gradient,gradient_fwd")
     a = ap.parse_args(argv)
     b, xyz, se, plain, adjoints = workload(a.nmol)
     outs = (torch.empty(b.ndense, dtype=torch.float64, device="cuda"),
@@ -56,7 +56,8 @@ def main(argv=None):
            # with the forward's S and D (the autograd backward); outs hold
            # them once the forward has run
            "adjoint_fwd": lambda: b.adjoint(xyz, se, *adjoints, S=outs[0], D=outs[1]),
-           "gradient": lambda: b.gradient(xyz, se, *plain)}
+           "gradient": lambda: b.gradient(xyz, se, *plain),
+           "gradient_fwd": lambda: b.gradient(xyz, se, *plain, S=outs[0], D=outs[1])}
     names = [k for k in a.kernels.split(",") if k]
     for k in names:
         run[k]()  # warm-up
@@ -67,7 +68,8 @@ def main(argv=None):
             torch.cuda.synchronize()
             t = b.last_timing()
             ms = {"forward": t["forward_ms"], "adjoint": t["adjoint_ms"],
-                  "adjoint_fwd": t["adjoint_ms"], "gradient": t["gradient_ms"]}[k]
+                  "adjoint_fwd": t["adjoint_ms"], "gradient": t["gradient_ms"],
+                  "gradient_fwd": t["gradient_ms"]}[k]
             print(f"{k} {ms:.3f} ms", flush=True)
     time.sleep(0.1)
     return 0

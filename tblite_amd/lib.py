@@ -110,6 +110,7 @@ _SIGS = {
     "tbx_batch_offsets": (ct.c_int, [_P, _P, _P, _P, _P]),
     "tbx_batch_forward": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "tbx_batch_gradient": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "tbx_batch_gradient_with_forward": (ct.c_int, [_P] * 15),
     "tbx_batch_gradient_noalloc": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                               _P]),
     "tbx_batch_adjoint": (ct.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

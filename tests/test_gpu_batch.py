@@ -296,8 +296,9 @@ def test_pair_order_hint_keeps_results(batch_case):
         # persistent kernel when the hint is stale)
         a2 = [x.clone() for x in b.adjoint(xyz, se, *adj, S=f[0], D=f[1])]
         g = [x.clone() for x in b.gradient(xyz, se, dse, P, W, vao, vdp, vqp)]
+        g2 = [x.clone() for x in b.gradient(xyz, se, dse, P, W, vao, vdp, vqp, S=f[0], D=f[1])]
         torch.cuda.synchronize()
-        return f + a + a2 + g
+        return f + a + a2 + g + g2
 
     # hints displaced by less and by more than half the skin (0.25 bohr):
     # the far units are skipped in the first case and walked in the second

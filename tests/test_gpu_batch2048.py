@@ -3,7 +3,8 @@ molecules of 8-256 atoms (log-uniform, H/C/N/O/F/S/Cl, seed 1004) that
 bench.py times, with the same near-first pair order.  Every molecule's
 forward (S, D, Q, H0), plain gradient (dE/dR, dEdcn, sigma) and adjoint
 (dE/dR, dE/dse for non-symmetric random adjoints of the four outputs) is
-compared with the oracle.
+compared with the oracle; the gradient and the adjoint both through the
+plain entry points and through the ones given the forward's S and D.
 
 Tolerances: integrals and H0 1e-10, gradients 1e-8 (absolute).  Device
 results are copied to the host one molecule at a time, so host memory holds
@@ -93,12 +94,17 @@ def test_plain_gradient_every_molecule(workload):
     vdp = rng.normal(scale=1e-3, size=3 * b.nat)
     vqp = rng.normal(scale=1e-3, size=6 * b.nat)
     dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
-    dEdcn, grad, sigma = b.gradient(b.tensor_xyz(), se, dse, dev(P), dev(W), dev(vao), dev(vdp),
-                                    dev(vqp))
+    xyz = b.tensor_xyz()
+    args = (dev(P), dev(W), dev(vao), dev(vdp), dev(vqp))
+    res = [[t.cpu().numpy() for t in b.gradient(xyz, se, dse, *args)]]
+    # tbx_batch_gradient_with_forward: the terms linear in S, D per element
+    S, D, Q, H = b.forward(xyz, se)
+    del Q, H
+    res.append([t.cpu().numpy() for t in b.gradient(xyz, se, dse, *args, S=S, D=D)])
+    del S, D, args
     torch.cuda.synchronize()
-    dEdcn, grad, sigma = dEdcn.cpu().numpy(), grad.cpu().numpy(), sigma.cpu().numpy()
     se_h, dse_h = se.cpu().numpy(), dse.cpu().numpy()
-    worst = 0.0
+    worst = [0.0, 0.0]
     for m, st in enumerate(sts):
         (a0, a1), (s0, s1), (o0, o1), (d0, d1) = _slices(b, m)
         n = o1 - o0
@@ -107,12 +113,14 @@ def test_plain_gradient_every_molecule(workload):
             vdp[3 * a0: 3 * a1].reshape(3, -1, 1, order="F"),
             vqp[6 * a0: 6 * a1].reshape(6, -1, 1, order="F"),
             P[d0:d1].reshape(n, n, 1, order="F"), W[d0:d1].reshape(n, n, 1, order="F"))
-        errs = (np.abs(grad[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
-                np.abs(dEdcn[a0:a1] - cr).max(),
-                np.abs(sigma[9 * m: 9 * m + 9].reshape(3, 3, order="F") - sr).max())
-        worst = max(worst, *errs)
-        assert max(errs) < TOL_GRAD, (m, errs)
-    _say(f"    plain gradient: {b.nmol} molecules, max |GPU - oracle| = {worst:.2e}")
+        for k, (dEdcn, grad, sigma) in enumerate(res):
+            errs = (np.abs(grad[3 * a0: 3 * a1].reshape(-1, 3).T - gr).max(),
+                    np.abs(dEdcn[a0:a1] - cr).max(),
+                    np.abs(sigma[9 * m: 9 * m + 9].reshape(3, 3, order="F") - sr).max())
+            worst[k] = max(worst[k], *errs)
+            assert max(errs) < TOL_GRAD, (m, k, errs)
+    _say(f"    plain gradient: {b.nmol} molecules, max |GPU - oracle| = {worst[0]:.2e}; "
+         f"with the forward's S, D: {worst[1]:.2e}")
 
 
 def test_adjoint_every_molecule(workload):
