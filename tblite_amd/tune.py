@@ -59,7 +59,7 @@ def tune(b, xyz, se, adjoints, plain=None, reps: int = 2, fwd_unit_tasks=FWD_UNI
             f += t["forward_ms"] / reps
             a += t["adjoint_ms"] / reps
             if plain is not None:
-                b.gradient(xyz, se, *plain)
+                b.gradient(xyz, se, *plain, S=S, D=D)
                 torch.cuda.synchronize()
                 g += b.last_timing()["gradient_ms"] / reps
         row = {"fwd_unit_tasks": tf, "grad_unit_tasks": tg, "chunk": chunk, "forward_ms": f,
