@@ -1028,6 +1028,66 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 }
 
 
+// The explicit pair-vector terms of one spherical element (mu on atom j, nu
+// on atom i; e1 = (mu, nu), e2 = (nu, mu)) from the forward's S and
+// i-centred D there.  Per Cartesian pair the work items would add
+// S_c ex_c + 2 Wj_c D_c and S_c hsw_c (task_setup's R_EX, R_WQ, R_HSW), with
+// weights that are the spherical weights X pulled back through the
+// spherical transform T; since sum_c (T^T X T)_c S_c = sum_e X_e (T S T^T)_e,
+// the same sum runs over the block's elements with the forward's spherical
+// S and D.  explicit_core takes the element's weights: hsum (of H0), db (the
+// j-centred dipole), wq (the j-centred quadrupole, trace map folded in), and
+// adds its share of the pair sum dE/dv to g and of dE/dse_i = dE/dse_j to ds.
+__device__ __forceinline__ void explicit_core(const GradArgs &A, long long e1, int i, int j,
+                                              const double v[3], double hs, double sesum,
+                                              double dsr, double hsum, const double db[3],
+                                              const double wq[6], double g[3], double &ds)
+{
+    const double s = A.Sf[e1];
+    const double d[3] = {A.Df[3 * e1], A.Df[3 * e1 + 1], A.Df[3 * e1 + 2]};
+    const double hd = sesum * hs * hsum * dsr;
+    const double X[3] = {db[0] + 2.0 * (wq[0] * v[0] + 0.5 * (wq[1] * v[1] + wq[3] * v[2])) + hd * v[0],
+                         db[1] + 2.0 * (wq[2] * v[1] + 0.5 * (wq[1] * v[0] + wq[4] * v[2])) + hd * v[1],
+                         db[2] + 2.0 * (wq[5] * v[2] + 0.5 * (wq[3] * v[0] + wq[4] * v[1])) + hd * v[2]};
+    g[0] += s * X[0] + 2.0 * (wq[0] * d[0] + 0.5 * (wq[1] * d[1] + wq[3] * d[2]));
+    g[1] += s * X[1] + 2.0 * (wq[2] * d[1] + 0.5 * (wq[1] * d[0] + wq[4] * d[2]));
+    g[2] += s * X[2] + 2.0 * (wq[5] * d[2] + 0.5 * (wq[3] * d[0] + wq[4] * d[1]));
+    ds += (j == i ? 0.5 : 1.0) * 0.5 * hs * hsum * s;
+}
+
+// explicit_core for an element whose weights are read here (shell pairs with
+// a d shell, where Cartesian and spherical elements are not one to one)
+template <int MODE>
+__device__ __forceinline__ void explicit_element(const GradArgs &A, long long e1, long long e2,
+                                                 int i, int j, const double v[3], double hs,
+                                                 double sesum, double dsr, double g[3],
+                                                 double &ds)
+{
+    double hsum, db[3], qb[6];
+    if (MODE == GRAD_PLAIN) {  // weights -P x (atom j's potentials), as in task_setup
+        const double p = A.P[e1];
+        hsum = 2.0 * p;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) db[k] = -p * A.vdp[3 * j + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qb[k] = -p * A.vqp[6 * j + k];
+    } else {
+        hsum = A.Hb[e1] + A.Hb[e2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) db[k] = A.Db[3 * e2 + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qb[k] = A.Qb[6 * e2 + k];
+    }
+    const double tq = 0.5 * (qb[0] + qb[2] + qb[5]);  // the trace-removal map
+    double wq[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wq[k] = 1.5 * qb[k];
+    wq[0] -= tq;
+    wq[2] -= tq;
+    wq[5] -= tq;
+    explicit_core(A, e1, i, j, v, hs, sesum, dsr, hsum, db, wq, g, ds);
+}
+
 // Task set-up: one Cartesian function pair (ki of shell ish on atom i, kj of
 // shell jsh on atom j) at lattice vector itr.  Folds the spherical weights of
 // the pair's block into Cartesian ones (transpose of the spherical map),
@@ -1036,7 +1096,8 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
 template <int MODE, bool EX>
 __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v[3], int ish,
                            int jsh, int li, int lj, int ki, int kj, int nao, long long dense0,
-                           int ao_m, double hs, double sesum, double dsr, double *rec, int ld)
+                           int ao_m, double hs, double sesum, double dsr, double *rec, int ld,
+                           double gx[3], double &dsx)
 {
     const int i = pr.i, j = pr.j;
     const double vx = v[0], vy = v[1], vz = v[2];
@@ -1126,7 +1187,17 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
     rec[(R_W + 3) * ld] = wDi[2] + wDj[2] + 2.0 * Wv[2];
 #pragma unroll
     for (int k = 0; k < 6; ++k) rec[(R_W + 4 + k) * ld] = wQi[k] + wQj[k];
-    if (!EX) return;  // the explicit terms come from explicit_element
+    if (!EX) {
+        // s/p shell pairs: the Cartesian pair is one spherical element, whose
+        // weights are the ones gathered above; its explicit terms (the lane
+        // owns it) go to gx, dsx.  Shell pairs with a d shell: explicit_element.
+        if (li <= 1 && lj <= 1) {
+            const long long e1 = dense0 + (long long)(nui0 + cart_m(li, ki, 0)) * nao + muj0 +
+                                 cart_m(lj, kj, 0);
+            explicit_core(A, e1, i, j, v, hs, sesum, dsr, wH, wDj, wQj, gx, dsx);
+        }
+        return;
+    }
     const double hd = hw * dsr;
     rec[(R_EX + 0) * ld] = wDj[0] + 2.0 * Wv[0] + hd * vx;
     rec[(R_EX + 1) * ld] = wDj[1] + 2.0 * Wv[1] + hd * vy;
@@ -1136,54 +1207,6 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
     rec[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
 }
 
-// The explicit pair-vector terms of one spherical element (mu on atom j, nu
-// on atom i; e1 = (mu, nu), e2 = (nu, mu)) from the forward's S and
-// i-centred D there.  Per Cartesian pair the work items would add
-// S_c ex_c + 2 Wj_c D_c and S_c hsw_c (task_setup's R_EX, R_WQ, R_HSW), with
-// weights that are the spherical weights X pulled back through the
-// spherical transform T; since sum_c (T^T X T)_c S_c = sum_e X_e (T S T^T)_e,
-// the same sum runs over the block's elements with the forward's spherical
-// S and D.  Adds the element's share of the pair sum dE/dv to g and of
-// dE/dse_i = dE/dse_j to ds.
-template <int MODE>
-__device__ __forceinline__ void explicit_element(const GradArgs &A, long long e1, long long e2,
-                                                 int i, int j, const double v[3], double hs,
-                                                 double sesum, double dsr, double g[3],
-                                                 double &ds)
-{
-    double hsum, db[3], qb[6];
-    if (MODE == GRAD_PLAIN) {  // weights -P x (atom j's potentials), as in task_setup
-        const double p = A.P[e1];
-        hsum = 2.0 * p;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) db[k] = -p * A.vdp[3 * j + k];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) qb[k] = -p * A.vqp[6 * j + k];
-    } else {
-        hsum = A.Hb[e1] + A.Hb[e2];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) db[k] = A.Db[3 * e2 + k];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) qb[k] = A.Qb[6 * e2 + k];
-    }
-    const double tq = 0.5 * (qb[0] + qb[2] + qb[5]);  // the trace-removal map
-    double wq[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) wq[k] = 1.5 * qb[k];
-    wq[0] -= tq;
-    wq[2] -= tq;
-    wq[5] -= tq;
-    const double s = A.Sf[e1];
-    const double d[3] = {A.Df[3 * e1], A.Df[3 * e1 + 1], A.Df[3 * e1 + 2]};
-    const double hd = sesum * hs * hsum * dsr;
-    const double X[3] = {db[0] + 2.0 * (wq[0] * v[0] + 0.5 * (wq[1] * v[1] + wq[3] * v[2])) + hd * v[0],
-                         db[1] + 2.0 * (wq[2] * v[1] + 0.5 * (wq[1] * v[0] + wq[4] * v[2])) + hd * v[1],
-                         db[2] + 2.0 * (wq[5] * v[2] + 0.5 * (wq[3] * v[0] + wq[4] * v[1])) + hd * v[2]};
-    g[0] += s * X[0] + 2.0 * (wq[0] * d[0] + 0.5 * (wq[1] * d[1] + wq[3] * d[2]));
-    g[1] += s * X[1] + 2.0 * (wq[2] * d[1] + 0.5 * (wq[1] * d[0] + wq[4] * d[2]));
-    g[2] += s * X[2] + 2.0 * (wq[5] * d[2] + 0.5 * (wq[3] * d[0] + wq[4] * d[1]));
-    ds += (j == i ? 0.5 : 1.0) * 0.5 * hs * hsum * s;
-}
 
 
 // Primitive pairs [p0, p1) of one task of type (LI, LJ): accumulate the
@@ -1556,9 +1579,10 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
             const double v[3] = {U.pv[e.x][0], U.pv[e.x][1], U.pv[e.x][2]};
             const int li = d.x / 3, lj = d.x % 3;
             task_setup<MODE, EX>(A, pr, v, e.z, e.y, li, lj, d.y, d.z, nao, dense0, ao_m, hs,
-                                 sesum, dsr, rec + tid, GTHREADS);
-            // the spherical element (ki, kj) of the shell pair is this lane's
-            if (!EX && d.y < 2 * li + 1 && d.z < 2 * lj + 1) {
+                                 sesum, dsr, rec + tid, GTHREADS, gx, dsx);
+            // with a d shell, the spherical element (ki, kj) of the shell pair
+            // is this lane's
+            if (!EX && (li == 2 || lj == 2) && d.y < 2 * li + 1 && d.z < 2 * lj + 1) {
                 const int mu = pr.aoj + ao_first(pr.dj, e.y) + d.z;
                 const int nu = pr.aoi + ao_first(pr.di, e.z) + d.y;
                 explicit_element<MODE>(A, dense0 + (long long)nu * nao + mu,
@@ -1746,7 +1770,7 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
 template <int MODE, bool EX>
 __global__ void __launch_bounds__(GTHREADS, EX ? 2 : 3) k_gradient(GradArgs A)
 {
-    gradient_unit<MODE, EX>(A, blockIdx.x);
+    gradient_unit<MODE, EX>(A, xcd_unit(blockIdx.x, gridDim.x));  // XCD-contiguous
 }
 
 // The far gradient units (no pair within cutoff + skin at the pair-order
