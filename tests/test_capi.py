@@ -162,7 +162,24 @@ def test_scf_consumers_reject_null_batch():
     assert so.tbx_batch_last_scf_timing(None, buf) == TBX_ERR_ARG
 
 
-_SHELLS = {1: (0,), 6: (0, 1), 7: (0, 1), 8: (0, 1), 9: (0, 1), 16: (0, 1, 2), 17: (0, 1, 2)}
+def test_batch_entry_points_reject_null():
+    """The batched gradient and adjoint entry points, with and without the
+    forward's S and D, refuse a null batch or a null S / D before any device
+    work."""
+    so = L.lib()
+    buf = (ct.c_double * 8)()
+    assert so.tbx_batch_adjoint(None, *([buf] * 8), None) == TBX_ERR_ARG
+    assert so.tbx_batch_adjoint_with_forward(None, *([buf] * 10), None) == TBX_ERR_ARG
+    assert so.tbx_batch_adjoint_with_forward(None, buf, buf, None, buf, *([buf] * 6),
+                                             None) == TBX_ERR_ARG
+    assert b"null" in so.tbx_last_error()
+    assert so.tbx_batch_gradient(None, *([buf] * 11), None) == TBX_ERR_ARG
+    assert so.tbx_batch_gradient_with_forward(None, *([buf] * 13), None) == TBX_ERR_ARG
+    assert so.tbx_batch_gradient_with_forward(None, buf, buf, buf, None, *([buf] * 9),
+                                              None) == TBX_ERR_ARG
+
+
+_SHELLS ={1: (0,), 6: (0, 1), 7: (0, 1), 8: (0, 1), 9: (0, 1), 16: (0, 1, 2), 17: (0, 1, 2)}
 
 
 def _expected_plan_totals(mols):
