@@ -1622,8 +1622,16 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
             // segment words, first task of each shell pair from the shell-pair list
             const int4 *seg = reinterpret_cast<const int4 *>(v.sinfo);
             const TbxPair *pinf = reinterpret_cast<const TbxPair *>(v.pairs);
-            std::vector<int> off(2 * v.nseg + 1);
+            std::vector<int> off(2 * v.nseg + 1), roff(v.nseg + 1);
             tbx_elem_offsets(seg, v.nseg, off.data());
+            // the row map the kernel reads the pair from (every row carries an
+            // element, so it has at most 2 * TBX_UNIT_TASKS entries)
+            tbx_row_offsets(seg, v.nseg, roff.data());
+            std::vector<unsigned char> rmap(2 * TBX_UNIT_TASKS, 255);
+            if (roff[v.nseg] > 2 * TBX_UNIT_TASKS) ++s.bad_units;
+            else
+                for (int q = 0; q < v.npair; ++q)
+                    tbx_row_map_pair(seg, roff.data(), pinf, v.pext, v.npair, q, rmap.data());
             std::vector<int> spmap((size_t)16 * v.npair, -1);
             for (int q = 0; q < v.nspair; ++q) {
                 const int e = v.spl[q], ty = e >> 16, li = ty / 3, lj = ty % 3;
@@ -1638,7 +1646,13 @@ extern "C" int tbx_batch_plan_stats(const tbx_batch_desc *desc, const tbx_tiles 
                 const TbxElem el = tbx_elem_locate(off.data(), v.nseg, seg, pinf, v.pext, v.npair, e);
                 const int p = el.p, nul = el.nul, muj = el.muj;
                 const bool mirror = el.mirror != 0;
-                if (!block_ok || p < 0 || p >= v.npair) {
+                bool map_ok = roff[v.nseg] <= 2 * TBX_UNIT_TASKS;
+                if (map_ok) {
+                    const TbxElem em = tbx_elem_locate_map(off.data(), v.nseg, seg, pinf,
+                                                           roff.data(), rmap.data(), e);
+                    map_ok = em.p == p && em.nul == nul && em.muj == muj && em.mirror == el.mirror;
+                }
+                if (!block_ok || !map_ok || p < 0 || p >= v.npair) {
                     ++s.bad_elems;
                     continue;
                 }

@@ -822,6 +822,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     __shared__ int tinfo[TBX_UNIT_TASKS];    // type | ki << 4 | kj << 8 | shell pair << 12 | pair << 20
     __shared__ __align__(16) int coff[TBX_UNIT_TASKS + 1];  // chunk offsets, then element offsets
     __shared__ unsigned char spmap[TBX_UNIT_PAIRS * 16];     // first task of (pair, ish, jsh)
+    __shared__ unsigned char rowpair[2 * TBX_UNIT_TASKS];    // pair of each block-A row
     const TbxSpecies &sp = A.sp;
     const TbxMols &mt = A.mol;
     const int tid = threadIdx.x;
@@ -931,12 +932,18 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_forward(FwdArgs A)
     // the mirror (rows of atom i, columns of the j < i); every element of the
     // molecule's matrices is written by exactly one lane of one unit
     __syncthreads();  // the items are done with coff
-    if (tid == 0) tbx_elem_offsets(U.sinfo, nseg, coff);
+    int *roff = coff + 2 * nseg + 1;  // row offsets after the element offsets
+    if (tid == 0) {
+        tbx_elem_offsets(U.sinfo, nseg, coff);
+        tbx_row_offsets(U.sinfo, nseg, roff);
+    }
+    __syncthreads();
+    if (tid < npair) tbx_row_map_pair(U.sinfo, roff, U.pinf, U.pext, npair, tid, rowpair);
     __syncthreads();
     for (int base = 0; base < nel; base += NTHREADS) {
         const int e = base + tid;
         if (e < nel) {
-            const TbxElem el = tbx_elem_locate(coff, nseg, U.sinfo, U.pinf, U.pext, npair, e);
+            const TbxElem el = tbx_elem_locate_map(coff, nseg, U.sinfo, U.pinf, roff, rowpair, e);
             const int p = el.p, nul = el.nul, muj = el.muj;
             const bool mirror = el.mirror != 0;
             const PairInfo pr = U.pinf[p];

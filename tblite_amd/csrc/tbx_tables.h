@@ -172,6 +172,65 @@ __host__ __device__ inline TbxElem tbx_elem_locate(const int *off, int nseg, con
     return r;
 }
 
+// The pair search of tbx_elem_locate as a table, one byte per block-A row
+// of the unit (at most 2 * TBX_UNIT_TASKS, since every row carries at least
+// one element): roff[k] = the first entry of segment k (nseg + 1 of them),
+// map[roff[k] + r] = the last pair of segment k whose first AO of atom j is
+// at most the row's AO.  tbx_row_map_pair writes pair p's rows, from its
+// first AO of atom j up to the next pair's of the segment (one pair per lane).
+__host__ __device__ inline void tbx_row_offsets(const int4 *seg, int nseg, int *roff)
+{
+    int n = 0;
+    for (int k = 0; k < nseg; ++k) {
+        roff[k] = n;
+        n += seg[k].z & 0xffff;
+    }
+    roff[nseg] = n;
+}
+
+__host__ __device__ inline void tbx_row_map_pair(const int4 *seg, const int *roff,
+                                                 const TbxPair *pinf, const int *pext, int npair,
+                                                 int p, unsigned char *map)
+{
+    const int k = pext[p] & 31;
+    const int4 g = seg[k];
+    const int rows = g.z & 0xffff;
+    const int r1 = (p + 1 < npair && (pext[p + 1] & 31) == k) ? pinf[p + 1].aoj - g.y : rows;
+    for (int r = pinf[p].aoj - g.y; r < r1; ++r)
+        if (r >= 0 && r < rows) map[roff[k] + r] = (unsigned char)p;
+}
+
+// tbx_elem_locate with the pair read from the row map
+__host__ __device__ inline TbxElem tbx_elem_locate_map(const int *off, int nseg, const int4 *seg,
+                                                      const TbxPair *pinf, const int *roff,
+                                                      const unsigned char *map, int e)
+{
+    int lo = 0, hi = 2 * nseg;  // last half-segment whose first element is <= e
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int k = lo >> 1;
+    const int4 g = seg[k];
+    const int local = e - off[lo];
+    TbxElem r;
+    r.mirror = lo & 1;
+    int jao;  // molecule-local AO on the atom-j side
+    if (!r.mirror) {
+        const int rows = g.z & 0xffff;
+        const int c = local / rows;
+        r.nul = c;
+        jao = g.y + (local - c * rows);
+    } else {
+        const int c = local / g.w;
+        r.nul = local - c * g.w;
+        jao = g.y + c;
+    }
+    r.p = map[roff[k] + (jao - g.y)];
+    r.muj = jao - pinf[r.p].aoj;
+    return r;
+}
+
 // Molecule / atom tables of a batch (device pointers).  "local" indices
 // count from the start of the atom's molecule.
 struct TbxMols {
