@@ -24,6 +24,18 @@
 #include "../../include/tbx.h"
 #include "tbx_tables.h"
 
+// Row atoms per band of the unit builders (build_units_list): gradient and
+// forward units.  Bands of 4 rows shorten the gradient's reads of the
+// mirrored blocks (2048-molecule batch: adjoint given S, D 5.78 -> 5.66 ms;
+// bands of 2 and 8: 5.67 and 5.76); the forward is fastest with single rows
+// (bands of 2, 4, 8: 6.43, 6.47, 6.54 ms)
+#ifndef TBX_GRAD_BAND
+#define TBX_GRAD_BAND 4
+#endif
+#ifndef TBX_FWD_BAND
+#define TBX_FWD_BAND 1
+#endif
+
 extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
                                          const double *xyz, const double *se, double *S, double *D,
                                          double *Q, double *H, long long ndense, const int *nnl,
@@ -415,9 +427,10 @@ struct HostMols {
 // is still in exactly one unit and the kernels still screen every pair, so
 // the hint changes the speed, never the results.
 static void build_units_list(const HostSpecies &hs, const HostMols &hm, bool diag,
-                             int max_tasks, int max_pairs, std::vector<TbxSeg> &useg,
+                             int max_tasks, int max_pairs, int band, std::vector<TbxSeg> &useg,
                              std::vector<int> &unit_seg, std::vector<char> *unit_far = nullptr)
 {
+    band = std::max(band, 1);
     const bool hinted = hm.hint_xyz.size() == 3 * (size_t)hm.nat;
     auto near = [&](int i, int j, int itr) {
         if (!hinted) return true;
@@ -441,37 +454,51 @@ static void build_units_list(const HostSpecies &hs, const HostMols &hm, bool dia
         tasks = pairs = segs = 0;
         open_seg = false;
     };
+    // the open segment of each row of the band in the current unit (-1: none)
+    std::vector<int> seg_of(band > 0 ? band : 1, -1);
+    auto close_all = [&]() {
+        close_unit();
+        std::fill(seg_of.begin(), seg_of.end(), -1);
+    };
     for (int m = 0; m < hm.nmol; ++m) {
         const int a0 = hm.mol_atom[m], a1 = hm.mol_atom[m + 1];
         const int2 tr = hm.mol_tr[m];
         for (int pass = 0; pass < (hinted ? 2 : 1); ++pass) {
             cur_pass = pass;
             for (int itr = tr.x; itr < tr.x + tr.y; ++itr)
-                for (int i = a0; i < a1; ++i) {
+                // bands of `band` row atoms, walked column by column: a unit's
+                // blocks then span a few rows i as well as a run of columns j,
+                // so the mirrored blocks (rows of the atoms i) are longer runs
+                for (int i0 = a0; i0 < a1; i0 += band) {
+                    const int i1 = std::min(i0 + band, a1);
                     // with translations the self images (i, i, T) are pairs too
-                    const int jend = (diag || tr.y > 1) ? i + 1 : i;
-                    const int nci = hs.ncart[hm.at_sp[i]];
-                    open_seg = false;
-                    for (int j = a0; j < jend; ++j) {
-                        if (hinted && near(i, j, itr) != (pass == 0)) {
-                            open_seg = false;  // a segment is a run of consecutive j
-                            continue;
+                    const int jlast = (diag || tr.y > 1) ? i1 : i1 - 1;
+                    std::fill(seg_of.begin(), seg_of.end(), -1);
+                    for (int j = a0; j < jlast; ++j) {
+                        const int ncj = hs.ncart[hm.at_sp[j]];
+                        for (int i = i0; i < i1; ++i) {
+                            int &sg = seg_of[i - i0];
+                            const int jend = (diag || tr.y > 1) ? i + 1 : i;
+                            if (j >= jend || (hinted && near(i, j, itr) != (pass == 0))) {
+                                sg = -1;  // a segment is a run of consecutive j
+                                continue;
+                            }
+                            const int c = hs.ncart[hm.at_sp[i]] * ncj;
+                            if (tasks + c > max_tasks || pairs == max_pairs ||
+                                (sg < 0 && segs == TBX_UNIT_SEGS))
+                                close_all();
+                            if (sg < 0) {
+                                sg = (int)useg.size();
+                                useg.push_back(TbxSeg{i, j, j, itr});
+                                ++segs;
+                            }
+                            useg[sg].j1 = j + 1;
+                            tasks += c;
+                            ++pairs;
                         }
-                        const int c = nci * hs.ncart[hm.at_sp[j]];
-                        if (tasks + c > max_tasks || pairs == max_pairs ||
-                            (!open_seg && segs == TBX_UNIT_SEGS))
-                            close_unit();
-                        if (!open_seg) {
-                            useg.push_back(TbxSeg{i, j, j, itr});
-                            ++segs;
-                            open_seg = true;
-                        }
-                        useg.back().j1 = j + 1;
-                        tasks += c;
-                        ++pairs;
                     }
                 }
-            close_unit();  // units never span molecules, nor near and far pairs
+            close_all();  // units never span molecules, nor near and far pairs
         }
     }
 }
@@ -592,10 +619,10 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
 static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
 {
     std::vector<char> far, ffar;
-    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, hm.useg, hm.unit_seg,
-                     &far);
-    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, hm.fseg, hm.funit_seg,
-                     &ffar);
+    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, TBX_GRAD_BAND, hm.useg,
+                     hm.unit_seg, &far);
+    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, TBX_FWD_BAND, hm.fseg,
+                     hm.funit_seg, &ffar);
     if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, TBX_GRAD_UNIT_TASKS,
                      TBX_GRAD_UNIT_PAIRS, hm.uhead, hm.uplan) ||
         !build_plans(hs, hm, hm.fseg, hm.funit_seg, true, TBX_UNIT_TASKS, TBX_UNIT_PAIRS,
