@@ -390,8 +390,10 @@ int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pai
  * *_unit_tasks Cartesian-pair tasks (>= the largest atom pair; <= 256 for
  * the forward, <= 128 for the gradient) and *_unit_pairs atom pairs (<= 128
  * and <= 64); kept primitive pairs are processed in chunks of *_chunk
- * (1..64) per work item.  Defaults: forward 256, 128, chunk 3; gradient
- * 128, 64, chunk 1.
+ * (1..64) per work item.  The pairs go into units in bands of *_band
+ * consecutive row atoms (1..16), walked column by column, so that a unit
+ * covers a few rows i and a run of columns j.  Defaults: forward 256, 128,
+ * chunk 3, band 1; gradient 128, 64, chunk 1, band 4.
  * tbx_batch_set_tiles rebuilds the units (host) and waits for the whole
  * device (every stream) before it replaces them, so no launch still reads
  * the old units; tblite_amd/tune.py chooses them by timing the kernels.  A
@@ -400,6 +402,7 @@ int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pai
 typedef struct {
     int fwd_unit_tasks, fwd_unit_pairs, fwd_chunk;
     int grad_unit_tasks, grad_unit_pairs, grad_chunk;
+    int fwd_band, grad_band;
 } tbx_tiles;
 
 int tbx_batch_set_tiles(tbx_batch *b, const tbx_tiles *tiles);

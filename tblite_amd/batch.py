@@ -69,6 +69,11 @@ class _Description:
                                 ct.pointer(self.h0), self.cutoff)
 
 
+# the library's default bands (tbx_tiles' fwd_band, grad_band), for tile
+# dictionaries that leave them out
+DEFAULT_BANDS = {"fwd_band": 1, "grad_band": 4}
+
+
 def plan_stats(molecules, tiles: dict | None = None, hint: bool = False,
                skin: float = 1.0) -> dict:
     """Work-unit plans of a batch of ``(numbers, xyz_bohr)``, built and checked
@@ -80,6 +85,8 @@ def plan_stats(molecules, tiles: dict | None = None, hint: bool = False,
     and count the gradient units with a pair inside the cutoff there
     (``nunit_grad_live``)."""
     d = _Description(molecules)
+    if tiles is not None:
+        tiles = {**DEFAULT_BANDS, **tiles}
     t = None if tiles is None else ct.pointer(L.Tiles(**tiles))
     xyz = None
     if hint:

@@ -24,17 +24,14 @@
 #include "../../include/tbx.h"
 #include "tbx_tables.h"
 
-// Row atoms per band of the unit builders (build_units_list): gradient and
-// forward units.  Bands of 4 rows shorten the gradient's reads of the
+// Default row atoms per band of the unit builders (build_units_list, the
+// tiles' *_band).  Bands of 4 rows shorten the gradient's reads of the
 // mirrored blocks (2048-molecule batch: adjoint given S, D 5.78 -> 5.66 ms;
 // bands of 2 and 8: 5.67 and 5.76); the forward is fastest with single rows
 // (bands of 2, 4, 8: 6.43, 6.47, 6.54 ms)
-#ifndef TBX_GRAD_BAND
 #define TBX_GRAD_BAND 4
-#endif
-#ifndef TBX_FWD_BAND
 #define TBX_FWD_BAND 1
-#endif
+#define TBX_MAX_BAND 16
 
 extern "C" hipError_t tbx_launch_forward(const TbxSpecies *sp, const TbxMols *mol,
                                          const double *xyz, const double *se, double *S, double *D,
@@ -619,9 +616,9 @@ static bool build_plans(const HostSpecies &hs, const HostMols &hm, const std::ve
 static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
 {
     std::vector<char> far, ffar;
-    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, TBX_GRAD_BAND, hm.useg,
+    build_units_list(hs, hm, false, t.grad_unit_tasks, t.grad_unit_pairs, t.grad_band, hm.useg,
                      hm.unit_seg, &far);
-    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, TBX_FWD_BAND, hm.fseg,
+    build_units_list(hs, hm, true, t.fwd_unit_tasks, t.fwd_unit_pairs, t.fwd_band, hm.fseg,
                      hm.funit_seg, &ffar);
     if (!build_plans(hs, hm, hm.useg, hm.unit_seg, false, TBX_GRAD_UNIT_TASKS,
                      TBX_GRAD_UNIT_PAIRS, hm.uhead, hm.uplan) ||
@@ -647,8 +644,9 @@ static int build_units(const HostSpecies &hs, HostMols &hm, const tbx_tiles &t)
 }
 
 // defaults: the autotuner's choice on the 2048-molecule batch
-static const tbx_tiles kDefaultTiles = {TBX_UNIT_TASKS, TBX_UNIT_PAIRS, 3,
-                                        TBX_GRAD_UNIT_TASKS, TBX_GRAD_UNIT_PAIRS, 1};
+static const tbx_tiles kDefaultTiles = {TBX_UNIT_TASKS,      TBX_UNIT_PAIRS,      3,
+                                        TBX_GRAD_UNIT_TASKS, TBX_GRAD_UNIT_PAIRS, 1,
+                                        TBX_FWD_BAND,        TBX_GRAD_BAND};
 
 // Tile shapes must fit the kernels' LDS arrays and hold the largest atom pair.
 static int check_tiles(const HostSpecies &hs, const tbx_tiles *t)
@@ -668,6 +666,9 @@ static int check_tiles(const HostSpecies &hs, const tbx_tiles *t)
         if (chunk[k] < 1 || chunk[k] > 64)
             return fail(TBX_ERR_ARG, "chunk %d outside 1..64", chunk[k]);
     }
+    for (int band : {t->fwd_band, t->grad_band})
+        if (band < 1 || band > TBX_MAX_BAND)
+            return fail(TBX_ERR_ARG, "band %d outside 1..%d", band, TBX_MAX_BAND);
     return TBX_OK;
 }
 

@@ -63,7 +63,7 @@ _P = ct.c_void_p
 class Tiles(ct.Structure):
     _fields_ = [("fwd_unit_tasks", ct.c_int), ("fwd_unit_pairs", ct.c_int), ("fwd_chunk", ct.c_int),
                 ("grad_unit_tasks", ct.c_int), ("grad_unit_pairs", ct.c_int),
-                ("grad_chunk", ct.c_int)]
+                ("grad_chunk", ct.c_int), ("fwd_band", ct.c_int), ("grad_band", ct.c_int)]
 
 
 class PlanStats(ct.Structure):

@@ -241,8 +241,9 @@ def test_graph_replay_matches_eager(batch_case):
 
 
 def test_tile_shapes_keep_results(batch_case):
-    """Other work-unit caps and chunk lengths (the autotuner's search space)
-    give the same integrals and adjoint gradient; invalid shapes are refused."""
+    """Other work-unit caps, chunk lengths and bands (the autotuner's search
+    space) give the same integrals and adjoint gradient; invalid shapes are
+    refused."""
     mols, b, cn, se, dse = batch_case
     _say("[gpu] tile shapes")
     rng = np.random.default_rng(5)
@@ -255,10 +256,10 @@ def test_tile_shapes_keep_results(batch_case):
         assert torch.abs(r - g).max().item() < 1e-10
     default = b.tiles()
     try:
-        for tasks, gtasks, chunk in [(128, 128, 4), (192, 112, 12), (256, 128, 1),
-                                     (256, 104, 16)]:
+        for tasks, gtasks, chunk, fb, gb in [(128, 128, 4, 1, 4), (192, 112, 12, 2, 1),
+                                             (256, 128, 1, 6, 16), (256, 104, 16, 1, 3)]:
             b.set_tiles(fwd_unit_tasks=tasks, fwd_chunk=chunk, grad_unit_tasks=gtasks,
-                        grad_chunk=chunk)
+                        grad_chunk=chunk, fwd_band=fb, grad_band=gb)
             for r, g in zip(ref_f, b.forward(xyz, se)):
                 assert torch.abs(r - g).max().item() < 1e-12, (tasks, chunk)
             for r, g in zip(ref_a, b.adjoint(xyz, se, *adj)):
@@ -271,6 +272,8 @@ def test_tile_shapes_keep_results(batch_case):
             b.set_tiles(grad_unit_tasks=256)
         with pytest.raises(L.TbxError):
             b.set_tiles(grad_chunk=0)
+        with pytest.raises(L.TbxError):
+            b.set_tiles(grad_band=0)
     finally:
         b.set_tiles(**default)
 
