@@ -316,3 +316,45 @@ def test_pair_order_hint_keeps_results(batch_case):
                 assert torch.abs(r - g).max().item() < 1e-10, k
     finally:
         b.order_pairs(b.xyz0)
+
+
+def test_autotuner_choice_and_cache(batch_case, tmp_path):
+    """tblite_amd.tune over a reduced search space (caps, chunks, then bands):
+    the chosen tiles come from that space, every candidate is in the table,
+    the chosen tiles leave the forward and the adjoint unchanged, and a
+    second call with the same cache file applies the stored tiles without
+    timing anything."""
+    from tblite_amd import tune
+    mols, b, cn, se, dse = batch_case
+    _say("[gpu] autotuner")
+    rng = np.random.default_rng(8)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+    adj = [t(rng.normal(size=k * b.ndense)) for k in (1, 3, 6, 1)]
+    P, W, vao, vdp, vqp = (t(a) for a in _random_inputs(b, 9))
+    xyz = b.tensor_xyz()
+    default = b.tiles()
+    ref_f = [x.clone() for x in b.forward(xyz, se)]
+    ref_a = [x.clone() for x in b.adjoint(xyz, se, *adj, S=ref_f[0], D=ref_f[1])]
+    cache = tmp_path / "tiles.json"
+    space = dict(fwd_unit_tasks=(192, 256), grad_unit_tasks=(112, 128), chunks=(1, 3),
+                 fwd_bands=(1, 2), grad_bands=(2, 4))
+    try:
+        res = tune.tune(b, xyz, se, adj, (dse, P, W, vao, vdp, vqp), reps=1, cache=cache,
+                        **space)
+        tl = res["tiles"]
+        assert tl["fwd_unit_tasks"] in (192, 256) and tl["grad_unit_tasks"] in (112, 128)
+        assert tl["fwd_chunk"] in (1, 3) and tl["grad_chunk"] in (1, 3)
+        assert tl["fwd_band"] in (1, 2) and tl["grad_band"] in (2, 4)
+        assert len(res["table"]) == 2 * 2 + 2  # caps x chunks, then the bands
+        assert b.tiles() == tl
+        S, D, Q, H = b.forward(xyz, se)
+        for r, g in zip(ref_f, (S, D, Q, H)):
+            assert torch.abs(r - g).max().item() < 1e-12
+        for r, g in zip(ref_a, b.adjoint(xyz, se, *adj, S=S, D=D)):
+            assert torch.abs(r - g).max().item() < 1e-10
+        # the cached choice comes back without a sweep
+        b.set_tiles(**default)
+        again = tune.tune(b, xyz, se, adj, None, reps=1, cache=cache, **space)
+        assert again["tiles"] == tl and b.tiles() == tl
+    finally:
+        b.set_tiles(**default)
