@@ -118,14 +118,56 @@ def copy_probe(gib: float = 1.0, reps: int = 3, device: int = 0):
     return 8 * n, 8 * n, 16 * n / (ms * 1e6)
 
 
+def peaks_probe(gib: float = 8.0, n_gemm: int = 8192, reps: int = 5, device: int = 0) -> dict:
+    """What this GPU reaches on plain work, beside its published peaks: the
+    library's streaming copy (read + write GB/s), a write-only fill of the
+    same size (torch's zero_, GB/s) and an FP64 DGEMM of n_gemm^3 through
+    torch.matmul (rocBLAS / hipBLASLt on FP64 MFMA, TFLOP/s)."""
+    import torch
+
+    dev = torch.device("cuda", device)
+    _, _, copy_gbs = copy_probe(gib, reps, device)
+    buf = torch.empty(int(gib * (1 << 30)) // 8, dtype=torch.float64, device=dev)
+    buf.zero_()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    t0.record()
+    for _ in range(reps):
+        buf.zero_()
+    t1.record()
+    torch.cuda.synchronize(dev)
+    fill_gbs = buf.numel() * 8 / (t0.elapsed_time(t1) / reps * 1e6)
+    del buf
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    a = torch.rand(n_gemm, n_gemm, dtype=torch.float64, device=dev, generator=g)
+    b = torch.rand(n_gemm, n_gemm, dtype=torch.float64, device=dev, generator=g)
+    c = torch.matmul(a, b)
+    torch.cuda.synchronize(dev)
+    t0.record()
+    for _ in range(reps):
+        c = torch.matmul(a, b)
+    t1.record()
+    torch.cuda.synchronize(dev)
+    tflops = 2.0 * n_gemm ** 3 / (t0.elapsed_time(t1) / reps * 1e-3) / 1e12
+    return {"copy_GBs": copy_gbs, "fill_GBs": fill_gbs, "dgemm_fp64_TFLOPs": tflops,
+            "gib": gib, "n_gemm": n_gemm, "reps": reps,
+            "published": {"hbm_GBs": PEAK_HBM_GBS, "fp64_TFLOPs": PEAK_FP64_TFLOPS}}
+
+
 if __name__ == "__main__":
     import argparse
     import json
 
-    ap = argparse.ArgumentParser(description="HBM traffic calibration probe")
+    ap = argparse.ArgumentParser(description="HBM traffic calibration probe; --peaks: the "
+                                 "copy, fill and FP64 DGEMM rates this GPU reaches")
     ap.add_argument("--gib", type=float, default=1.0)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--peaks", action="store_true")
     a = ap.parse_args()
-    rd, wr, gbs = copy_probe(a.gib, a.reps)
-    print(json.dumps({"probe": "k_copy", "bytes_read": rd, "bytes_written": wr,
-                      "launches": a.reps + 1, "GBs": gbs}))
+    if a.peaks:
+        print(json.dumps(peaks_probe(max(a.gib, 8.0))))
+    else:
+        rd, wr, gbs = copy_probe(a.gib, a.reps)
+        print(json.dumps({"probe": "k_copy", "bytes_read": rd, "bytes_written": wr,
+                          "launches": a.reps + 1, "GBs": gbs}))
