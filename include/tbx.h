@@ -387,13 +387,12 @@ int tbx_batch_last_timing(const tbx_batch *b, double *ms_forward, double *ms_pai
 
 /* Tile shapes of the batched kernels, no counterpart in the reference: the
  * half list of every molecule is cut into work units of at most
- * *_unit_tasks Cartesian-pair tasks (>= the largest atom pair; <= 256 for
- * the forward, <= 128 for the gradient) and *_unit_pairs atom pairs (<= 128
- * and <= 64); kept primitive pairs are processed in chunks of *_chunk
+ * *_unit_tasks Cartesian-pair tasks (>= the largest atom pair; <= 256) and
+ * *_unit_pairs atom pairs (<= 128); kept primitive pairs are processed in chunks of *_chunk
  * (1..64) per work item.  The pairs go into units in bands of *_band
  * consecutive row atoms (1..16), walked column by column, so that a unit
  * covers a few rows i and a run of columns j.  Defaults: forward 256, 128,
- * chunk 3, band 1; gradient 128, 64, chunk 1, band 4.
+ * chunk 3, band 1; gradient 256, 128, chunk 1, band 4.
  * tbx_batch_set_tiles rebuilds the units (host) and waits for the whole
  * device (every stream) before it replaces them, so no launch still reads
  * the old units; tblite_amd/tune.py chooses them by timing the kernels.  A

@@ -1712,12 +1712,14 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
     }
     __syncthreads();
 
-    // (5) one lane per pair (npair <= 64: the first wavefront).  The pair's
-    // sums go to atom j (HBM atomics).  The atom-i sums of each segment (a
-    // run of consecutive pairs with one row atom i) are reduced by a
-    // segmented scan across the wavefront and added by the segment's last
-    // lane; the unit's sigma is a wavefront sum added by nine lanes.
-    if (tid < 64) {
+    // (5) one lane per pair (npair <= TBX_GRAD_UNIT_PAIRS: the first two
+    // wavefronts).  The pair's sums go to atom j (HBM atomics).  The atom-i
+    // sums of each segment (a run of consecutive pairs with one row atom i)
+    // are reduced by a segmented scan across the wavefront and added by the
+    // segment's last lane in the wavefront (a segment across the wavefront
+    // boundary adds its two parts); each wavefront's share of the unit's
+    // sigma is a wavefront sum added by nine lanes.
+    if (tid < TBX_GRAD_UNIT_PAIRS) {
         const bool on = tid < npair;
         const int key = on ? (U.pext[tid] & 31) : 64 + tid;  // segment; unique on idle lanes
         double gi[3 + MAXSH], sg[6];
@@ -1756,24 +1758,25 @@ __device__ __forceinline__ void gradient_unit(const GradArgs &A, int unit)
 #pragma unroll
         for (int q = 0; q < 6; ++q) sg[q] = wave_sum_d(sg[q]);
         const int knext = __shfl_down(key, 1, 64);
-        if (on && (tid == npair - 1 || knext != key)) {  // last lane of its segment
+        if (on && (tid == npair - 1 || knext != key || (tid & 63) == 63)) {  // segment's last lane
             for (int q = 0; q < 3; ++q)
                 if (gi[q] != 0.0) unsafeAtomicAdd(&A.grad[3 * pr.i + q], gi[q]);
             for (int q = 0; q < (pr.di & 7); ++q)
                 if (gi[3 + q] != 0.0) unsafeAtomicAdd(&A.dse[pr.shi + q], gi[3 + q]);
         }
-        if (tid < 9) {
+        if ((tid & 63) < 9) {
             double v = 0.0;
 #pragma unroll
-            for (int k9 = 0; k9 < 9; ++k9) v = (tid == k9) ? sg[kSigPack[k9]] : v;
-            if (v != 0.0) unsafeAtomicAdd(&A.sigma[9 * m + tid], v);
+            for (int k9 = 0; k9 < 9; ++k9) v = ((tid & 63) == k9) ? sg[kSigPack[k9]] : v;
+            if (v != 0.0) unsafeAtomicAdd(&A.sigma[9 * m + (tid & 63)], v);
         }
     }
 }
 
 // EX: the explicit pair-vector terms in the work items (the forward's S and
 // D not given, or lattice images).  Without them the kernel fits three
-// wavefronts per SIMD: 168 VGPRs and under 27 KB of LDS per workgroup.
+// wavefronts per SIMD: 168 VGPRs and under 53 KB of LDS per 256-thread
+// workgroup (3 workgroups per CU).
 template <int MODE, bool EX>
 __global__ void __launch_bounds__(GTHREADS, EX ? 2 : 3) k_gradient(GradArgs A)
 {

@@ -61,16 +61,16 @@ struct TbxSpecies {
     double cutoff2;
 };
 
-// Work units: consecutive (i, j, image) atom pairs of one molecule, in
-// segments j in [j0, j1) of one row atom i at lattice vector itr, packed so
+// Work units: (i, j, image) atom pairs of one molecule, in segments j in
+// [j0, j1) of one row atom i at lattice vector itr, packed so
 // that a unit holds at most TBX_UNIT_TASKS Cartesian-pair tasks and
 // TBX_UNIT_PAIRS atom pairs (forward units; TBX_GRAD_UNIT_* for the gradient)
 // and TBX_UNIT_SEGS segments.  One workgroup runs one unit.
 #define TBX_UNIT_TASKS 256       /* forward units (and workgroup size) */
 #define TBX_UNIT_PAIRS 128
 #define TBX_UNIT_SEGS 32
-#define TBX_GRAD_UNIT_TASKS 128  /* gradient units (and workgroup size) */
-#define TBX_GRAD_UNIT_PAIRS 64
+#define TBX_GRAD_UNIT_TASKS 256  /* gradient units (and workgroup size) */
+#define TBX_GRAD_UNIT_PAIRS 128
 struct TbxSeg {
     int i, j0, j1, itr;
 };

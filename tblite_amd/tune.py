@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 FWD_UNIT_TASKS = (192, 256)   # forward workgroups hold up to 256 tasks
-GRAD_UNIT_TASKS = (112, 128)  # gradient workgroups up to 128
+GRAD_UNIT_TASKS = (192, 256)  # gradient workgroups hold up to 256 tasks
 CHUNKS = (1, 2, 3, 4, 6)
 FWD_BANDS = (1, 2)
 GRAD_BANDS = (2, 4, 8)

@@ -208,7 +208,7 @@ def _expected_plan_totals(mols):
     dict(fwd_unit_tasks=192, fwd_unit_pairs=7, fwd_chunk=2,
          grad_unit_tasks=112, grad_unit_pairs=13, grad_chunk=4),
     dict(fwd_unit_tasks=256, fwd_unit_pairs=128, fwd_chunk=3,  # bands of rows
-         grad_unit_tasks=128, grad_unit_pairs=64, grad_chunk=1, fwd_band=5, grad_band=16),
+         grad_unit_tasks=256, grad_unit_pairs=128, grad_chunk=1, fwd_band=5, grad_band=16),
     dict(fwd_unit_tasks=100, fwd_unit_pairs=1, fwd_chunk=1,
          grad_unit_tasks=100, grad_unit_pairs=1, grad_chunk=1, fwd_band=3, grad_band=2),
 ])
@@ -240,13 +240,13 @@ def test_unit_plans_cover_every_pair_and_element_once(tiles, hint):
 
 
 def test_unit_plans_reject_tiles_beyond_the_kernels():
-    """Gradient units beyond 128 tasks, or below the largest atom pair (an S-S
+    """Gradient units beyond 256 tasks, or below the largest atom pair (an S-S
     pair has (1 + 3 + 6)^2 = 100 Cartesian-pair tasks), are argument errors."""
     from tblite_amd.batch import plan_stats
     mols = [(np.array([16, 16]), np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 3.9]]))]
     base = dict(fwd_unit_tasks=256, fwd_unit_pairs=128, fwd_chunk=3,
-                grad_unit_tasks=128, grad_unit_pairs=64, grad_chunk=1)
-    for bad in (dict(grad_unit_tasks=256), dict(grad_unit_tasks=99), dict(fwd_unit_tasks=99),
+                grad_unit_tasks=256, grad_unit_pairs=128, grad_chunk=1)
+    for bad in (dict(grad_unit_tasks=257), dict(grad_unit_tasks=99), dict(fwd_unit_tasks=99),
                 dict(fwd_unit_pairs=129), dict(grad_chunk=0), dict(fwd_band=0),
                 dict(grad_band=17)):
         with pytest.raises(L.TbxError) as e:

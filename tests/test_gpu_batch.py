@@ -269,7 +269,7 @@ def test_tile_shapes_keep_results(batch_case):
         with pytest.raises(L.TbxError):
             b.set_tiles(fwd_unit_tasks=512)
         with pytest.raises(L.TbxError):
-            b.set_tiles(grad_unit_tasks=256)
+            b.set_tiles(grad_unit_tasks=257)
         with pytest.raises(L.TbxError):
             b.set_tiles(grad_chunk=0)
         with pytest.raises(L.TbxError):
@@ -336,13 +336,13 @@ def test_autotuner_choice_and_cache(batch_case, tmp_path):
     ref_f = [x.clone() for x in b.forward(xyz, se)]
     ref_a = [x.clone() for x in b.adjoint(xyz, se, *adj, S=ref_f[0], D=ref_f[1])]
     cache = tmp_path / "tiles.json"
-    space = dict(fwd_unit_tasks=(192, 256), grad_unit_tasks=(112, 128), chunks=(1, 3),
+    space = dict(fwd_unit_tasks=(192, 256), grad_unit_tasks=(128, 256), chunks=(1, 3),
                  fwd_bands=(1, 2), grad_bands=(2, 4))
     try:
         res = tune.tune(b, xyz, se, adj, (dse, P, W, vao, vdp, vqp), reps=1, cache=cache,
                         **space)
         tl = res["tiles"]
-        assert tl["fwd_unit_tasks"] in (192, 256) and tl["grad_unit_tasks"] in (112, 128)
+        assert tl["fwd_unit_tasks"] in (192, 256) and tl["grad_unit_tasks"] in (128, 256)
         assert tl["fwd_chunk"] in (1, 3) and tl["grad_chunk"] in (1, 3)
         assert tl["fwd_band"] in (1, 2) and tl["grad_band"] in (2, 4)
         assert len(res["table"]) == 2 * 2 + 2  # caps x chunks, then the bands
