@@ -160,7 +160,10 @@ __device__ __forceinline__ double mom(const Mom &M, int t)
     }
 }
 
-// integral of sum_k P[k] y^k (y + c)^m, m = 0, 1, 2; N coefficients
+// integral of sum_k P[k] y^k (y + c)^m, m = 0, 1, 2; N coefficients.  Only
+// the even moments are nonzero, so each k takes the terms of its parity
+// alone (a product with a literal 0.0 is not folded under IEEE rules and
+// would still be issued).
 template <int N>
 __device__ __forceinline__ void mom3(const double *P, double c, const Mom &M, double &i0,
                                      double &i1, double &i2)
@@ -168,10 +171,16 @@ __device__ __forceinline__ void mom3(const double *P, double c, const Mom &M, do
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        double mk = mom(M, k), mk1 = mom(M, k + 1), mk2 = mom(M, k + 2);
-        a0 += P[k] * mk;
-        a1 += P[k] * (mk1 + c * mk);
-        a2 += P[k] * (mk2 + 2.0 * c * mk1 + c * c * mk);
+        if ((k & 1) == 0) {  // y^k: moments k (m = 0, 1, 2) and k + 2 (m = 2)
+            const double mk = mom(M, k), mk2 = mom(M, k + 2);
+            a0 += P[k] * mk;
+            a1 += P[k] * (c * mk);
+            a2 += P[k] * (mk2 + c * c * mk);
+        } else {             // moment k + 1 (m = 1, 2)
+            const double mk1 = mom(M, k + 1);
+            a1 += P[k] * mk1;
+            a2 += P[k] * (2.0 * c * mk1);
+        }
     }
     i0 = a0;
     i1 = a1;
