@@ -632,14 +632,20 @@ __device__ __forceinline__ void fchunk_run(const double *__restrict__ pp, const 
         double V[3][3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) fwd1d_t<LI, LJ>(ei[k], ej[k], -vec[k] * ajo, vec[k] * aio, M, V[k]);
-        const double yz = V[1][0] * V[2][0];
-        o[0] += cc * (V[0][0] * yz);
-        o[1] += cc * (V[0][1] * yz);
-        o[2] += cc * (V[0][0] * V[1][1] * V[2][0]);
-        o[3] += cc * (V[0][0] * V[1][0] * V[2][1]);
-#pragma unroll
-        for (int q = 0; q < 6; ++q)
-            o[4 + q] += cc * (V[0][kQm[q][0]] * V[1][kQm[q][1]] * V[2][kQm[q][2]]);
+        // the x factors scaled by cc and the six y-z products, each formed once
+        const double cx0 = cc * V[0][0], cx1 = cc * V[0][1], cx2 = cc * V[0][2];
+        const double yz00 = V[1][0] * V[2][0], yz10 = V[1][1] * V[2][0], yz01 = V[1][0] * V[2][1];
+        const double yz20 = V[1][2] * V[2][0], yz11 = V[1][1] * V[2][1], yz02 = V[1][0] * V[2][2];
+        o[0] += cx0 * yz00;
+        o[1] += cx1 * yz00;
+        o[2] += cx0 * yz10;
+        o[3] += cx0 * yz01;
+        o[4] += cx2 * yz00;  // quadrupole xx xy yy xz yz zz (kQm)
+        o[5] += cx1 * yz10;
+        o[6] += cx0 * yz20;
+        o[7] += cx1 * yz01;
+        o[8] += cx0 * yz11;
+        o[9] += cx0 * yz02;
     }
 }
 
