@@ -300,6 +300,26 @@ __device__ __forceinline__ double contract_moments(const double *fx, const doubl
     return fx[0] * t0 + fx[1] * t1 + fx[2] * t2;
 }
 
+// The three derivative contractions of one Cartesian pair: c[k] is
+// contract_moments with the derivative factors G in direction k and the
+// moment factors V in the other two.  c[1] and c[2] share the x factors V[0],
+// so the weights are folded with them once: u_ab = sum over the x moments of
+// w V0, the weight of fy[a] fz[b].
+__device__ __forceinline__ void contract_grad(const double G[3][3], const double V[3][3],
+                                              const double w[10], double c[3])
+{
+    const double *wD = w + 1, *wQ = w + 4;  // D: x y z; Q: xx xy yy xz yz zz
+    c[0] = contract_moments(G[0], V[1], V[2], w[0], wD, wQ);
+    const double u00 = w[0] * V[0][0] + wD[0] * V[0][1] + wQ[0] * V[0][2];
+    const double u10 = wD[1] * V[0][0] + wQ[1] * V[0][1];
+    const double u01 = wD[2] * V[0][0] + wQ[3] * V[0][1];
+    const double u20 = wQ[2] * V[0][0], u11 = wQ[4] * V[0][0], u02 = wQ[5] * V[0][0];
+    c[1] = V[2][0] * (u00 * G[1][0] + u10 * G[1][1] + u20 * G[1][2]) +
+           V[2][1] * (u01 * G[1][0] + u11 * G[1][1]) + V[2][2] * (u02 * G[1][0]);
+    c[2] = G[2][0] * (u00 * V[1][0] + u10 * V[1][1] + u20 * V[1][2]) +
+           G[2][1] * (u01 * V[1][0] + u11 * V[1][1]) + G[2][2] * (u02 * V[1][0]);
+}
+
 // packed symmetric index (xx, xy, yy, xz, yz, zz) of the 3x3 element a + 3b
 static constexpr int kSigPack[9] = {0, 1, 3, 1, 2, 4, 3, 4, 5};
 
@@ -1268,9 +1288,11 @@ __device__ __forceinline__ void chunk_run(const double *__restrict__ pp, const d
             dacc[1] += cc * (V[0][0] * V[1][1] * V[2][0]);
             dacc[2] += cc * (V[0][0] * V[1][0] * V[2][1]);
         }
-        a3[0] += cc * contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
-        a3[1] += cc * contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
-        a3[2] += cc * contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
+        double c[3];
+        contract_grad(G, V, w, c);
+        a3[0] += cc * c[0];
+        a3[1] += cc * c[1];
+        a3[2] += cc * c[2];
     }
 }
 
@@ -1362,9 +1384,9 @@ __device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, con
                 double w[10];
 #pragma unroll
                 for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
-                const double c0 = contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
-                const double c1 = contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
-                const double c2 = contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
+                double cg[3];
+                contract_grad(G, V, w, cg);
+                const double c0 = cg[0], c1 = cg[1], c2 = cg[2];
                 if (EX) {
                     double ex[3];
                     explicit_terms(V, r, ld, cc, ex, ds);
@@ -1431,9 +1453,9 @@ __device__ __forceinline__ void pprow_run(const double *__restrict__ pp, const d
             double w[10];
 #pragma unroll
             for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
-            const double c0 = contract_moments(G[0], V[1], V[2], w[0], w + 1, w + 4);
-            const double c1 = contract_moments(V[0], G[1], V[2], w[0], w + 1, w + 4);
-            const double c2 = contract_moments(V[0], V[1], G[2], w[0], w + 1, w + 4);
+            double cg[3];
+            contract_grad(G, V, w, cg);
+            const double c0 = cg[0], c1 = cg[1], c2 = cg[2];
             if (EX) {
                 double ex[3];
                 explicit_terms(V, r, ld, cc, ex, ds);
