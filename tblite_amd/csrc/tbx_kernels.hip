@@ -1135,6 +1135,29 @@ __device__ __forceinline__ void explicit_element(const GradArgs &A, long long e1
 // the pair's block into Cartesian ones (transpose of the spherical map),
 // shifts the j-centred multipole weights to centre i and stores the record.
 // Returns the number of primitive pairs kept by the screening.
+// Cyclic axis frames of the p-p row items (pprow_run): the frame of row k
+// has axis s along (k + s) mod 3.  rot3 picks x_k; the vector and the packed
+// symmetric tensor (xx xy yy xz yz zz) in that frame are rot_vec, rot_sym.
+__device__ __forceinline__ double rot3(double x0, double x1, double x2, int k)
+{
+    return k == 0 ? x0 : (k == 1 ? x1 : x2);
+}
+__device__ __forceinline__ void rot_vec(const double x[3], int k, double y[3])
+{
+    y[0] = rot3(x[0], x[1], x[2], k);
+    y[1] = rot3(x[1], x[2], x[0], k);
+    y[2] = rot3(x[2], x[0], x[1], k);
+}
+__device__ __forceinline__ void rot_sym(const double q[6], int k, double y[6])
+{
+    y[0] = rot3(q[0], q[2], q[5], k);
+    y[1] = rot3(q[1], q[4], q[3], k);
+    y[2] = rot3(q[2], q[5], q[0], k);
+    y[3] = rot3(q[3], q[1], q[4], k);
+    y[4] = rot3(q[4], q[3], q[1], k);
+    y[5] = rot3(q[5], q[0], q[2], k);
+}
+
 template <int MODE, bool EX>
 __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v[3], int ish,
                            int jsh, int li, int lj, int ki, int kj, int nao, long long dense0,
@@ -1222,13 +1245,26 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
     // H0 = S * (se_i + se_j)/2 * hscale * shpoly(R): hs = hscale * shpoly and
     // dsr = (d shpoly / d r) / (r shpoly) come from the shell-pair phase
     const double hw = sesum * hs * wH;
-    rec[(R_W + 0) * ld] = wS + hw + wDj[0] * vx + wDj[1] * vy + wDj[2] * vz + Wv[0] * vx +
-                          Wv[1] * vy + Wv[2] * vz;
-    rec[(R_W + 1) * ld] = wDi[0] + wDj[0] + 2.0 * Wv[0];
-    rec[(R_W + 2) * ld] = wDi[1] + wDj[1] + 2.0 * Wv[1];
-    rec[(R_W + 3) * ld] = wDi[2] + wDj[2] + 2.0 * Wv[2];
+    // p-p shell pairs run by rows (pprow_run) in the frame of the row: the
+    // record of Cartesian pair (ki, kj) goes to slot (kj - ki) mod 3 of its
+    // row, with its vector and tensor weights in that frame
+    const int fr = (li == 1 && lj == 1) ? ki : 0;
+    double *rw = (li == 1 && lj == 1) ? rec + ((kj - ki + 3) % 3 - kj) : rec;
+    rw[(R_W + 0) * ld] = wS + hw + wDj[0] * vx + wDj[1] * vy + wDj[2] * vz + Wv[0] * vx +
+                         Wv[1] * vy + Wv[2] * vz;
+    {
+        const double wd[3] = {wDi[0] + wDj[0] + 2.0 * Wv[0], wDi[1] + wDj[1] + 2.0 * Wv[1],
+                              wDi[2] + wDj[2] + 2.0 * Wv[2]};
+        double wq[6], y[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) rec[(R_W + 4 + k) * ld] = wQi[k] + wQj[k];
+        for (int k = 0; k < 6; ++k) wq[k] = wQi[k] + wQj[k];
+        rot_vec(wd, fr, y);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rw[(R_W + 1 + k) * ld] = y[k];
+        rot_sym(wq, fr, y);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) rw[(R_W + 4 + k) * ld] = y[k];
+    }
     if (!EX) {
         // s/p shell pairs: the Cartesian pair is one spherical element, whose
         // weights are the ones gathered above; its explicit terms (the lane
@@ -1241,12 +1277,18 @@ __device__ void task_setup(const GradArgs &A, const PairInfo &pr, const double v
         return;
     }
     const double hd = hw * dsr;
-    rec[(R_EX + 0) * ld] = wDj[0] + 2.0 * Wv[0] + hd * vx;
-    rec[(R_EX + 1) * ld] = wDj[1] + 2.0 * Wv[1] + hd * vy;
-    rec[(R_EX + 2) * ld] = wDj[2] + 2.0 * Wv[2] + hd * vz;
+    {
+        const double ex[3] = {wDj[0] + 2.0 * Wv[0] + hd * vx, wDj[1] + 2.0 * Wv[1] + hd * vy,
+                              wDj[2] + 2.0 * Wv[2] + hd * vz};
+        double y[6];
+        rot_vec(ex, fr, y);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) rec[(R_WQ + k) * ld] = wQj[k];
-    rec[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
+        for (int k = 0; k < 3; ++k) rw[(R_EX + k) * ld] = y[k];
+        rot_sym(wQj, fr, y);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) rw[(R_WQ + k) * ld] = y[k];
+    }
+    rw[R_HSW * ld] = (j == i ? 0.5 : 1.0) * 0.5 * hs * wH;
 }
 
 
@@ -1402,73 +1444,82 @@ __device__ __forceinline__ void shellpair_run(const double *__restrict__ pp, con
     }
 }
 
-// Primitive pairs [p0, p1) of one row ki = row of a p-p shell pair: as
-// shellpair_run, with the row a run-time value so that the rows of a
-// wavefront share one code path (the exponent of direction k is k == row).
+// Primitive pairs [p0, p1) of one row ki = row of a p-p shell pair, in the
+// frame of the row (axis s along (row + s) mod 3, rot_vec): there the bra's
+// exponent is 1 on axis 0 and 0 on the others, and the ket's is 1 on axis t
+// for the Cartesian pair in slot t of the row's records (task_setup writes
+// them in this frame), so every exponent is a compile-time constant.  The
+// derivatives are rotated back into g at the end.
 template <bool EX>
 __device__ __forceinline__ void pprow_run(const double *__restrict__ pp, const double *rec,
                                           int ld, const double vec[3], int row, int p0, int p1,
                                           double g[3], double &ds)
 {
     constexpr int NA = 5, NB = 2;
+    double pv[3];
+    rot_vec(vec, row, pv);
     const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    double gs[3] = {0.0, 0.0, 0.0};
     for (int p = p0; p < p1; ++p) {
         const double *e = pp + TBX_PP_N * p;
         const double mu = e[TBX_PP_MU], ajo = e[TBX_PP_AJO], aio = e[TBX_PP_AIO];
         const double h = e[TBX_PP_H], ai = e[TBX_PP_AI], pre = e[TBX_PP_PRE];
         const double cc = exp(-(mu * r2)) * pre;
         const double ta = 2.0 * ai;
-        // per direction: moments m = 0..2 and derivatives for exponent a = 0
-        // or 1 of the bra and b = 0 or 1 of the ket, the bra's chosen by row
-        double V0[3][2][3], G0[3][2][3];  // [direction][b][m], bra exponent 0 / 1 merged
+        // per axis: moments m = 0..2 and derivatives for the ket exponent b
+        double V0[3][2][3], G0[3][2][3];  // [axis][b][m]
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             double O[NA + NB - 1][NB];
-            os_table<NA, NB>(-vec[k] * ajo, vec[k] * aio, h, O);
-            const bool up = (k == row);
+            os_table<NA, NB>(-pv[k] * ajo, pv[k] * aio, h, O);
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    const double v0 = O[m][b], v1 = O[m + 1][b];
-                    const double g0 = (m > 0) ? ta * O[m + 1][b] - (double)m * O[m - 1][b]
+                    if (k == 0) {  // bra exponent 1
+                        V0[k][b][m] = O[m + 1][b];
+                        G0[k][b][m] = ta * O[m + 2][b] - (double)(m + 1) * O[m][b];
+                    } else {       // bra exponent 0
+                        V0[k][b][m] = O[m][b];
+                        G0[k][b][m] = (m > 0) ? ta * O[m + 1][b] - (double)m * O[m - 1][b]
                                               : ta * O[1][b];
-                    const double g1 = ta * O[m + 2][b] - (double)(m + 1) * O[m][b];
-                    V0[k][b][m] = up ? v1 : v0;
-                    G0[k][b][m] = up ? g1 : g0;
+                    }
                 }
         }
 #pragma unroll
-        for (int kj = 0; kj < 3; ++kj) {
+        for (int t = 0; t < 3; ++t) {
             __builtin_amdgcn_sched_barrier(0);
-            const double *r = rec + kj;
+            const double *r = rec + t;
             double V[3][3], G[3][3];
 #pragma unroll
             for (int k = 0; k < 3; ++k)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
-                    V[k][m] = V0[k][k == kj ? 1 : 0][m];
-                    G[k][m] = G0[k][k == kj ? 1 : 0][m];
+                    V[k][m] = V0[k][k == t ? 1 : 0][m];
+                    G[k][m] = G0[k][k == t ? 1 : 0][m];
                 }
             double w[10];
 #pragma unroll
             for (int q = 0; q < 10; ++q) w[q] = r[(R_W + q) * ld];
             double cg[3];
             contract_grad(G, V, w, cg);
-            const double c0 = cg[0], c1 = cg[1], c2 = cg[2];
             if (EX) {
                 double ex[3];
                 explicit_terms(V, r, ld, cc, ex, ds);
-                g[0] += cc * (c0 + ex[0]);
-                g[1] += cc * (c1 + ex[1]);
-                g[2] += cc * (c2 + ex[2]);
+                gs[0] += cc * (cg[0] + ex[0]);
+                gs[1] += cc * (cg[1] + ex[1]);
+                gs[2] += cc * (cg[2] + ex[2]);
             } else {
-                g[0] += cc * c0;
-                g[1] += cc * c1;
-                g[2] += cc * c2;
+                gs[0] += cc * cg[0];
+                gs[1] += cc * cg[1];
+                gs[2] += cc * cg[2];
             }
         }
     }
+    // axis s of the frame is (row + s) mod 3
+    g[0] += rot3(gs[0], gs[2], gs[1], row);
+    g[1] += rot3(gs[1], gs[0], gs[2], row);
+    g[2] += rot3(gs[2], gs[1], gs[0], row);
 }
 
 // One step of a segmented inclusive scan of (g, ds) over lanes with equal
