@@ -6,7 +6,7 @@ seed 1004, near-first pair order, default tiles); each kernel is launched
 so that ``rocprofv3 --kernel-include-regex`` can pick them out:
 
     python -m tblite_amd.kernel_driver [--reps 3] [--nmol 2048]
-        [--kernels forward,adjoint,adjoint_fwd,gradient,gradient_fwd]
+        [--kernels forward,adjoint,adjoint_fwd,gradient,gradient_fwd,density]
 """
 from __future__ import annotations
 
@@ -59,11 +59,27 @@ def main(argv=None):
            "gradient": lambda: b.gradient(xyz, se, *plain),
            "gradient_fwd": lambda: b.gradient(xyz, se, *plain, S=outs[0], D=outs[1])}
     names = [k for k in a.kernels.split(",") if k]
+    if "density" in names:
+        # P = C diag(f) C^T on random coefficients (packed per molecule)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(12)
+        coeff = torch.randn(b.ndense, generator=g, dtype=torch.float64, device="cuda")
+        focc = torch.rand(b.nao, generator=g, dtype=torch.float64, device="cuda") * 2.0
+        pd = torch.empty(b.ndense, dtype=torch.float64, device="cuda")
+        run["density"] = lambda: b.density_matrix(coeff, focc, out=pd)
     for k in names:
         run[k]()  # warm-up
     torch.cuda.synchronize()
     for k in names:
         for _ in range(a.reps):
+            if k == "density":  # one launch on torch's current stream
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                run[k]()
+                e1.record()
+                torch.cuda.synchronize()
+                print(f"{k} {e0.elapsed_time(e1):.3f} ms", flush=True)
+                continue
             run[k]()
             torch.cuda.synchronize()
             t = b.last_timing()
