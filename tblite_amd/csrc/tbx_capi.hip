@@ -404,7 +404,7 @@ struct HostMols {
     std::vector<int2> mol_tr;
     long long ndense = 0;
     std::vector<int> mol_atom, mol_nao, mol_cart, mol_ao, at_sp, at_mol, at_sh, at_ao, at_cart,
-        cart_atom, ao_atom, sh_off, unit_seg, funit_seg, uplan, fplan, sh_aog, tiles;
+        cart_atom, ao_atom, sh_off, unit_seg, funit_seg, uplan, fplan, sh_aog, tiles, dtiles;
     std::vector<long long> mol_dense;
     std::vector<TbxSeg> useg, fseg;
     std::vector<TbxUnitHead> uhead, fhead;
@@ -728,6 +728,9 @@ static int build_mols(const HostSpecies &hs, int nmol, const int *mol_nat, const
         const int t = (hm.mol_nao[m] + 31) / 32;
         for (int bj = 0; bj < t; ++bj)
             for (int bi = 0; bi <= bj; ++bi) hm.tiles.insert(hm.tiles.end(), {m, bi, bj, 0});
+        const int td = (hm.mol_nao[m] + 63) / 64;
+        for (int bj = 0; bj < td; ++bj)
+            for (int bi = 0; bi <= bj; ++bi) hm.dtiles.insert(hm.dtiles.end(), {m, bi, bj, 0});
     }
     hm.nsh = sh;
     hm.nao = aog;
@@ -837,7 +840,7 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
                  o_at_cart = a.add(hm.at_cart), o_cart_atom = a.add(hm.cart_atom),
                  o_ao_atom = a.add(hm.ao_atom), o_mol_ao = a.add(hm.mol_ao),
                  o_uhead = a.add(hm.uhead), o_uplan = a.add(hm.uplan), o_fhead = a.add(hm.fhead),
-                 o_fplan = a.add(hm.fplan), o_sh_aog = a.add(hm.sh_aog), o_tiles = a.add(hm.tiles),
+                 o_fplan = a.add(hm.fplan), o_sh_aog = a.add(hm.sh_aog), o_tiles = a.add(hm.tiles), o_dtiles = a.add(hm.dtiles),
                  o_hint = a.add(hm.hint_xyz);
     const hipError_t e = upload_arena(d.mols, a, s);
     if (e != hipSuccess) return e;
@@ -877,6 +880,8 @@ static hipError_t upload_mols(DevTables &d, const HostMols &hm, hipStream_t s)
     mt.sh_aog = reinterpret_cast<const int *>(at(o_sh_aog));
     mt.ntile = (int)(hm.tiles.size() / 4);
     mt.tiles = reinterpret_cast<const int *>(at(o_tiles));
+    mt.ndtile = (int)(hm.dtiles.size() / 4);
+    mt.dtiles = reinterpret_cast<const int *>(at(o_dtiles));
     // the far gradient units may be skipped while no atom moved more than
     // (a little less than) half the skin away from the hint
     const bool hinted = hm.hint_xyz.size() == 3 * (size_t)hm.nat && hm.nat > 0;

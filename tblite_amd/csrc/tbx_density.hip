@@ -6,13 +6,16 @@
 //   get_fermi_filling  (wavefunction/fermi.f90:31-117), one spin channel
 //
 // The products are the dense contractions of this path, on FP64 MFMA
-// (v_mfma_f64_16x16x4_f64): one wavefront computes a 32 x 32 tile of one
-// molecule's P (the upper tile pairs bi <= bj of the batch's tile list) as
-// 2 x 2 blocks of 16 x 16, four k per instruction, and writes the tile and
-// its mirror.  Operands come straight from memory (C is read column-wise, 16
-// consecutive rows per k; the tiles of a molecule run on one XCD, so C is
-// fetched from HBM about once per molecule and re-read from L2); zero
-// padding covers rows and k beyond nao.
+// (v_mfma_f64_16x16x4_f64): one workgroup computes a 64 x 64 tile pair
+// (bi <= bj) of one molecule's P from the batch's list of such tiles, each of
+// its four wavefronts a 32 x 32 quarter as 2 x 2 blocks of 16 x 16, four k per
+// instruction, and writes the tile and its mirror.  The operands are panels of
+// C (64 rows by 16 columns for each factor) staged in LDS by the whole
+// workgroup, the next panel loaded while the products run on the current one,
+// so each element of C is fetched once per tile pair instead of once per
+// wavefront; the tiles of a molecule run on one XCD, so C is fetched from HBM
+// about once per molecule and re-read from L2.  Zero padding covers rows and
+// k beyond nao.
 #include <hip/hip_runtime.h>
 
 #include "tbx_tables.h"
@@ -24,46 +27,74 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // v_mfma_f64_16x16x4_f64 operand layout on gfx950 (checked on the device):
 // lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; its accumulator
 // register r holds D[l / 16 + 4 r][l % 16], r = 0..3.
+constexpr int KB = 16;       // k per panel
+constexpr int PR = 64 + 16;  // panel row length in LDS (doubles; the pad spreads the banks)
+
+// One 64 x 64 tile pair (bi <= bj) of one molecule's P per workgroup: the
+// four wavefronts compute its 32 x 32 quarters from panels of C staged in LDS
+// (64 rows x KB columns for each factor, the left one scaled by f_k), each
+// panel loaded by the whole workgroup while the products run on the previous one.
 __global__ void __launch_bounds__(256) k_density(TbxMols mt, const double *__restrict__ C,
                                                  const double *__restrict__ focc,
                                                  const double *__restrict__ eval,
                                                  double *__restrict__ P)
 {
-    // four consecutive tiles per workgroup, workgroups XCD-contiguous: the tiles
-    // of a molecule read its C through one L2
-    const int w = tbx_xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (w >= mt.ntile) return;  // whole wavefronts leave together
-    const int m = mt.tiles[4 * w], bi = mt.tiles[4 * w + 1], bj = mt.tiles[4 * w + 2];
+    __shared__ double As[KB][PR], Bs[KB][PR];
+    const int w = tbx_xcd_block(blockIdx.x, gridDim.x);
+    if (w >= mt.ndtile) return;  // the whole workgroup leaves together
+    const int m = mt.dtiles[4 * w], bi = mt.dtiles[4 * w + 1], bj = mt.dtiles[4 * w + 2];
     const int n = mt.mol_nao[m];
     const long long d0 = mt.mol_dense[m];
     const int ao0 = mt.mol_ao[m];
     const double *Cm = C + d0;
-    const int r0 = 32 * bi, c0 = 32 * bj;
+    const int R0 = 64 * bi, C0 = 64 * bj;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
+    const int sr = 32 * (wv >> 1), sc = 32 * (wv & 1);  // the wavefront's quarter
+    const int kk = tid >> 4, rr = tid & 15;             // this thread's panel column and rows
+    double pa[4], pb[4];
+    auto load = [&](int k0) {
+        const int k = k0 + kk;
+        const bool kin = k < n;
+        double f = 0.0;
+        if (kin) f = eval ? focc[ao0 + k] * eval[ao0 + k] : focc[ao0 + k];
+        const double *col = Cm + (long long)k * n;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = R0 + rr + 16 * q, j = C0 + rr + 16 * q;
+            pa[q] = (kin && i < n) ? col[i] * f : 0.0;
+            pb[q] = (kin && j < n) ? col[j] : 0.0;
+        }
+    };
     dbl4 acc[2][2];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int u = 0; u < 2; ++u) acc[s][u] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += 4) {
-        const int k = k0 + lk;
-        const bool kin = k < n;
-        double f = 0.0;
-        if (kin) f = eval ? focc[ao0 + k] * eval[ao0 + k] : focc[ao0 + k];
-        const double *col = Cm + (long long)k * n;
-        double a[2], b[2];
+    load(0);
+    for (int k0 = 0; k0 < n; k0 += KB) {
+        __syncthreads();  // the products of the previous panel are done
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int i = r0 + 16 * s + li, j = c0 + 16 * s + li;
-            a[s] = (kin && i < n) ? col[i] * f : 0.0;
-            b[s] = (kin && j < n) ? col[j] : 0.0;
+        for (int q = 0; q < 4; ++q) {
+            As[kk][rr + 16 * q] = pa[q];
+            Bs[kk][rr + 16 * q] = pb[q];
         }
+        __syncthreads();
+        if (k0 + KB < n) load(k0 + KB);
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+        for (int ks = 0; ks < KB; ks += 4) {
+            double a[2], b[2];
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
-                acc[s][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[u], acc[s][u], 0, 0, 0);
+            for (int s = 0; s < 2; ++s) {
+                a[s] = As[ks + lk][sr + 16 * s + li];
+                b[s] = Bs[ks + lk][sc + 16 * s + li];
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+                    acc[s][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[u], acc[s][u], 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -71,7 +102,7 @@ __global__ void __launch_bounds__(256) k_density(TbxMols mt, const double *__res
         for (int u = 0; u < 2; ++u)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = r0 + 16 * s + lk + 4 * r, j = c0 + 16 * u + li;
+                const int i = R0 + sr + 16 * s + lk + 4 * r, j = C0 + sc + 16 * u + li;
                 if (i < n && j < n) {
                     const double v = acc[s][u][r];
                     P[d0 + (long long)j * n + i] = v;
@@ -144,9 +175,9 @@ __global__ void __launch_bounds__(256) k_fermi(TbxMols mt, const double *__restr
 extern "C" hipError_t tbx_launch_density(const TbxMols *mol, const double *C, const double *focc,
                                          const double *eval, double *P, hipStream_t stream)
 {
-    if (mol->ntile <= 0) return hipSuccess;
+    if (mol->ndtile <= 0) return hipSuccess;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_density, dim3((mol->ntile + 3) / 4), dim3(256), 0, stream, *mol, C, focc,
+    hipLaunchKernelGGL(k_density, dim3(mol->ndtile), dim3(256), 0, stream, *mol, C, focc,
                        eval, P);
     return hipGetLastError();
 }

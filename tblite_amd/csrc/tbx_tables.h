@@ -267,6 +267,8 @@ struct TbxMols {
     const int *sh_aog;             // [nsh_tot+1] first AO (global) of every shell
     int ntile;
     const int *tiles;              // [4*ntile] (mol, bi, bj, 0): 32 x 32 tile pairs bi <= bj
+    int ndtile;
+    const int *dtiles;             // [4*ndtile] the same for 64 x 64 (k_density)
                                    // of the dense blocks
 };
 
